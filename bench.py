@@ -1,0 +1,242 @@
+"""Benchmark: batched Y = M X over a RePair-compressed genotype matrix on one
+MI355X, against the plain per-level sweep and the reference's CPU sweep.
+
+Workload (defaults): the 500,000 x 10,000 genotype-like 0/1 matrix (per-row
+allele count k with P(k) ~ 1/k, mosaics of 256 founders in 16 density classes,
+segment mean 512, flips to hit k exactly; seed 20261205), RePair-compressed by
+the engine's builder, B = 64 FP32 right-hand sides ~N(0, 1), X row-major
+cols x B. A step is one sweep of the grammar over the B vectors with X and Y
+resident in HBM. The launch schedule is chosen by the exhaustive autotuner
+before timing. Prints one JSON line (rank 0).
+
+Also measured, outside the headline value:
+  plain      the plain per-level sweep at B = 1 (reference layout, one launch
+             per level, atomics on the spot), in nnz/s;
+  fp32_err   max_i |y_i - y*_i| / sum_j |M_ij x_j| against the FP64 oracle on a
+             sample of columns;
+  cpu_baseline  the reference's OpenMP CPU sweep (oracle restatement) on the
+             same grammar, on a bounded sample of single-vector sweeps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "g-mm-repair_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "effective nnz·vectors/s and HBM GB/s (% of peak), batched vs plain sweep"
+HBM_PEAK_GBS = 8000.0
+T0 = time.time()
+
+
+def log(*a):
+    print(f"[bench {time.time() - T0:6.1f}s]", *a, file=sys.stderr, flush=True)
+
+
+def dag_bytes(info, B):
+    """Algorithmic bytes of one sweep over B vectors, counted on the grammar DAG
+    (schedule-independent): per rule 2 child ids + per vector 2 reads, 1 write;
+    per emission (slot, row) + per vector 1 read and an RMW of y; per terminal
+    (value, column) + per vector an x gather and a T write; y initialised."""
+    R, E, Et, U, rows = info["num_raw_nt"], info["emit_total"], info["num_term_emit"], info["U"], info["rows"]
+    return 8 * R + 8 * (E + Et) + 8 * (U - 1) + 4 * B * (3 * R + 3 * (E + Et) + 2 * (U - 1) + rows)
+
+
+def read_traffic(workload):
+    """HBM bytes per sweep measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    from profiles/traffic.txt lines `<workload> <bytes>`; None if absent."""
+    p = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 2 and parts[0] == workload and not line.startswith("#"):
+                return float(parts[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--B", type=int, default=64)
+    ap.add_argument("--rows", type=int, default=500000)
+    ap.add_argument("--cols", type=int, default=10000)
+    ap.add_argument("--seed", type=int, default=20261205)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="use the default schedule instead of autotuning")
+    ap.add_argument("--variant", default=None, help="force a variant (levels|graph|persistent)")
+    ap.add_argument("--emit", default=None, help="force an emission strategy (spot|gather)")
+    ap.add_argument("--b-tile", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this engine's sweep runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not a measured path). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import gmr
+    from gmr import autotune
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    B = args.B
+    log("generating", args.rows, "x", args.cols)
+    m = gmr.Matrix.genotype(args.seed, args.rows, args.cols)
+    nnz = m.nnz()
+    log("nnz", nnz, "; RePair ...")
+    g = m.repair()
+    log(f"RePair {g.build_seconds:.1f}s rounds={g.build_rounds}", g.info())
+    t = time.time()
+    s_res = g.schedule(args.rows, args.cols, "resident")
+    s_ref = g.schedule(args.rows, args.cols, "refcompat")
+    sched_s = time.time() - t
+    info = s_res.info()
+    info_ref = s_ref.info()
+    log("schedules", f"{sched_s:.1f}s", "resident", info, "refcompat", info_ref)
+    eng = s_res.engine()
+    log("engine uploaded", eng.info())
+
+    gen = torch.Generator(device="cuda").manual_seed(args.seed)
+    X = torch.randn(args.cols, B, device="cuda", generator=gen, dtype=torch.float32)
+    Y = torch.empty(args.rows, B, device="cuda", dtype=torch.float32)
+
+    # ---- schedule choice
+    if args.no_tune or args.variant:
+        best = dict(variant=gmr.VARIANTS[args.variant or "persistent"], emit=gmr.EMITS[args.emit or "gather"],
+                    b_tile=args.b_tile or 0, block=256)
+        tuned = []
+    else:
+        cands = autotune.candidates(B, gmr.RESIDENT)
+        best, tuned = autotune.tune(eng, 0, X, Y, cands, reps=2, log=log)
+    log("schedule:", autotune.describe(best))
+
+    # ---- timed region: W warm-up steps, then K steps, one event pair per step
+    for _ in range(args.warmup):
+        eng.run_torch(0, X, Y, **best)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for i in range(args.steps):
+        ev[i][0].record()
+        eng.run_torch(0, X, Y, **best)
+        ev[i][1].record()
+    stop.record()
+    torch.cuda.synchronize()
+    if best["variant"] == gmr.VARIANT_PERSISTENT:
+        eng.check()
+    total_ms = start.elapsed_time(stop)
+    per = [a.elapsed_time(b) for a, b in ev]
+    ms_step = total_ms / args.steps
+    kern_ms = float(np.mean(per))
+    value = nnz * B / (ms_step * 1e-3)
+    log(f"batched B={B}: {ms_step:.3f} ms/step, launch avg {kern_ms:.3f} ms, {value / 1e9:.1f} G nnz*vec/s")
+
+    # ---- FP32 error against the FP64 oracle on a sample of columns
+    from oracle import oracle as O
+    seq, V = m.arrays()
+    cols_chk = [0, B // 3, (2 * B) // 3, B - 1]
+    Xs = X[:, cols_chk].contiguous().cpu().numpy()
+    Ys, den = O.brute_force_batched(0, args.rows, args.cols, seq, V, Xs)
+    Yg = Y[:, cols_chk].cpu().numpy().astype(np.float64)
+    d = np.abs(Yg - Ys)
+    err = float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1), np.where(d == 0, 0, np.inf))))
+    log(f"fp32 err (4 columns vs FP64 oracle): {err:.3e}")
+    del seq
+
+    # ---- the plain per-level sweep (reference layout, B = 1, spot atomics)
+    eng_ref = s_ref.engine()
+    x1 = X[:, 0].contiguous()
+    y1 = torch.empty(args.rows, device="cuda", dtype=torch.float32)
+    plain = dict(variant=gmr.VARIANT_LEVELS, emit=gmr.EMIT_SPOT, b_tile=0, block=256)
+    for _ in range(3):
+        eng_ref.run_torch(0, x1, y1, **plain)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        eng_ref.run_torch(0, x1, y1, **plain)
+    b.record()
+    torch.cuda.synchronize()
+    plain_ms = a.elapsed_time(b) / args.steps
+    plain_val = nnz / (plain_ms * 1e-3)
+    log(f"plain per-level sweep B=1: {plain_ms:.3f} ms, {plain_val / 1e9:.2f} G nnz/s")
+    # best single-vector schedule, for reference
+    best1, _ = autotune.tune(eng, 0, x1, y1, autotune.candidates(1, gmr.RESIDENT), reps=3)
+    one_ms = autotune.time_candidate(eng, 0, x1, y1, best1, reps=args.steps)
+    del eng_ref
+
+    # ---- roofline of the dominant kernel (one launch = one sweep for the
+    # persistent variant; otherwise the sweep's launches together)
+    bytes_alg = dag_bytes(info, B)
+    achieved = bytes_alg / (kern_ms * 1e-3) / 1e9
+    workload = f"genotype{args.rows}x{args.cols}_B{B}"
+    traffic = read_traffic(workload)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_persistent<PlusTimesOp>" if best["variant"] == gmr.VARIANT_PERSISTENT else "sweep",
+                "bytes_per_launch": bytes_alg}
+
+    # ---- CPU baseline: the reference's OpenMP FP32 sweep, bounded sample
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        sa = s_ref.arrays()
+        xs = X[:, 0].cpu().numpy()
+        O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)  # warm-up
+        n, t0 = 0, time.time()
+        while n < 1 or (time.time() - t0 < args.cpu_seconds and n < 50):
+            O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)
+            n += 1
+        cpu_s = (time.time() - t0) / n
+        cpu = {"value": nnz / cpu_s, "unit": "nnz*vectors/s", "cores": threads, "kind": "port",
+               "sample": f"{n} single-vector sweeps of the same grammar (reference layout, FP32, OpenMP "
+                         f"{threads} threads), {cpu_s * 1e3:.1f} ms each"}
+        log("cpu baseline", cpu)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "nnz*vectors/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded genotype mosaic, RePair-compressed by the engine's builder; X ~ N(0,1))",
+        "config": {"workload": workload, "rows": args.rows, "cols": args.cols, "nnz": nnz, "B": B,
+                   "grammar": {"rules": info["num_raw_nt"], "C": g.info()["clen"], "depth": info["max_depth"],
+                               "emissions": info["emit_total"], "term_emissions": info["num_term_emit"],
+                               "U": info["U"], "refcompat_passthrough": info_ref["num_passthrough"]},
+                   "schedule": autotune.describe(best), "repair_s": round(g.build_seconds, 1),
+                   "schedule_build_s": round(sched_s, 1)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_sweep": {"value": plain_val, "unit": "nnz/s", "ms": plain_ms,
+                        "schedule": "refcompat/levels/spot B=1"},
+        "batched_over_plain": value / plain_val,
+        "best_single_vector": {"value": nnz / (one_ms * 1e-3), "ms": one_ms, "schedule": autotune.describe(best1)},
+        "fp32_err": err,
+        "tuned": [(autotune.describe(c), round(ms, 4)) for c, ms in tuned],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

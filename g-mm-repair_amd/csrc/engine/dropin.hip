@@ -1,0 +1,228 @@
+// The reference's two extern "C" entry points (gpu_engine.h) on top of the
+// device-resident engine.
+//
+// Each call does what the reference's does (gpu-engine/gpu_engine.cu:135-587):
+// take host buffers, run `iters` sweeps, time them with events, print the same
+// lines, return the last y. The schedule it receives is the reference's
+// double-buffered layout; before upload it is re-laid out for the engine:
+// identity copies (pass-throughs: right child = the zero terminal, coeff 1) are
+// resolved to the slot they copy, every remaining rule gets a slot in one
+// resident value array, and the top sequence is regrouped by row so y is
+// produced by one row gather with no atomics. The values are the same as the
+// reference's sweep: an identity copy returns its input in every semiring (for
+// Tropical any value >= 1e9 acts as +infinity either way).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "gmr.h"
+#include "gpu_engine.h"
+
+const char* gmr_engine_last_error_internal();
+
+namespace {
+
+[[noreturn]] void die(const std::string& what) {
+    std::cerr << "GPU engine error: " << what << std::endl;
+    std::exit(1);
+}
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) die(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int semiring_from_env() {
+    const char* s = std::getenv("SEMIRING");
+    std::string v = s ? s : "plustimes";
+    if (v == "boolean" || v == "bool") return GMR_BOOLEAN;
+    if (v == "tropical" || v == "minplus") return GMR_TROPICAL;
+    return GMR_PLUS_TIMES;
+}
+
+// Reference layout -> Resident layout (global slots, no identity copies).
+struct Resident {
+    std::vector<LayeredRule> rules;
+    std::vector<int> level_off, emit_off, emit_pos, emit_row;
+    int L = 0, max_width = 0;
+};
+
+Resident to_resident(int total, const LayeredRule* r, const GPUSchedule& s) {
+    Resident o;
+    const int L = s.max_depth;
+    if (L < 0) die("negative max_depth");
+    std::vector<int> prev, cur;  // canonical global slot of each slot of level k-1 / k
+    o.level_off.assign(1, 0);
+    o.emit_off.assign(1, 0);
+    for (int k = 1; k <= L; ++k) {
+        const int lo = s.level_offsets[k - 1], hi = s.level_offsets[k];
+        if (lo < 0 || hi < lo || hi > total) die("schedule level offsets out of range");
+        cur.assign(hi - lo, -1);
+        const int base = static_cast<int>(o.rules.size());
+        auto canon = [&](int c) -> int {
+            if (c < 0) return c;
+            if (c >= static_cast<int>(prev.size())) die("rule child out of range");
+            return prev[c];
+        };
+        for (int i = lo; i < hi; ++i) {
+            const LayeredRule& x = r[i];
+            if (x.right == -1 && x.coeff == 1.0f && x.left >= 0) {
+                cur[i - lo] = canon(x.left);  // identity copy
+            } else {
+                LayeredRule y{canon(x.left), canon(x.right), x.coeff};
+                cur[i - lo] = static_cast<int>(o.rules.size());
+                o.rules.push_back(y);
+            }
+        }
+        const int eb = s.emit_offsets[k - 1], ee = s.emit_offsets[k];
+        if (eb < 0 || ee < eb || ee > s.emit_total) die("schedule emit offsets out of range");
+        // Emissions of level k go to the level that computed the value.
+        for (int e = eb; e < ee; ++e) {
+            int p = s.emit_pos[e];
+            if (p < 0 || p >= hi - lo) die("emission position out of range");
+            o.emit_pos.push_back(cur[p]);
+            o.emit_row.push_back(s.emit_row[e]);
+        }
+        const int w = static_cast<int>(o.rules.size()) - base;
+        o.level_off.push_back(static_cast<int>(o.rules.size()));
+        o.max_width = std::max(o.max_width, w);
+        prev.swap(cur);
+    }
+    // An emission of an identity copy refers to a slot of an earlier level:
+    // regroup emissions by the level that owns their slot.
+    o.L = L;
+    std::vector<int> owner_cnt(L + 2, 0);
+    auto owner = [&](int slot) {
+        int k = static_cast<int>(std::upper_bound(o.level_off.begin(), o.level_off.end(), slot) - o.level_off.begin());
+        return k;  // level_off[k-1] <= slot < level_off[k]
+    };
+    std::vector<int> lv(o.emit_pos.size());
+    for (std::size_t e = 0; e < o.emit_pos.size(); ++e) {
+        lv[e] = owner(o.emit_pos[e]);
+        ++owner_cnt[lv[e]];
+    }
+    o.emit_off.assign(L + 1, 0);
+    for (int k = 1; k <= L; ++k) o.emit_off[k] = o.emit_off[k - 1] + owner_cnt[k];
+    std::vector<int> fill(o.emit_off.begin(), o.emit_off.end() - 1), ep(o.emit_pos.size()), er(ep.size());
+    for (std::size_t e = 0; e < o.emit_pos.size(); ++e) {
+        int k = lv[e];
+        int d = fill[k - 1]++;
+        ep[d] = o.emit_pos[e];
+        er[d] = o.emit_row[e];
+    }
+    o.emit_pos.swap(ep);
+    o.emit_row.swap(er);
+    return o;
+}
+
+void dropin(int rows, int cols, int alpha, int total, const LayeredRule* h_rules, const int* h_val,
+            const int* h_col, const float* h_V, int nvals, const float* h_X, const GPUSchedule& s,
+            float* h_Y, int B, int iters, bool batched) {
+    const int sr = semiring_from_env();
+    if (!batched) {
+        std::cout << "[GPU Setup Debug] rows=" << rows << ", cols=" << cols << ", alpha=" << alpha
+                  << ", total_rules=" << total << ", distinct_vals=" << nvals
+                  << ", sched_max_depth=" << s.max_depth << ", sched_max_width=" << s.max_width
+                  << ", emit_total=" << s.emit_total << ", num_term_emit=" << s.num_term_emit << std::endl;
+    }
+    // Reference analytic footprint (gpu_engine.cu:174-184), with B-wide buffers.
+    const std::size_t vb = 4;
+    const std::size_t mem_analytic = static_cast<std::size_t>(alpha) * vb * B + static_cast<std::size_t>(total) * sizeof(LayeredRule) +
+                                     static_cast<std::size_t>(alpha > 0 ? alpha - 1 : 0) * sizeof(int) * 2 +
+                                     static_cast<std::size_t>(nvals) * sizeof(float) + static_cast<std::size_t>(cols) * vb * B +
+                                     static_cast<std::size_t>(rows) * vb * B + static_cast<std::size_t>(s.max_width) * vb * 2 * B +
+                                     static_cast<std::size_t>(s.emit_total) * sizeof(int) * 2 +
+                                     static_cast<std::size_t>(s.num_term_emit) * sizeof(int) * 2;
+    std::size_t free0 = 0, tot0 = 0;
+    hip_check(hipMemGetInfo(&free0, &tot0), "hipMemGetInfo");
+
+    Resident res = to_resident(total, h_rules, s);
+    GPUSchedule rs{};
+    rs.max_depth = res.L;
+    rs.max_width = res.max_width;
+    std::vector<int> widths(res.L + 1, 0);
+    for (int k = 1; k <= res.L; ++k) widths[k] = res.level_off[k] - res.level_off[k - 1];
+    rs.level_offsets = res.level_off.data();
+    rs.level_widths = widths.data();
+    rs.emit_offsets = res.emit_off.data();
+    rs.emit_pos = res.emit_pos.data();
+    rs.emit_row = res.emit_row.data();
+    rs.emit_total = static_cast<int>(res.emit_pos.size());
+    rs.term_emit_sym = s.term_emit_sym;
+    rs.term_emit_row = s.term_emit_row;
+    rs.num_term_emit = s.num_term_emit;
+    gmr_engine* eng = nullptr;
+    if (gmr_engine_create(rows, cols, alpha, static_cast<int>(res.rules.size()), res.rules.data(), h_val, h_col, h_V,
+                          nvals, &rs, GMR_LAYOUT_RESIDENT, &eng) != GMR_OK)
+        die(gmr_engine_last_error_internal());
+
+    float* dX = nullptr;
+    void* dY = nullptr;
+    hip_check(hipMalloc(&dX, sizeof(float) * static_cast<std::size_t>(cols) * B), "hipMalloc(X)");
+    hip_check(hipMalloc(&dY, vb * static_cast<std::size_t>(rows > 0 ? rows : 1) * B), "hipMalloc(Y)");
+    hip_check(hipMemcpy(dX, h_X, sizeof(float) * static_cast<std::size_t>(cols) * B, hipMemcpyHostToDevice), "copy X");
+    gmr_run_opts o{};
+    o.variant = GMR_VARIANT_GRAPH;
+    o.emit = GMR_EMIT_GATHER;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
+    for (int it = 0; it < iters; ++it) {
+        if (gmr_engine_run(eng, sr, dX, B, dY, &o, nullptr) != GMR_OK) die(gmr_engine_last_error_internal());
+    }
+    hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    if (batched) {
+        double per_batch = static_cast<double>(ms) / iters;
+        std::cout << "GPU batched (B=" << B << "): " << per_batch << " ms/batch, " << per_batch / B
+                  << " ms/vector over " << iters << " batches." << std::endl;
+    } else {
+        std::cout << "GPU kernel: " << ms << " ms total over " << iters << " vectors = " << (ms / iters)
+                  << " ms/vector." << std::endl;
+    }
+    std::size_t free1 = 0, tot1 = 0;
+    hip_check(hipMemGetInfo(&free1, &tot1), "hipMemGetInfo");
+    if (!batched) {
+        std::cout << "MEM analytic bytes: " << mem_analytic << std::endl;
+        std::cout << "MEM peak bytes: " << (free0 > free1 ? free0 - free1 : 0) << std::endl;
+        // No energy counter is read here; the reference prints -1 in that case.
+        std::cout << "ENERGY mJ/vector: " << -1.0 << std::endl;
+        std::cout << "ENERGY avg power mW: " << -1.0 << std::endl;
+    }
+    if (iters > 0 && rows > 0)
+        hip_check(hipMemcpy(h_Y, dY, vb * static_cast<std::size_t>(rows) * B, hipMemcpyDeviceToHost), "copy Y");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    gmr_engine_destroy(eng);
+}
+
+}  // namespace
+
+extern "C" void run_gpu_right_multiply_layered(int rows, int cols, int alpha, int total_layered_rules,
+                                               const LayeredRule* h_rules, const int* h_val_indices,
+                                               const int* h_col_indices, const float* h_V,
+                                               int num_distinct_vals, const float* h_x,
+                                               const GPUSchedule& h_sched, float* h_y, int iters) {
+    dropin(rows, cols, alpha, total_layered_rules, h_rules, h_val_indices, h_col_indices, h_V, num_distinct_vals,
+           h_x, h_sched, h_y, 1, iters, false);
+}
+
+extern "C" void run_gpu_right_multiply_batched(int rows, int cols, int alpha, int total_layered_rules,
+                                               const LayeredRule* h_rules, const int* h_val_indices,
+                                               const int* h_col_indices, const float* h_V,
+                                               int num_distinct_vals, const float* h_X,
+                                               const GPUSchedule& h_sched, float* h_Y, int B, int iters) {
+    if (B < 1) die("B must be >= 1");
+    dropin(rows, cols, alpha, total_layered_rules, h_rules, h_val_indices, h_col_indices, h_V, num_distinct_vals,
+           h_X, h_sched, h_Y, B, iters, true);
+}

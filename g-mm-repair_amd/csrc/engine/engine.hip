@@ -1,0 +1,518 @@
+// Device engine: upload once, sweep many times (the gmr.h C ABI).
+//
+// The grammar structure lives in HBM for the life of the engine:
+//   kids      int2 (left, right) per layered rule, level by level (8 B/rule;
+//             the reference's 12-byte AoS LayeredRule minus the coefficient)
+//   coeff     float per rule, only when some rule is run-length (coeff != 1)
+//   tcol/tval column and value of every compact terminal
+//   emits     int2 (pos, row) per top-sequence non-terminal, level by level,
+//             row-sorted inside a level
+//   term_emits int2 (sym, row) per top-sequence terminal
+//   crow/csym row-ordered top sequence (Resident layout, gather emission)
+// Values (per run, sized by the vector tile Bt): T = U x Bt terminal values and
+// the frontier: two w* x Bt buffers for the double-buffered layouts, or one
+// num_layered x Bt resident array.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "build_schedule.h"
+#include "engine_internal.hpp"
+#include "gmr.h"
+#include "sweep_kernels.hpp"
+
+const BuiltSchedule* gmr_schedule_internal(const gmr_schedule* s);
+void gmr_host_set_error(const char* msg);
+
+namespace gmr {
+
+namespace {
+
+thread_local std::string g_dev_err;
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define GMR_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw DeviceError(std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+template <class T>
+T* dev_upload(const T* h, std::size_t n, std::size_t* acc) {
+    if (n == 0) return nullptr;
+    T* d = nullptr;
+    GMR_HIP(hipMalloc(&d, n * sizeof(T)));
+    GMR_HIP(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    if (acc) *acc += n * sizeof(T);
+    return d;
+}
+
+inline int pow2_at_least(int v) {
+    int g = 1;
+    while (g < v && g < 64) g <<= 1;
+    return g;
+}
+
+}  // namespace
+
+Engine::~Engine() {
+    for (auto& kv : graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_level_off,
+                    d_emit_off, ws_T, ws_W0, ws_W1, d_barrier};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+}
+
+void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRule* rules,
+                    const int* val_idx, const int* col_idx, const float* V, int nvals_,
+                    const GPUSchedule& s, int layout_) {
+    if (rows_ < 0 || cols_ < 1 || U_ < 1 || num_rules < 0 || nvals_ < 0 || s.max_depth < 0)
+        throw std::invalid_argument("bad engine dimensions");
+    if (layout_ < 0 || layout_ > 2) throw std::invalid_argument("unknown layout");
+    rows = rows_;
+    cols = cols_;
+    U = U_;
+    NL = num_rules;
+    L = s.max_depth;
+    nvals = nvals_;
+    layout = layout_;
+    resident = layout == 2;
+    level_off.assign(L + 1, 0);
+    emit_off.assign(L + 1, 0);
+    if (L > 0 && (!s.level_offsets || !s.emit_offsets))
+        throw std::invalid_argument("schedule offsets are NULL");
+    for (int k = 0; k <= L; ++k) {
+        level_off[k] = s.level_offsets ? s.level_offsets[k] : 0;
+        emit_off[k] = s.emit_offsets ? s.emit_offsets[k] : 0;
+    }
+    if (level_off[0] != 0 || emit_off[0] != 0) throw std::invalid_argument("offsets must start at 0");
+    max_width = 0;
+    for (int k = 1; k <= L; ++k) {
+        if (level_off[k] < level_off[k - 1] || emit_off[k] < emit_off[k - 1])
+            throw std::invalid_argument("schedule offsets decrease");
+        max_width = std::max(max_width, level_off[k] - level_off[k - 1]);
+    }
+    if (level_off[L] != NL) throw std::invalid_argument("level offsets do not cover the rules");
+    E = emit_off[L];
+    if (E != s.emit_total) throw std::invalid_argument("emit offsets do not cover the emissions");
+    Et = s.num_term_emit;
+    if (Et < 0 || (Et > 0 && (!s.term_emit_sym || !s.term_emit_row)) || (E > 0 && (!s.emit_pos || !s.emit_row)))
+        throw std::invalid_argument("schedule emission arrays are NULL");
+    if (NL > 0 && !rules) throw std::invalid_argument("rules are NULL");
+    if (U > 1 && (!val_idx || !col_idx)) throw std::invalid_argument("terminal arrays are NULL");
+    if (nvals > 0 && !V) throw std::invalid_argument("V is NULL");
+
+    // Range checks: a bad index in any of these would be a wild device access.
+    std::vector<int2> kids(NL);
+    std::vector<float> coeff;
+    has_runlen = false;
+    for (int k = 1; k <= L; ++k) {
+        const int lo = level_off[k - 1], hi = level_off[k];
+        const int prev_w = k > 1 ? level_off[k - 1] - level_off[k - 2] : 0;
+        const int lim = resident ? lo : prev_w;
+        for (int i = lo; i < hi; ++i) {
+            const LayeredRule& r = rules[i];
+            for (int c : {r.left, r.right}) {
+                if (c >= 0 ? c >= lim : -static_cast<long long>(c) - 1 >= U)
+                    throw std::invalid_argument("rule " + std::to_string(i) + " child out of range");
+            }
+            kids[i] = make_int2(r.left, r.right);
+            if (r.coeff != 1.0f) has_runlen = true;
+        }
+    }
+    if (has_runlen) {
+        coeff.resize(NL);
+        for (int i = 0; i < NL; ++i) coeff[i] = rules[i].coeff;
+    }
+    std::vector<int2> em(E), te(Et);
+    for (int k = 1; k <= L; ++k) {
+        const int w = level_off[k] - level_off[k - 1];
+        for (int e = emit_off[k - 1]; e < emit_off[k]; ++e) {
+            int p = s.emit_pos[e], r = s.emit_row[e];
+            bool ok = resident ? (p >= level_off[k - 1] && p < level_off[k]) : (p >= 0 && p < w);
+            if (!ok || r < 0 || r >= rows) throw std::invalid_argument("emission out of range");
+            em[e] = make_int2(p, r);
+        }
+    }
+    for (int e = 0; e < Et; ++e) {
+        int sy = s.term_emit_sym[e], r = s.term_emit_row[e];
+        if (sy < 0 || sy >= U || r < 0 || r >= rows) throw std::invalid_argument("terminal emission out of range");
+        te[e] = make_int2(sy, r);
+    }
+    // Emissions must be row-sorted inside a level for the run pre-reduction;
+    // the order only changes which atomics combine first, so sort when needed.
+    auto row_sort = [](int2* b, int2* e) {
+        if (!std::is_sorted(b, e, [](const int2& x, const int2& y) { return x.y < y.y; }))
+            std::stable_sort(b, e, [](const int2& x, const int2& y) { return x.y < y.y; });
+    };
+    for (int k = 1; k <= L; ++k) row_sort(em.data() + emit_off[k - 1], em.data() + emit_off[k]);
+    row_sort(te.data(), te.data() + Et);
+
+    std::vector<int> tcol(U, 0);
+    std::vector<float> tval(U, 0.0f);
+    for (int q = 1; q < U; ++q) {
+        int v = val_idx[q - 1], c = col_idx[q - 1];
+        if (v < 0 || v >= nvals || c < 0 || c >= cols) throw std::invalid_argument("terminal decode out of range");
+        tcol[q] = c;
+        tval[q] = V[v];
+    }
+
+    // Resident layout: the row-ordered top sequence for gather emission.
+    std::vector<int> crow, csym;
+    if (resident) {
+        crow.assign(static_cast<std::size_t>(rows) + 1, 0);
+        for (const int2& p : em) ++crow[p.y + 1];
+        for (const int2& p : te) ++crow[p.y + 1];
+        for (int r = 0; r < rows; ++r) crow[r + 1] += crow[r];
+        csym.resize(static_cast<std::size_t>(E) + Et);
+        std::vector<int> fill(crow.begin(), crow.end() - 1);
+        // Terminals first, then levels in order (C order is not needed).
+        for (const int2& p : te) csym[fill[p.y]++] = -p.x - 1;
+        for (const int2& p : em) csym[fill[p.y]++] = p.x;
+    }
+
+    struct_bytes = 0;
+    d_kids = dev_upload(kids.data(), kids.size(), &struct_bytes);
+    d_coeff = has_runlen ? dev_upload(coeff.data(), coeff.size(), &struct_bytes) : nullptr;
+    d_tcol = dev_upload(tcol.data(), tcol.size(), &struct_bytes);
+    d_tval = dev_upload(tval.data(), tval.size(), &struct_bytes);
+    d_emits = dev_upload(em.data(), em.size(), &struct_bytes);
+    d_term = dev_upload(te.data(), te.size(), &struct_bytes);
+    if (resident) {
+        d_crow = dev_upload(crow.data(), crow.size(), &struct_bytes);
+        d_csym = dev_upload(csym.data(), csym.size(), &struct_bytes);
+    }
+    d_level_off = dev_upload(level_off.data(), level_off.size(), &struct_bytes);
+    d_emit_off = dev_upload(emit_off.data(), emit_off.size(), &struct_bytes);
+    GMR_HIP(hipEventCreate(&ev0));
+    GMR_HIP(hipEventCreate(&ev1));
+    int dev = 0;
+    GMR_HIP(hipGetDevice(&dev));
+    GMR_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    GMR_HIP(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev));
+    if (wall_khz <= 0) wall_khz = 100000;
+    GMR_HIP(hipMalloc(&d_barrier, sizeof(dev::GridBarrier)));
+    GMR_HIP(hipMemset(d_barrier, 0, sizeof(dev::GridBarrier)));
+    GMR_HIP(hipDeviceSynchronize());
+    barrier_base = 0;
+}
+
+int Engine::check_barrier() {
+    dev::GridBarrier h{};
+    GMR_HIP(hipDeviceSynchronize());
+    GMR_HIP(hipMemcpy(&h, d_barrier, sizeof(h), hipMemcpyDeviceToHost));
+    if (h.error || h.arrived != barrier_base) {
+        // Restart the barrier from a clean state.
+        GMR_HIP(hipMemset(d_barrier, 0, sizeof(dev::GridBarrier)));
+        GMR_HIP(hipDeviceSynchronize());
+        barrier_base = 0;
+        return 1;
+    }
+    return 0;
+}
+
+void Engine::ensure_workspace(int Bt, int vbytes) {
+    const std::size_t tb = static_cast<std::size_t>(U) * Bt * vbytes;
+    const std::size_t wn = resident ? static_cast<std::size_t>(NL) : static_cast<std::size_t>(max_width);
+    const std::size_t wb = std::max<std::size_t>(wn, 1) * Bt * vbytes;
+    if (tb > ws_T_bytes || wb > ws_W_bytes) {
+        // Buffers move: captured graphs hold the old addresses.
+        for (auto& kv : graphs) {
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+            if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+        }
+        graphs.clear();
+        GMR_HIP(hipDeviceSynchronize());
+    }
+    if (tb > ws_T_bytes) {
+        if (ws_T) GMR_HIP(hipFree(ws_T));
+        ws_T = nullptr;
+        GMR_HIP(hipMalloc(&ws_T, tb));
+        ws_T_bytes = tb;
+    }
+    if (wb > ws_W_bytes) {
+        if (ws_W0) GMR_HIP(hipFree(ws_W0));
+        if (ws_W1) GMR_HIP(hipFree(ws_W1));
+        ws_W0 = ws_W1 = nullptr;
+        GMR_HIP(hipMalloc(&ws_W0, wb));
+        if (!resident) GMR_HIP(hipMalloc(&ws_W1, wb));
+        ws_W_bytes = wb;
+    }
+}
+
+namespace {
+
+using dev::SweepArgs;
+
+inline unsigned grid_for(long work, int block, int cap = 1 << 20) {
+    long g = (work + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return static_cast<unsigned>(g);
+}
+
+// The plain sweep: one launch per phase and level, in the reference's order
+// (init, terminal emits, then per level the node update and its emits).
+template <class S>
+void launch_levels(const Engine& en, const SweepArgs& a, int block, hipStream_t st) {
+    const int G = a.G;
+    const int gpb = block / G;  // groups per block
+    hipLaunchKernelGGL(dev::k_init<S>, dim3(grid_for(static_cast<long>(en.U) * a.Bt +
+                                                      static_cast<long>(en.rows) * a.Bt, block, 4096)),
+                       dim3(block), 0, st, a);
+    if (!a.gather && en.Et > 0) {
+        long work = (a.Bt == 1 && a.B == 1) ? en.Et : ((static_cast<long>(en.Et) + dev::kEmitChunk - 1) / dev::kEmitChunk) * G;
+        hipLaunchKernelGGL(dev::k_emit<S>, dim3(grid_for(work, block)), dim3(block), 0, st, a, 0);
+    }
+    for (int k = 1; k <= en.L; ++k) {
+        const int w = en.level_off[k] - en.level_off[k - 1];
+        if (w > 0)
+            hipLaunchKernelGGL(dev::k_eval<S>, dim3(grid_for(static_cast<long>(w), gpb)), dim3(block), 0, st, a, k);
+        const int ne = en.emit_off[k] - en.emit_off[k - 1];
+        if (!a.gather && ne > 0) {
+            long work = (a.Bt == 1 && a.B == 1) ? ne : ((static_cast<long>(ne) + dev::kEmitChunk - 1) / dev::kEmitChunk) * G;
+            hipLaunchKernelGGL(dev::k_emit<S>, dim3(grid_for(work, block)), dim3(block), 0, st, a, k);
+        }
+    }
+    if (a.gather)
+        hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(static_cast<long>(en.rows), gpb)), dim3(block), 0, st, a);
+    GMR_HIP(hipGetLastError());
+}
+
+// One launch for the whole sweep, sized to one resident wave of workgroups
+// (occupancy x CUs, capped) so every workgroup is on the chip while the
+// barrier waits; a wait that outlives its deadline ends with the error word set.
+template <class S>
+void launch_persistent(Engine& en, const SweepArgs& a, int block, int grid_cap, hipStream_t st) {
+    int per_cu = 0;
+    GMR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_persistent<S>, block, 0));
+    if (per_cu < 1) throw DeviceError("persistent kernel does not fit on a CU");
+    per_cu = std::min(per_cu, 2048 / block);
+    int grid = per_cu * en.num_cus;
+    if (grid_cap > 0) grid = std::min(grid, grid_cap);
+    const unsigned long long deadline = static_cast<unsigned long long>(en.wall_khz) * 1000ULL * 2ULL;  // 2 s
+    hipLaunchKernelGGL(dev::k_persistent<S>, dim3(grid), dim3(block), 0, st, a, en.d_barrier, en.barrier_base,
+                       deadline);
+    GMR_HIP(hipGetLastError());
+    en.barrier_base += static_cast<unsigned long long>(grid) * static_cast<unsigned long long>(a.L + 1);
+}
+
+template <class S>
+void run_tile(Engine& en, int semiring, SweepArgs a, const gmr_run_opts& o, hipStream_t st) {
+    const int block = o.block > 0 ? o.block : 256;
+    if (o.variant == GMR_VARIANT_PERSISTENT) {
+        launch_persistent<S>(en, a, block, o.grid_cap, st);
+    } else if (o.variant == GMR_VARIANT_GRAPH) {
+        auto key = std::make_tuple(semiring, reinterpret_cast<uintptr_t>(a.X), reinterpret_cast<uintptr_t>(a.Y), a.B, a.Bt,
+                                   a.gather, block, reinterpret_cast<uintptr_t>(st));
+        auto it = en.graphs.find(key);
+        if (it == en.graphs.end()) {
+            // Capture on the engine's own stream (the caller's may be the
+            // NULL stream, which cannot be captured); replay on the caller's.
+            if (!en.capture_stream) GMR_HIP(hipStreamCreateWithFlags(&en.capture_stream, hipStreamNonBlocking));
+            Engine::Graph g;
+            GMR_HIP(hipStreamBeginCapture(en.capture_stream, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_levels<S>(en, a, block, en.capture_stream);
+            } catch (...) {
+                hipGraph_t dummy;
+                (void)hipStreamEndCapture(en.capture_stream, &dummy);
+                throw;
+            }
+            GMR_HIP(hipStreamEndCapture(en.capture_stream, &g.graph));
+            GMR_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+            it = en.graphs.emplace(key, g).first;
+        }
+        GMR_HIP(hipGraphLaunch(it->second.exec, st));
+    } else {
+        launch_levels<S>(en, a, block, st);
+    }
+}
+
+template <class S>
+void run_all(Engine& en, int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st) {
+    using T = typename S::T;
+    const int Bt = (o.b_tile > 0 && o.b_tile < B) ? o.b_tile : B;
+    const bool gather = o.emit == GMR_EMIT_GATHER;
+    if (gather && !en.resident) throw std::invalid_argument("gather emission needs the Resident layout");
+    en.ensure_workspace(Bt, static_cast<int>(sizeof(T)));
+    for (int b0 = 0; b0 < B; b0 += Bt) {
+        SweepArgs a{};
+        a.kids = en.d_kids;
+        a.coeff = en.d_coeff;
+        a.tcol = en.d_tcol;
+        a.tval = en.d_tval;
+        a.emits = en.d_emits;
+        a.term_emits = en.d_term;
+        a.crow = en.d_crow;
+        a.csym = en.d_csym;
+        a.level_off = en.d_level_off;
+        a.emit_off = en.d_emit_off;
+        a.X = X + b0;
+        a.Y = static_cast<T*>(Y) + b0;
+        a.Tv = en.ws_T;
+        a.W0 = en.ws_W0;
+        a.W1 = en.ws_W1;
+        a.rows = en.rows;
+        a.cols = en.cols;
+        a.U = en.U;
+        a.L = en.L;
+        a.n_term = en.Et;
+        a.B = B;
+        a.Bt = std::min(Bt, B - b0);
+        a.G = pow2_at_least(a.Bt);
+        a.resident = en.resident ? 1 : 0;
+        a.gather = gather ? 1 : 0;
+        run_tile<S>(en, semiring, a, o, st);
+    }
+}
+
+}  // namespace
+
+void Engine::run(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st) {
+    if (B < 1) throw std::invalid_argument("B must be >= 1");
+    if (!X || !Y) throw std::invalid_argument("X or Y is NULL");
+    if (o.block != 0 && (o.block < 64 || o.block > 1024 || o.block % 64))
+        throw std::invalid_argument("block must be a multiple of 64 in [64, 1024]");
+    if (o.variant < 0 || o.variant > 2) throw std::invalid_argument("unknown variant");
+    const bool timed = o.reserved[0] == 1;
+    if (timed) GMR_HIP(hipEventRecord(ev0, st));
+    switch (semiring) {
+        case GMR_PLUS_TIMES: run_all<dev::PlusTimesOp>(*this, semiring, X, B, Y, o, st); break;
+        case GMR_BOOLEAN: run_all<dev::BooleanOp>(*this, semiring, X, B, Y, o, st); break;
+        case GMR_TROPICAL: run_all<dev::TropicalOp>(*this, semiring, X, B, Y, o, st); break;
+        default: throw std::invalid_argument("unknown semiring");
+    }
+    if (timed) {
+        GMR_HIP(hipEventRecord(ev1, st));
+        GMR_HIP(hipEventSynchronize(ev1));
+        float ms = 0;
+        GMR_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        last_ms = ms;
+    }
+}
+
+}  // namespace gmr
+
+// ---------------------------------------------------------------- C ABI
+struct gmr_engine {
+    gmr::Engine e;
+};
+
+namespace {
+
+int dev_fail(int code, const char* msg) {
+    gmr::g_dev_err = msg;
+    gmr_host_set_error(msg);
+    return code;
+}
+
+template <class F>
+int dev_guarded(F&& f) {
+    try {
+        f();
+        return GMR_OK;
+    } catch (const gmr::DeviceError& e) {
+        return dev_fail(GMR_ERR_DEVICE, e.what());
+    } catch (const std::bad_alloc&) {
+        return dev_fail(GMR_ERR_NOMEM, "host allocation failed");
+    } catch (const std::invalid_argument& e) {
+        return dev_fail(GMR_ERR_INVALID, e.what());
+    } catch (const std::exception& e) {
+        return dev_fail(GMR_ERR_INTERNAL, e.what());
+    }
+}
+
+}  // namespace
+
+const char* gmr_engine_last_error_internal() { return gmr::g_dev_err.c_str(); }
+
+extern "C" {
+
+int gmr_engine_create(int rows, int cols, int U, int num_rules, const LayeredRule* rules,
+                      const int* val_idx, const int* col_idx, const float* V, int nvals,
+                      const GPUSchedule* sched, int layout, gmr_engine** out) {
+    if (!out || !sched) return dev_fail(GMR_ERR_INVALID, "NULL argument");
+    gmr_engine* h = new (std::nothrow) gmr_engine;
+    if (!h) return GMR_ERR_NOMEM;
+    int rc = dev_guarded([&] { h->e.upload(rows, cols, U, num_rules, rules, val_idx, col_idx, V, nvals, *sched, layout); });
+    if (rc != GMR_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return GMR_OK;
+}
+
+int gmr_engine_from_schedule(const gmr_schedule* s, const float* V, int nvals, gmr_engine** out) {
+    const BuiltSchedule* b = gmr_schedule_internal(s);
+    if (!b || !out) return dev_fail(GMR_ERR_INVALID, "NULL argument");
+    std::vector<int> vi(b->alpha > 0 ? b->alpha - 1 : 0), ci(vi.size());
+    for (int q = 1; q < b->alpha; ++q) {
+        int t = b->term_orig[q];
+        vi[q - 1] = (t - 1) / b->cols;
+        ci[q - 1] = (t - 1) % b->cols;
+    }
+    GPUSchedule view = const_cast<BuiltSchedule*>(b)->gpu_schedule();
+    return gmr_engine_create(b->rows, b->cols, b->alpha, static_cast<int>(b->num_layered),
+                             b->flattened_rules.data(), vi.data(), ci.data(), V, nvals, &view,
+                             static_cast<int>(b->layout), out);
+}
+
+int gmr_engine_info(const gmr_engine* h, long long info[16]) {
+    if (!h || !info) return GMR_ERR_INVALID;
+    const gmr::Engine& e = h->e;
+    long long v[16] = {e.rows, e.cols, e.U, e.NL, e.L, e.max_width, e.E, e.Et, e.layout,
+                       e.has_runlen ? 1 : 0, static_cast<long long>(e.struct_bytes), e.nvals,
+                       0, 0, 0, 0};
+    std::memcpy(info, v, sizeof(v));
+    return GMR_OK;
+}
+
+int gmr_engine_run(gmr_engine* h, int semiring, const void* d_X, int B, void* d_Y,
+                   const gmr_run_opts* opts, void* stream) {
+    if (!h) return dev_fail(GMR_ERR_INVALID, "engine is NULL");
+    gmr_run_opts o{};
+    if (opts) o = *opts;
+    return dev_guarded([&] {
+        h->e.run(semiring, static_cast<const float*>(d_X), B, d_Y, o, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gmr_engine_check(gmr_engine* h, int* barrier_error) {
+    if (!h || !barrier_error) return dev_fail(GMR_ERR_INVALID, "NULL argument");
+    return dev_guarded([&] { *barrier_error = h->e.check_barrier(); });
+}
+
+int gmr_engine_last_kernel_ms(const gmr_engine* h, double* ms) {
+    if (!h || !ms) return GMR_ERR_INVALID;
+    *ms = h->e.last_ms;
+    return GMR_OK;
+}
+
+void gmr_engine_destroy(gmr_engine* h) { delete h; }
+
+int gmr_device_sync(void) {
+    return hipDeviceSynchronize() == hipSuccess ? GMR_OK : GMR_ERR_DEVICE;
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// The device engine object behind the gmr_engine handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "gmr.h"
+#include "layered_types.h"
+#include "sweep_kernels.hpp"
+
+namespace gmr {
+
+struct Engine {
+    // structure (host copies of what the launch loop needs)
+    int rows = 0, cols = 0, U = 0, NL = 0, L = 0, nvals = 0, layout = 0;
+    bool resident = false, has_runlen = false;
+    int max_width = 0, E = 0, Et = 0;
+    std::vector<int> level_off, emit_off;
+    std::size_t struct_bytes = 0;
+
+    // structure (device)
+    int2* d_kids = nullptr;
+    float* d_coeff = nullptr;
+    int* d_tcol = nullptr;
+    float* d_tval = nullptr;
+    int2* d_emits = nullptr;
+    int2* d_term = nullptr;
+    int* d_crow = nullptr;
+    int* d_csym = nullptr;
+    int* d_level_off = nullptr;
+    int* d_emit_off = nullptr;
+
+    // value workspace, grown on demand
+    void* ws_T = nullptr;
+    void* ws_W0 = nullptr;
+    void* ws_W1 = nullptr;
+    std::size_t ws_T_bytes = 0, ws_W_bytes = 0;
+
+    // device facts and timing
+    int num_cus = 0, wall_khz = 0;
+    ::gmr::dev::GridBarrier* d_barrier = nullptr;
+    unsigned long long barrier_base = 0;  // arrivals recorded by earlier launches
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t capture_stream = nullptr;
+    double last_ms = 0.0;
+
+    // captured per-level launch sequences, keyed by everything they bake in
+    struct Graph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    using GraphKey = std::tuple<int, uintptr_t, uintptr_t, int, int, int, int, uintptr_t>;
+    std::map<GraphKey, Graph> graphs;
+
+    Engine() = default;
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+    ~Engine();
+
+    void upload(int rows, int cols, int U, int num_rules, const LayeredRule* rules, const int* val_idx,
+                const int* col_idx, const float* V, int nvals, const GPUSchedule& s, int layout);
+    void ensure_workspace(int Bt, int vbytes);
+    void run(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st);
+    // 1 if a persistent launch's barrier timed out since the last check (and
+    // the barrier state was reset), else 0. Synchronises the device.
+    int check_barrier();
+};
+
+}  // namespace gmr

@@ -1,0 +1,352 @@
+// Device code of the level sweep (gfx950, wave64).
+//
+// Work is written as "phase" functions over a range of items, parameterised by
+// a group index and a group count, so the same code runs as one launch per
+// level (the plain sweep), inside captured graphs, and inside the persistent
+// cooperative kernel where phases are separated by grid barriers.
+//
+// Lane mapping: a group of G lanes (G = 1, 2, 4, ..., 64, a divisor of the
+// wave) owns one item (a rule, an emission chunk, a row); its lanes stride over
+// the B vectors of that item. With vectors innermost (W[slot*B + b]) a G = 64
+// group reads and writes 256-byte contiguous runs, so a wave moves whole cache
+// lines for every child gather and every emit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gmr {
+namespace dev {
+
+constexpr int kTropicalInf = 1000000000;
+
+// Semiring policies (same algebra as the reference's gpu_engine.h:46-97).
+struct PlusTimesOp {
+    using T = float;
+    static __device__ __forceinline__ T zero() { return 0.0f; }
+    static __device__ __forceinline__ T oplus(T a, T b) { return a + b; }
+    static __device__ __forceinline__ T leaf(float v, float x) { return v * x; }
+    static __device__ __forceinline__ T apply(T v, float c) { return c * v; }
+    static __device__ __forceinline__ bool skip(T) { return false; }
+    // Hardware global_atomic_add_f32 (no compare-and-swap loop).
+    static __device__ __forceinline__ void emit(T* y, T v) { unsafeAtomicAdd(y, v); }
+};
+
+struct BooleanOp {
+    using T = int;
+    static __device__ __forceinline__ T zero() { return 0; }
+    static __device__ __forceinline__ T oplus(T a, T b) { return a | b; }
+    static __device__ __forceinline__ T leaf(float v, float x) { return (v != 0.0f && x != 0.0f) ? 1 : 0; }
+    static __device__ __forceinline__ T apply(T v, float) { return v; }
+    static __device__ __forceinline__ bool skip(T v) { return v == 0; }  // OR with 0 is a no-op
+    static __device__ __forceinline__ void emit(T* y, T v) { atomicOr(y, v); }
+};
+
+struct TropicalOp {
+    using T = int;
+    static __device__ __forceinline__ T zero() { return kTropicalInf; }
+    static __device__ __forceinline__ T oplus(T a, T b) { return a < b ? a : b; }
+    static __device__ __forceinline__ T leaf(float v, float x) {
+        int iv = static_cast<int>(v), ix = static_cast<int>(x);
+        if (iv >= kTropicalInf || ix >= kTropicalInf) return kTropicalInf;
+        return iv + ix;
+    }
+    static __device__ __forceinline__ T apply(T v, float) { return v; }
+    // y starts at +inf, so min with anything >= +inf leaves it unchanged.
+    static __device__ __forceinline__ bool skip(T v) { return v >= kTropicalInf; }
+    static __device__ __forceinline__ void emit(T* y, T v) { atomicMin(y, v); }
+};
+
+// Everything one sweep needs, passed by value to every kernel.
+struct SweepArgs {
+    // grammar structure (device)
+    const int2* kids;        // (left, right) per layered rule
+    const float* coeff;      // per layered rule, or nullptr when every coeff is 1
+    const int* tcol;         // column of compact terminal p (p >= 1)
+    const float* tval;       // value of compact terminal p (p >= 1)
+    const int2* emits;       // (pos, row) per emission, grouped by level, row-sorted
+    const int2* term_emits;  // (sym, row) per terminal occurrence, row-sorted
+    const int* crow;         // gather mode: row r's symbols are csym[crow[r], crow[r+1])
+    const int* csym;         // >= 0 global slot of a rule, < 0 terminal -c-1
+    const int* level_off;    // max_depth + 1
+    const int* emit_off;     // max_depth + 1
+    // values (device)
+    const float* X;          // cols x B (this tile's column base folded in)
+    void* Y;                 // rows x B
+    void* Tv;                // U x Bt terminal values
+    void* W0;                // frontier buffer 0 (or the resident value array)
+    void* W1;                // frontier buffer 1 (double-buffered layouts)
+    // sizes
+    int rows, cols, U, L;
+    int n_term;              // terminal emissions
+    int B;                   // row stride of X and Y
+    int Bt;                  // vectors in this tile (stride of T and W)
+    int G;                   // lanes per item
+    int resident;            // children index one resident array
+    int gather;              // emit by a final row gather
+};
+
+template <class S>
+__device__ __forceinline__ typename S::T* tp(void* p) {
+    return static_cast<typename S::T*>(p);
+}
+
+// T[p*Bt + b] = leaf(V, X[col*B + b]) (T[0..Bt) = zero); Y = zero for spot emits.
+template <class S>
+__device__ void ph_init(const SweepArgs& a, long tid, long nth) {
+    using T = typename S::T;
+    T* Tt = tp<S>(a.Tv);
+    const long n = static_cast<long>(a.U) * a.Bt;
+    for (long i = tid; i < n; i += nth) {
+        int p = static_cast<int>(i / a.Bt);
+        int b = static_cast<int>(i - static_cast<long>(p) * a.Bt);
+        Tt[i] = p == 0 ? S::zero() : S::leaf(a.tval[p], a.X[static_cast<long>(a.tcol[p]) * a.B + b]);
+    }
+    if (!a.gather) {
+        T* Y = tp<S>(a.Y);
+        const long ny = static_cast<long>(a.rows) * a.Bt;
+        for (long i = tid; i < ny; i += nth) {
+            long r = i / a.Bt;
+            Y[r * a.B + (i - r * a.Bt)] = S::zero();
+        }
+    }
+}
+
+// Rules [i0, i1) of one level. src is the previous frontier (or the resident
+// array), dst this level's frontier slice.
+template <class S>
+__device__ void ph_eval(const SweepArgs& a, int i0, int i1, const typename S::T* src,
+                        typename S::T* dst, long gid, long ng, int lane) {
+    using T = typename S::T;
+    const T* Tt = tp<S>(a.Tv);
+    const int Bt = a.Bt, G = a.G;
+    for (long i = i0 + gid; i < i1; i += ng) {
+        int2 c = a.kids[i];
+        float cf = a.coeff ? a.coeff[i] : 1.0f;
+        const T* lp = c.x >= 0 ? src + static_cast<long>(c.x) * Bt : Tt + static_cast<long>(-c.x - 1) * Bt;
+        const T* rp = c.y >= 0 ? src + static_cast<long>(c.y) * Bt : Tt + static_cast<long>(-c.y - 1) * Bt;
+        T* o = dst + (i - i0) * Bt;
+        for (int b = lane; b < Bt; b += G) o[b] = S::oplus(S::apply(lp[b], cf), rp[b]);
+    }
+}
+
+// Emit-on-the-spot for one vector: 64 consecutive (pos, row) pairs per wave,
+// pre-reduced by a segmented scan over equal rows (rows are sorted), then one
+// atomic per row run.
+template <class S>
+__device__ void ph_emit_wave(const int2* em, int n, const typename S::T* src, typename S::T* Y,
+                             long wid, long nw, int lane) {
+    using T = typename S::T;
+    for (long base = wid * 64; base < n; base += nw * 64) {
+        long e = base + lane;
+        int row = -1;
+        T v = S::zero();
+        if (e < n) {
+            int2 pr = em[e];
+            row = pr.y;
+            v = src[pr.x];
+        }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            T ov = __shfl_up(v, d, 64);
+            int orow = __shfl_up(row, d, 64);
+            if (lane >= d && orow == row) v = S::oplus(v, ov);
+        }
+        int nrow = __shfl_down(row, 1, 64);
+        bool tail = lane == 63 || nrow != row;
+        if (row >= 0 && tail && !S::skip(v)) S::emit(Y + row, v);
+    }
+}
+
+// Emit-on-the-spot, B-wide: a group takes CH consecutive emissions, keeps a
+// running sum per vector while the row repeats and issues one atomic per run.
+template <class S, int CH>
+__device__ void ph_emit_chunk(const int2* em, int n, const typename S::T* src, typename S::T* Y,
+                              int B, int Bt, int G, long gid, long ng, int lane) {
+    using T = typename S::T;
+    const long nch = (static_cast<long>(n) + CH - 1) / CH;
+    for (long ch = gid; ch < nch; ch += ng) {
+        const int e0 = static_cast<int>(ch * CH);
+        const int e1 = min(n, e0 + CH);
+        for (int b = lane; b < Bt; b += G) {
+            int row = -1;
+            T acc = S::zero();
+            for (int e = e0; e < e1; ++e) {
+                int2 pr = em[e];
+                T v = src[static_cast<long>(pr.x) * Bt + b];
+                if (pr.y != row) {
+                    if (row >= 0 && !S::skip(acc)) S::emit(Y + static_cast<long>(row) * B + b, acc);
+                    row = pr.y;
+                    acc = v;
+                } else {
+                    acc = S::oplus(acc, v);
+                }
+            }
+            if (row >= 0 && !S::skip(acc)) S::emit(Y + static_cast<long>(row) * B + b, acc);
+        }
+    }
+}
+
+constexpr int kEmitChunk = 8;
+
+template <class S>
+__device__ __forceinline__ void ph_emit(const SweepArgs& a, const int2* em, int n,
+                                        const typename S::T* src, long tid, long nth, int lane) {
+    if (a.Bt == 1 && a.B == 1) {
+        ph_emit_wave<S>(em, n, src, tp<S>(a.Y), tid / 64, nth / 64, static_cast<int>(tid & 63));
+    } else {
+        ph_emit_chunk<S, kEmitChunk>(em, n, src, tp<S>(a.Y), a.B, a.Bt, a.G, tid / a.G, nth / a.G, lane);
+    }
+}
+
+// Resident layout, gather emission: Y[r] = (+) over row r's top-sequence symbols.
+template <class S>
+__device__ void ph_gather(const SweepArgs& a, long gid, long ng, int lane) {
+    using T = typename S::T;
+    const T* W = tp<S>(a.W0);
+    const T* Tt = tp<S>(a.Tv);
+    T* Y = tp<S>(a.Y);
+    const int Bt = a.Bt;
+    for (long r = gid; r < a.rows; r += ng) {
+        const int s0 = a.crow[r], s1 = a.crow[r + 1];
+        for (int b = lane; b < Bt; b += a.G) {
+            T acc = S::zero();
+            for (int j = s0; j < s1; ++j) {
+                int c = a.csym[j];
+                T v = c >= 0 ? W[static_cast<long>(c) * Bt + b] : Tt[static_cast<long>(-c - 1) * Bt + b];
+                acc = S::oplus(acc, v);
+            }
+            Y[r * a.B + b] = acc;
+        }
+    }
+}
+
+// Frontier that level k writes (k >= 1) and the source it reads.
+template <class S>
+__device__ __forceinline__ typename S::T* level_dst(const SweepArgs& a, int k) {
+    using T = typename S::T;
+    if (a.resident) return tp<S>(a.W0) + static_cast<long>(a.level_off[k - 1]) * a.Bt;
+    return (k & 1) ? tp<S>(a.W1) : tp<S>(a.W0);
+}
+template <class S>
+__device__ __forceinline__ const typename S::T* level_src(const SweepArgs& a, int k) {
+    using T = typename S::T;
+    if (a.resident) return tp<S>(a.W0);
+    return (k & 1) ? tp<S>(a.W0) : tp<S>(a.W1);  // frontier k-1
+}
+
+// ---- one launch per phase (levels / graph variants) ----
+
+template <class S>
+__global__ void k_init(SweepArgs a) {
+    long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    ph_init<S>(a, tid, static_cast<long>(gridDim.x) * blockDim.x);
+}
+
+template <class S>
+__global__ void k_eval(SweepArgs a, int k) {
+    long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    long nth = static_cast<long>(gridDim.x) * blockDim.x;
+    ph_eval<S>(a, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k),
+               tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+}
+
+// k = 0: terminal emissions; k >= 1: level k's emissions.
+template <class S>
+__global__ void k_emit(SweepArgs a, int k) {
+    long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    long nth = static_cast<long>(gridDim.x) * blockDim.x;
+    int lane = static_cast<int>(tid % a.G);
+    if (k == 0) {
+        ph_emit<S>(a, a.term_emits, a.n_term, tp<S>(a.Tv), tid, nth, lane);
+    } else {
+        const int e0 = a.emit_off[k - 1], e1 = a.emit_off[k];
+        // Resident positions are global slots; double-buffered ones are local.
+        const typename S::T* src = a.resident ? tp<S>(a.W0) : level_dst<S>(a, k);
+        ph_emit<S>(a, a.emits + e0, e1 - e0, src, tid, nth, lane);
+    }
+}
+
+template <class S>
+__global__ void k_gather(SweepArgs a) {
+    long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    long nth = static_cast<long>(gridDim.x) * blockDim.x;
+    ph_gather<S>(a, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+}
+
+// ---- persistent variant: the whole sweep in one launch ----
+//
+// Grid-wide barrier of our own. Every workgroup's first thread publishes the
+// workgroup's writes (agent-scope release), adds one arrival to a monotonic
+// 64-bit counter and waits for the counter to reach `target` (all workgroups of
+// this barrier), then acquires. Every wait is bounded: after `deadline` ticks
+// of the 100 MHz wall clock, or as soon as another workgroup has given up, it
+// stops waiting and sets the sticky error word, so the kernel always ends.
+struct GridBarrier {
+    unsigned long long arrived;  // monotonic across launches
+    int error;                   // 1 = a wait timed out (results invalid)
+    int pad;
+};
+
+__device__ __forceinline__ void grid_barrier(GridBarrier* gb, unsigned long long target,
+                                             unsigned long long deadline) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(&gb->arrived, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long t0 = wall_clock64();
+        while (__hip_atomic_load(&gb->arrived, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(&gb->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            if (wall_clock64() - t0 > deadline) {
+                __hip_atomic_store(&gb->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+// Phases (L + 1 barriers per launch):
+//   phase 0        T (and Y for spot emission) initialised            | barrier
+//   phase k        level k; spot: also level k-1's emits (terminal    | barrier
+//                  emits for k = 1) -- both only read frontier k-1
+//   after level L  spot: level L's emits; gather: the row gather
+// `base` is the arrival count before this launch.
+template <class S>
+__global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long base,
+                             unsigned long long deadline) {
+    const long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long nth = static_cast<long>(gridDim.x) * blockDim.x;
+    const int lane = static_cast<int>(tid % a.G);
+    const long gid = tid / a.G, ng = nth / a.G;
+    const unsigned long long nb = gridDim.x;
+    ph_init<S>(a, tid, nth);
+    grid_barrier(gb, base + nb, deadline);
+    for (int k = 1; k <= a.L; ++k) {
+        if (!a.gather) {
+            if (k == 1) {
+                ph_emit<S>(a, a.term_emits, a.n_term, tp<S>(a.Tv), tid, nth, lane);
+            } else {
+                const int e0 = a.emit_off[k - 2], e1 = a.emit_off[k - 1];
+                const typename S::T* src = a.resident ? tp<S>(a.W0) : level_dst<S>(a, k - 1);
+                ph_emit<S>(a, a.emits + e0, e1 - e0, src, tid, nth, lane);
+            }
+        }
+        ph_eval<S>(a, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k), gid, ng,
+                   lane);
+        grid_barrier(gb, base + nb * static_cast<unsigned long long>(k + 1), deadline);
+    }
+    if (a.gather) {
+        ph_gather<S>(a, gid, ng, lane);
+    } else if (a.L == 0) {
+        ph_emit<S>(a, a.term_emits, a.n_term, tp<S>(a.Tv), tid, nth, lane);
+    } else {
+        const int e0 = a.emit_off[a.L - 1], e1 = a.emit_off[a.L];
+        const typename S::T* src = a.resident ? tp<S>(a.W0) : level_dst<S>(a, a.L);
+        ph_emit<S>(a, a.emits + e0, e1 - e0, src, tid, nth, lane);
+    }
+}
+
+}  // namespace dev
+}  // namespace gmr

@@ -1,0 +1,306 @@
+"""Python mirror of the engine's host interface, over its C ABI.
+
+The reference's interface for this path is C++ (gpu-engine/gpu_engine.h,
+grammar.h, layered_types.h, build_schedule.h); that C++ API is provided as-is by
+the headers in include/ and the two shared libraries. This module is the thin
+Python harness over the same C ABI that the tests and the bench drive:
+
+  Matrix    a CSRV matrix (seeded generators, dense input)          gmr_matrix_*
+  Grammar   a RePair grammar (builder, .val/.vc.R/.vc.C files)      gmr_grammar_*
+  Schedule  the level schedule (RefCompat / Wave64 / Resident)      gmr_schedule_*
+  Engine    the device-resident grammar and its HIP sweep           gmr_engine_*
+
+Nothing here computes y on the CPU: Engine.run is the HIP path, and it raises
+if the device library is missing.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import GPUSchedule, LayeredRule, RunOpts
+
+PLUS_TIMES, BOOLEAN, TROPICAL = 0, 1, 2
+SEMIRINGS = {"plustimes": PLUS_TIMES, "boolean": BOOLEAN, "bool": BOOLEAN,
+             "tropical": TROPICAL, "minplus": TROPICAL}
+REFCOMPAT, WAVE64, RESIDENT = 0, 1, 2
+LAYOUTS = {"refcompat": REFCOMPAT, "wave64": WAVE64, "resident": RESIDENT}
+
+# Kernel variants (gmr.h gmr_variant) and emission strategies (gmr_emit).
+VARIANT_LEVELS, VARIANT_GRAPH, VARIANT_PERSISTENT = 0, 1, 2
+EMIT_SPOT, EMIT_GATHER = 0, 1
+VARIANTS = {"levels": VARIANT_LEVELS, "graph": VARIANT_GRAPH, "persistent": VARIANT_PERSISTENT}
+EMITS = {"spot": EMIT_SPOT, "gather": EMIT_GATHER}
+
+TROPICAL_INF = 1000000000
+
+
+class GmrError(RuntimeError):
+    pass
+
+
+def _check(rc, lib=None):
+    if rc != 0:
+        msg = _lib.host().gmr_last_error()
+        raise GmrError(f"gmr error {rc}: {msg.decode() if msg else ''}")
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Matrix:
+    """A matrix as its CSRV string (row terminals, then 0)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    @classmethod
+    def haplotype(cls, seed, rows, cols, founders=64, density=0.08, seg_mean=128.0, flip=0.01):
+        h = C.c_void_p()
+        _check(_lib.host().gmr_matrix_haplotype(seed, rows, cols, founders, density, seg_mean, flip, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def genotype(cls, seed, rows, cols, founders=256, classes=16, seg_mean=512.0):
+        h = C.c_void_p()
+        _check(_lib.host().gmr_matrix_genotype(seed, rows, cols, founders, classes, seg_mean, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, seed, scale, edges, a=0.57, b=0.19, c=0.19, nweights=1):
+        h = C.c_void_p()
+        _check(_lib.host().gmr_matrix_rmat(seed, scale, edges, a, b, c, nweights, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_dense(cls, M):
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        h = C.c_void_p()
+        _check(_lib.host().gmr_matrix_from_dense(_ptr(M, C.c_double), M.shape[0], M.shape[1], C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_csrv(cls, rows, cols, seq, V):
+        seq = np.ascontiguousarray(seq, dtype=np.int32)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        h = C.c_void_p()
+        _check(_lib.host().gmr_matrix_from_csrv(rows, cols, _ptr(seq, C.c_int), len(seq), _ptr(V, C.c_double), len(V), C.byref(h)))
+        return cls(h)
+
+    def info(self):
+        r, c, nv, al = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        ln = C.c_longlong()
+        _check(_lib.host().gmr_matrix_info(self._h, C.byref(r), C.byref(c), C.byref(nv), C.byref(al), C.byref(ln)))
+        return dict(rows=r.value, cols=c.value, nvals=nv.value, alpha=al.value, len=ln.value)
+
+    @property
+    def rows(self):
+        return self.info()["rows"]
+
+    @property
+    def cols(self):
+        return self.info()["cols"]
+
+    def arrays(self):
+        """(seq int32, V float64)."""
+        i = self.info()
+        seq = np.empty(i["len"], dtype=np.int32)
+        V = np.empty(i["nvals"], dtype=np.float64)
+        _check(_lib.host().gmr_matrix_copy(self._h, _ptr(seq, C.c_int), _ptr(V, C.c_double)))
+        return seq, V
+
+    def nnz(self):
+        i = self.info()
+        return i["len"] - i["rows"]
+
+    def repair(self, max_rounds=0, threads=0):
+        h = C.c_void_p()
+        rounds, secs = C.c_int(), C.c_double()
+        _check(_lib.host().gmr_grammar_repair(self._h, max_rounds, threads, C.byref(h), C.byref(rounds), C.byref(secs)))
+        g = Grammar(h)
+        g.build_rounds, g.build_seconds = rounds.value, secs.value
+        return g
+
+    def __del__(self):
+        lib = getattr(_lib, "_host", None) if _lib is not None else None
+        if getattr(self, "_h", None) and lib is not None:
+            lib.gmr_matrix_free(self._h)
+            self._h = None
+
+
+class Grammar:
+    """A grammar: alpha, rules (kind, left, right, rep), top sequence C, values V."""
+
+    build_rounds = 0
+    build_seconds = 0.0
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    @classmethod
+    def from_arrays(cls, alpha, left, right, C_seq, V, kind=None, rep=None):
+        left = np.ascontiguousarray(left, dtype=np.int32)
+        right = np.ascontiguousarray(right, dtype=np.int32)
+        n = len(left)
+        kind = np.zeros(n, np.int32) if kind is None else np.ascontiguousarray(kind, dtype=np.int32)
+        rep = np.ones(n, np.int32) if rep is None else np.ascontiguousarray(rep, dtype=np.int32)
+        C_seq = np.ascontiguousarray(C_seq, dtype=np.int32)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        h = C.c_void_p()
+        _check(_lib.host().gmr_grammar_from_arrays(
+            alpha, n, _ptr(kind, C.c_int), _ptr(left, C.c_int), _ptr(right, C.c_int), _ptr(rep, C.c_int),
+            _ptr(C_seq, C.c_int), len(C_seq), _ptr(V, C.c_double), len(V), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def read(cls, base):
+        h = C.c_void_p()
+        _check(_lib.host().gmr_grammar_read(str(base).encode(), C.byref(h)))
+        return cls(h)
+
+    def write(self, base):
+        _check(_lib.host().gmr_grammar_write(self._h, str(base).encode()))
+
+    def info(self):
+        a, n, nv = C.c_int(), C.c_int(), C.c_int()
+        cl = C.c_longlong()
+        _check(_lib.host().gmr_grammar_info(self._h, C.byref(a), C.byref(n), C.byref(cl), C.byref(nv)))
+        return dict(alpha=a.value, nrules=n.value, clen=cl.value, nvals=nv.value)
+
+    def arrays(self):
+        i = self.info()
+        n = i["nrules"]
+        kind, left, right, rep = (np.empty(n, np.int32) for _ in range(4))
+        Cs = np.empty(i["clen"], np.int32)
+        V = np.empty(i["nvals"], np.float64)
+        _check(_lib.host().gmr_grammar_copy(self._h, _ptr(kind, C.c_int), _ptr(left, C.c_int), _ptr(right, C.c_int),
+                                            _ptr(rep, C.c_int), _ptr(Cs, C.c_int), _ptr(V, C.c_double)))
+        return dict(alpha=i["alpha"], kind=kind, left=left, right=right, rep=rep, C=Cs, V=V)
+
+    def expand(self):
+        n = C.c_longlong()
+        _check(_lib.host().gmr_grammar_expand(self._h, None, 0, C.byref(n)))
+        out = np.empty(n.value, np.int32)
+        _check(_lib.host().gmr_grammar_expand(self._h, _ptr(out, C.c_int), n.value, C.byref(n)))
+        return out
+
+    def schedule(self, rows, cols, layout=REFCOMPAT):
+        if isinstance(layout, str):
+            layout = LAYOUTS[layout]
+        h = C.c_void_p()
+        _check(_lib.host().gmr_schedule_build(self._h, rows, cols, layout, C.byref(h)))
+        s = Schedule(h)
+        s.V = self.arrays()["V"]
+        return s
+
+    def __del__(self):
+        lib = getattr(_lib, "_host", None) if _lib is not None else None
+        if getattr(self, "_h", None) and lib is not None:
+            lib.gmr_grammar_free(self._h)
+            self._h = None
+
+
+class Schedule:
+    """A built level schedule (owns the BuiltSchedule)."""
+
+    V = None
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    def info(self):
+        a = (C.c_longlong * 11)()
+        _check(_lib.host().gmr_schedule_info(self._h, a))
+        keys = ["max_depth", "max_width", "num_raw_nt", "num_layered", "num_passthrough",
+                "emit_total", "num_term_emit", "U", "rows", "cols", "layout"]
+        return dict(zip(keys, list(a)))
+
+    def arrays(self):
+        i = self.info()
+        L = i["max_depth"]
+        rules = (LayeredRule * max(1, i["num_layered"]))()
+        lo, lw, eo = (np.empty(L + 1, np.int32) for _ in range(3))
+        ep, er = np.empty(i["emit_total"], np.int32), np.empty(i["emit_total"], np.int32)
+        ts, tr = np.empty(i["num_term_emit"], np.int32), np.empty(i["num_term_emit"], np.int32)
+        to = np.empty(i["U"], np.int32)
+        _check(_lib.host().gmr_schedule_copy(self._h, rules, _ptr(lo, C.c_int), _ptr(lw, C.c_int), _ptr(eo, C.c_int),
+                                             _ptr(ep, C.c_int), _ptr(er, C.c_int), _ptr(ts, C.c_int), _ptr(tr, C.c_int),
+                                             _ptr(to, C.c_int)))
+        rl = np.frombuffer(rules, dtype=np.dtype([("left", "<i4"), ("right", "<i4"), ("coeff", "<f4")]),
+                           count=i["num_layered"]).copy()
+        return dict(left=rl["left"].copy(), right=rl["right"].copy(), coeff=rl["coeff"].copy(),
+                    level_offsets=lo, level_widths=lw, emit_offsets=eo, emit_pos=ep, emit_row=er,
+                    term_sym=ts, term_row=tr, term_orig=to, info=i)
+
+    def engine(self, V=None):
+        """Upload to the GPU (device-resident engine)."""
+        V = self.V if V is None else V
+        Vf = np.ascontiguousarray(V, dtype=np.float32)
+        h = C.c_void_p()
+        _check(_lib.dev().gmr_engine_from_schedule(self._h, _ptr(Vf, C.c_float), len(Vf), C.byref(h)))
+        return Engine(h)
+
+    def __del__(self):
+        lib = getattr(_lib, "_host", None) if _lib is not None else None
+        if getattr(self, "_h", None) and lib is not None:
+            lib.gmr_schedule_free(self._h)
+            self._h = None
+
+
+class Engine:
+    """A device-resident grammar with its HIP sweep."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    def info(self):
+        a = (C.c_longlong * 16)()
+        _check(_lib.dev().gmr_engine_info(self._h, a))
+        keys = ["rows", "cols", "U", "num_layered", "max_depth", "max_width", "emit_total",
+                "num_term_emit", "layout", "has_runlen", "device_bytes", "nvals",
+                "clen_rows", "r0", "r1", "r2"]
+        return dict(zip(keys, list(a)))
+
+    def run(self, semiring, X_ptr, B, Y_ptr, variant=VARIANT_LEVELS, emit=EMIT_SPOT, b_tile=0,
+            block=0, grid_cap=0, stream=0):
+        """Y = M X on device pointers: X is cols x B (X[c*B+b]), Y is rows x B.
+
+        semiring: 0 (+,x) float, 1 Boolean int32, 2 Tropical int32 (Y holds int
+        bits; X is float as in the reference API)."""
+        if isinstance(semiring, str):
+            semiring = SEMIRINGS[semiring]
+        o = RunOpts(variant, emit, b_tile, block, grid_cap)
+        _check(_lib.dev().gmr_engine_run(self._h, semiring, C.c_void_p(X_ptr), B, C.c_void_p(Y_ptr),
+                                         C.byref(o), C.c_void_p(stream)))
+
+    def check(self):
+        """Raise if a persistent launch's grid barrier timed out since the last
+        check (its results are invalid). Synchronises the device."""
+        err = C.c_int()
+        _check(_lib.dev().gmr_engine_check(self._h, C.byref(err)))
+        if err.value:
+            raise GmrError("persistent sweep: a grid-barrier wait timed out; results are invalid")
+
+    def run_torch(self, semiring, X, Y=None, **kw):
+        """Convenience for torch device tensors: X float32 (cols, B) or (cols,)."""
+        import torch
+        if isinstance(semiring, str):
+            semiring = SEMIRINGS[semiring]
+        assert X.is_cuda and X.dtype == torch.float32 and X.is_contiguous()
+        B = 1 if X.dim() == 1 else X.shape[1]
+        i = self.info()
+        if X.shape[0] != i["cols"]:
+            raise ValueError(f"X has {X.shape[0]} rows, the matrix has {i['cols']} columns")
+        dt = torch.float32 if semiring == PLUS_TIMES else torch.int32
+        if Y is None:
+            Y = torch.empty((i["rows"], B) if X.dim() == 2 else (i["rows"],), dtype=dt, device=X.device)
+        stream = torch.cuda.current_stream(X.device).cuda_stream
+        self.run(semiring, X.data_ptr(), B, Y.data_ptr(), stream=stream, **kw)
+        return Y
+
+    def __del__(self):
+        lib = getattr(_lib, "_dev", None) if _lib is not None else None
+        if getattr(self, "_h", None) and lib is not None:
+            lib.gmr_engine_destroy(self._h)
+            self._h = None

@@ -1,0 +1,82 @@
+"""Exhaustive schedule autotuner.
+
+For one engine, semiring and batch width B, every candidate launch schedule
+(kernel variant x emission strategy x vector tile x block size) is timed on
+the device with HIP events, and the fastest is kept. Schedules never change the
+arithmetic, so every candidate computes the same Y (FP32 sums may associate
+differently; Boolean and Tropical are identical). Timings are cached on disk
+keyed by a hash of the grammar shape, so a repeated run skips the search.
+"""
+import hashlib
+import json
+import os
+
+from . import EMIT_GATHER, EMIT_SPOT, RESIDENT, VARIANT_GRAPH, VARIANT_LEVELS, VARIANT_PERSISTENT
+
+VARIANT_NAMES = {VARIANT_LEVELS: "levels", VARIANT_GRAPH: "graph", VARIANT_PERSISTENT: "persistent"}
+EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
+
+
+def candidates(B, layout, blocks=(256,)):
+    """The search space for one engine and batch width."""
+    emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
+    tiles = sorted({B} | {t for t in (16, 32, 64) if t < B})
+    out = []
+    for v in (VARIANT_LEVELS, VARIANT_GRAPH, VARIANT_PERSISTENT):
+        for e in emits:
+            for t in tiles:
+                for blk in blocks:
+                    out.append(dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk))
+    return out
+
+
+def describe(c):
+    return f"{VARIANT_NAMES[c['variant']]}/{EMIT_NAMES[c['emit']]}/tile={c['b_tile'] or 'B'}/blk={c['block']}"
+
+
+def shape_key(info, B, semiring):
+    s = json.dumps({k: info[k] for k in sorted(info)}, sort_keys=True) + f"|B={B}|sr={semiring}"
+    return hashlib.sha1(s.encode()).hexdigest()[:16]
+
+
+def time_candidate(engine, semiring, X, Y, cand, reps=3, warm=1):
+    """Mean device milliseconds of one sweep under `cand` (torch events on the
+    current stream, which is where the engine launches)."""
+    import torch
+    for _ in range(warm):
+        engine.run_torch(semiring, X, Y, **cand)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        engine.run_torch(semiring, X, Y, **cand)
+    e1.record()
+    torch.cuda.synchronize()
+    if cand["variant"] == VARIANT_PERSISTENT:
+        engine.check()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune(engine, semiring, X, Y, cands, cache_path=None, reps=3, log=None):
+    """Time every candidate; return (best candidate, [(cand, ms), ...])."""
+    info = engine.info()
+    B = 1 if X.dim() == 1 else X.shape[1]
+    key = shape_key(info, B, semiring)
+    db = {}
+    if cache_path and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            db = json.load(f)
+    results = []
+    for c in cands:
+        ms = time_candidate(engine, semiring, X, Y, c, reps=reps)
+        results.append((c, ms))
+        if log:
+            log(f"  tune {describe(c):40s} {ms * 1e3:9.1f} us")
+    best = min(results, key=lambda r: r[1])[0]
+    if cache_path:
+        db[key] = [dict(c, ms=ms) for c, ms in results]
+        os.makedirs(os.path.dirname(os.path.abspath(cache_path)), exist_ok=True)
+        with open(cache_path, "w") as f:
+            json.dump(db, f)
+    return best, results

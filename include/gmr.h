@@ -1,0 +1,86 @@
+/* gmr.h -- C ABI of the device engine: a grammar made resident in HBM once and
+ * swept many times by hand-written HIP kernels for gfx950.
+ *
+ * Replaces the inside of the reference's run_gpu_right_multiply_layered /
+ * run_gpu_right_multiply_batched (gpu-engine/gpu_engine.cu:135-587), which
+ * allocate, upload, sweep and free on every call. Here the upload
+ * (gmr_engine_create) and the sweep (gmr_engine_run) are separate, the
+ * semiring is an argument instead of the SEMIRING environment variable
+ * (gpu_engine.cu:379-396), inputs and outputs are device pointers on a caller
+ * stream, and failures return a status instead of exit(1) (gpu_engine.cu:38-46).
+ * The drop-in entry points with the reference's exact signatures are declared
+ * in gpu_engine.h and are built on this ABI.
+ *
+ * Value layout (reference gpu_engine.h:116-120): X is cols x B with X[c*B + b],
+ * Y is rows x B with Y[r*B + b]. X is float for every semiring (Boolean and
+ * Tropical read it as the reference does: x != 0, (int)x); Y holds float for
+ * (+,x) and int32 for Boolean and Tropical.
+ */
+#ifndef GMR_H
+#define GMR_H
+
+#include "gmr_host.h"
+#include "layered_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gmr_semiring { GMR_PLUS_TIMES = 0, GMR_BOOLEAN = 1, GMR_TROPICAL = 2 };
+
+/* How a sweep is launched. */
+enum gmr_variant {
+    GMR_VARIANT_LEVELS = 0,     /* one launch per phase and level (the plain sweep) */
+    GMR_VARIANT_GRAPH = 1,      /* the same launches replayed from a captured hipGraph */
+    GMR_VARIANT_PERSISTENT = 2  /* one cooperative launch, grid barrier between levels */
+};
+
+/* How top-sequence symbols reach y. */
+enum gmr_emit {
+    GMR_EMIT_SPOT = 0,   /* emit-on-the-spot: atomics right after each level */
+    GMR_EMIT_GATHER = 1  /* Resident layout only: one row-ordered gather after the sweep */
+};
+
+typedef struct gmr_run_opts {
+    int variant;   /* gmr_variant */
+    int emit;      /* gmr_emit */
+    int b_tile;    /* vectors per pass over the grammar (0: all B at once) */
+    int block;     /* threads per workgroup (0: 256) */
+    int grid_cap;  /* persistent variant: max workgroups (0: full occupancy) */
+    int reserved[3];
+} gmr_run_opts;
+
+typedef struct gmr_engine gmr_engine;
+
+/* Upload a layered schedule. `layout` says how children are numbered:
+ * GMR_LAYOUT_REFCOMPAT / GMR_LAYOUT_WAVE64 -- slots of the previous level;
+ * GMR_LAYOUT_RESIDENT -- global slots of earlier levels. val_idx / col_idx
+ * have U-1 entries (compact terminal q >= 1 is value V[val_idx[q-1]] in column
+ * col_idx[q-1]). Every index is range-checked on the host before upload. */
+int gmr_engine_create(int rows, int cols, int U, int num_rules, const LayeredRule* rules,
+                      const int* val_idx, const int* col_idx, const float* V, int nvals,
+                      const GPUSchedule* sched, int layout, gmr_engine** out);
+/* Upload a schedule built by gmr_schedule_build. */
+int gmr_engine_from_schedule(const gmr_schedule* s, const float* V, int nvals, gmr_engine** out);
+/* info[0..11] = rows, cols, U, num_layered, max_depth, max_width, emit_total,
+ * num_term_emit, layout, has_runlen, device bytes (structure), nvals. */
+int gmr_engine_info(const gmr_engine* e, long long info[16]);
+/* Y = M X on `stream` (a hipStream_t; NULL = default stream). Asynchronous:
+ * returns after enqueueing. opts may be NULL (levels variant, spot emit). */
+int gmr_engine_run(gmr_engine* e, int semiring, const void* d_X, int B, void* d_Y,
+                   const gmr_run_opts* opts, void* stream);
+/* Persistent variant health check: *barrier_error = 1 if a grid-barrier wait
+ * of an earlier persistent launch on this engine timed out (its results are
+ * invalid; the barrier has been reset), else 0. Synchronises the device. */
+int gmr_engine_check(gmr_engine* e, int* barrier_error);
+/* Device time of the last gmr_engine_run on this engine, when timing was
+ * requested with opts->reserved[0] = 1 (synchronises the stream). */
+int gmr_engine_last_kernel_ms(const gmr_engine* e, double* ms);
+void gmr_engine_destroy(gmr_engine* e);
+int gmr_device_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GMR_H */

@@ -1,0 +1,244 @@
+"""GPU parity: the HIP sweep against the oracle (FP64 brute force from the
+uncompressed matrix) through the C ABI, for every semiring, layout, launch
+variant, emission strategy and batch width. Bar: Boolean and Tropical
+bit-exact; (+,x) max_i |y_i - y*_i| / sum_j |M_ij x_j| <= 1e-5."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import gmr
+from gmr import _lib
+from helpers import fixture, grammar_dict, rel_err, x_for
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+torch = pytest.importorskip("torch")
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    _lib.dev()  # the HIP library must load; no fallback exists
+
+
+def run_engine(eng, sr, X, **kw):
+    Xt = torch.from_numpy(np.ascontiguousarray(X, np.float32)).cuda()
+    Y = eng.run_torch(sr, Xt, **kw)
+    torch.cuda.synchronize()
+    if kw.get("variant") == gmr.VARIANT_PERSISTENT:
+        eng.check()  # no grid-barrier wait may have timed out
+    return Y.cpu().numpy()
+
+
+def check(sr, y, y_star, den, what):
+    if sr == 0:
+        e = rel_err(y, y_star, den)
+        _say(f"  {what}: err={e:.3g}")
+        assert e <= TOL, f"{what}: FP32 error {e} > {TOL}"
+    else:
+        ok = np.array_equal(np.asarray(y, np.int64), np.asarray(y_star, np.int64))
+        assert ok, f"{what}: not bit-exact ({int((np.asarray(y, np.int64) != y_star).sum())} rows differ)"
+
+
+CASES = [("levels", "spot"), ("graph", "spot"), ("persistent", "spot")]
+RES_CASES = CASES + [("levels", "gather"), ("graph", "gather"), ("persistent", "gather")]
+
+
+@pytest.fixture(scope="module")
+def hap():
+    m = gmr.Matrix.haplotype(20261204, 4096, 2048)
+    g = m.repair()
+    seq, V = m.arrays()
+    return m, g, seq, V
+
+
+def test_unit_grammar_on_gpu():
+    f = fixture("unit_grammar")
+    g = gmr.Grammar.from_arrays(f["alpha"], f["left"], f["right"], f["C"], f["V"], kind=f["kind"], rep=f["rep"])
+    x = np.array(f["x"], np.float32)
+    for lay in ("refcompat", "wave64", "resident"):
+        eng = g.schedule(2, 3, lay).engine()
+        for var, em in (RES_CASES if lay == "resident" else CASES):
+            _say("unit", lay, var, em)
+            y = run_engine(eng, 0, x, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+            np.testing.assert_allclose(y, f["expect_y"], atol=1e-6)
+
+
+def test_readme_example_on_gpu():
+    f = fixture("readme_8x7")
+    m = gmr.Matrix.from_dense(np.array(f["M"]))
+    g = m.repair()
+    for lay in ("refcompat", "resident"):
+        eng = g.schedule(8, 7, lay).engine()
+        y = run_engine(eng, 0, np.array(f["x"], np.float32))
+        np.testing.assert_allclose(y, f["expect_y"], rtol=1e-6)
+
+
+@pytest.mark.parametrize("sr", [0, 1, 2])
+@pytest.mark.parametrize("lay", ["refcompat", "wave64", "resident"])
+def test_haplotype_single_vector(hap, sr, lay):
+    m, g, seq, V = hap
+    eng = g.schedule(4096, 2048, lay).engine()
+    x = x_for(sr, 2048, 100 + sr)
+    y_star, den = O.brute_force(sr, 4096, 2048, seq, V, x)
+    for var, em in (RES_CASES if lay == "resident" else CASES):
+        _say("hap B=1", sr, lay, var, em)
+        y = run_engine(eng, sr, x, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+        check(sr, y, y_star, den, f"{lay}/{var}/{em}")
+
+
+@pytest.mark.parametrize("B", [4, 16, 64, 256, 3, 100])
+def test_haplotype_batched_distinct_columns(hap, B):
+    m, g, seq, V = hap
+    X = x_for(0, 2048, 7 + B, B=B)
+    Y_star, den = O.brute_force_batched(0, 4096, 2048, seq, V, X)
+    for lay in ("wave64", "resident"):
+        eng = g.schedule(4096, 2048, lay).engine()
+        for var, em in (RES_CASES if lay == "resident" else CASES):
+            _say("hap B", B, lay, var, em)
+            Y = run_engine(eng, 0, X, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+            check(0, Y, Y_star, den, f"B={B} {lay}/{var}/{em}")
+
+
+@pytest.mark.parametrize("sr", [1, 2])
+def test_batched_integer_semirings(hap, sr):
+    m, g, seq, V = hap
+    X = x_for(sr, 2048, 55, B=16)
+    Y_star, _ = O.brute_force_batched(sr, 4096, 2048, seq, V, X)
+    eng = g.schedule(4096, 2048, "resident").engine()
+    for var, em in RES_CASES:
+        _say("int batched", sr, var, em)
+        Y = run_engine(eng, sr, X, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+        check(sr, Y, Y_star, None, f"sr={sr} {var}/{em}")
+
+
+def test_b_tiles_match_untiled(hap):
+    m, g, seq, V = hap
+    X = x_for(0, 2048, 77, B=64)
+    Y_star, den = O.brute_force_batched(0, 4096, 2048, seq, V, X)
+    eng = g.schedule(4096, 2048, "resident").engine()
+    for bt in (8, 16, 24):
+        for var in ("levels", "persistent"):
+            _say("b_tile", bt, var)
+            Y = run_engine(eng, 0, X, variant=gmr.VARIANTS[var], emit=gmr.EMIT_GATHER, b_tile=bt)
+            check(0, Y, Y_star, den, f"b_tile={bt} {var}")
+
+
+def test_graphs_replay_consistently(hap):
+    """A captured graph replayed several times, with eager work between the
+    replays, gives the eager result every time."""
+    m, g, seq, V = hap
+    eng = g.schedule(4096, 2048, "resident").engine()
+    X = x_for(0, 2048, 9, B=4)
+    eager = run_engine(eng, 0, X)
+    for i in range(4):
+        y = run_engine(eng, 0, X, variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_SPOT if i % 2 else gmr.EMIT_GATHER)
+        _ = run_engine(eng, 0, X)  # eager work in between
+        Y_star, den = O.brute_force_batched(0, 4096, 2048, seq, V, X)
+        check(0, y, Y_star, den, f"replay {i}")
+        assert rel_err(y, eager, den) <= TOL
+
+
+def test_rmat_graph_semirings():
+    for nw, sr in ((1, 1), (100, 2)):
+        m = gmr.Matrix.rmat(2026 + nw, 14, 200000, nweights=nw)
+        g = m.repair()
+        seq, V = m.arrays()
+        n = m.rows
+        if sr == 1:
+            x = (np.random.default_rng(1).random(n) < 0.01).astype(np.float32)
+        else:
+            x = np.full(n, np.float32(2 ** 31 - 1) // 2, np.float32)
+            x[np.random.default_rng(2).choice(n, 64, replace=False)] = 0
+        y_star, _ = O.brute_force(sr, n, n, seq, V, x)
+        for lay in ("wave64", "resident"):
+            eng = g.schedule(n, n, lay).engine()
+            for var, em in (RES_CASES if lay == "resident" else CASES):
+                _say("rmat", sr, lay, var, em)
+                y = run_engine(eng, sr, x, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+                check(sr, y, y_star, None, f"rmat sr={sr} {lay}/{var}/{em}")
+
+
+def test_empty_rows_and_terminal_only_grammar():
+    g = gmr.Grammar.from_arrays(4, [], [], [1, 2, 0, 0, 3, 0], [1.0])
+    x = np.array([1.0, 2.0, 4.0], np.float32)
+    for lay in ("refcompat", "resident"):
+        eng = g.schedule(3, 3, lay).engine()
+        for var, em in (RES_CASES if lay == "resident" else CASES):
+            y = run_engine(eng, 0, x, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+            np.testing.assert_array_equal(y, [3.0, 0.0, 4.0])
+            yt = run_engine(eng, 2, x, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+            np.testing.assert_array_equal(yt, [2, 1000000000, 5])
+
+
+def test_run_errors_are_reported(hap):
+    m, g, seq, V = hap
+    eng = g.schedule(4096, 2048, "wave64").engine()
+    X = torch.zeros(2048, dtype=torch.float32, device="cuda")
+    with pytest.raises(gmr.GmrError, match="Resident"):
+        eng.run_torch(0, X, emit=gmr.EMIT_GATHER)
+    with pytest.raises(gmr.GmrError, match="semiring"):
+        eng.run(7, X.data_ptr(), 1, X.data_ptr())
+    with pytest.raises(gmr.GmrError, match="B must"):
+        eng.run(0, X.data_ptr(), 0, X.data_ptr())
+
+
+def _dropin(batched, sched_arrays, rows, cols, V, X, B, semiring_env, monkeypatch):
+    monkeypatch.setenv("SEMIRING", semiring_env)
+    L = _lib.dev()
+    s = sched_arrays
+    to = s["term_orig"]
+    val_idx = np.ascontiguousarray((to[1:] - 1) // cols, np.int32)
+    col_idx = np.ascontiguousarray((to[1:] - 1) % cols, np.int32)
+    n = len(s["left"])
+    rules = (_lib.LayeredRule * max(n, 1))()
+    for i in range(n):
+        rules[i] = _lib.LayeredRule(int(s["left"][i]), int(s["right"][i]), float(s["coeff"][i]))
+    keep = []
+
+    def ip(a):
+        a = np.ascontiguousarray(a, np.int32)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(C.c_int))
+    gs = _lib.GPUSchedule(len(s["level_offsets"]) - 1, int(s["info"]["max_width"]), ip(s["level_offsets"]),
+                          ip(s["level_widths"]), ip(s["emit_offsets"]), ip(s["emit_pos"]), ip(s["emit_row"]),
+                          len(s["emit_pos"]), ip(s["term_sym"]), ip(s["term_row"]), len(s["term_sym"]))
+    Vf = np.ascontiguousarray(V, np.float32)
+    Xf = np.ascontiguousarray(X, np.float32)
+    Y = np.zeros(rows * B, np.float32)
+    fp = C.POINTER(C.c_float)
+    if batched:
+        L.run_gpu_right_multiply_batched(rows, cols, len(to), n, rules, ip(val_idx), ip(col_idx),
+                                         Vf.ctypes.data_as(fp), len(Vf), Xf.ctypes.data_as(fp), C.byref(gs),
+                                         Y.ctypes.data_as(fp), B, 2)
+    else:
+        L.run_gpu_right_multiply_layered(rows, cols, len(to), n, rules, ip(val_idx), ip(col_idx),
+                                         Vf.ctypes.data_as(fp), len(Vf), Xf.ctypes.data_as(fp), C.byref(gs),
+                                         Y.ctypes.data_as(fp), 2)
+    return Y if semiring_env == "plustimes" else Y.view(np.int32)
+
+
+def test_dropin_entry_points(hap, monkeypatch):
+    m, g, seq, V = hap
+    s = g.schedule(4096, 2048, "refcompat").arrays()
+    for sr, env in ((0, "plustimes"), (1, "boolean"), (2, "tropical")):
+        x = x_for(sr, 2048, 300 + sr)
+        y_star, den = O.brute_force(sr, 4096, 2048, seq, V, x)
+        _say("dropin layered", env)
+        y = _dropin(False, s, 4096, 2048, V, x, 1, env, monkeypatch)
+        check(sr, y, y_star, den, f"dropin {env}")
+    X = x_for(0, 2048, 400, B=8)
+    Y_star, den = O.brute_force_batched(0, 4096, 2048, seq, V, X)
+    _say("dropin batched")
+    Y = _dropin(True, s, 4096, 2048, V, X.reshape(-1), 8, "plustimes", monkeypatch).reshape(4096, 8)
+    check(0, Y, Y_star, den, "dropin batched")

@@ -1,0 +1,225 @@
+"""Host side on the CPU: seeded generators, the RePair builder, the grammar
+files and the three schedule layouts, checked against the oracle and, where it
+was built from the reference's sources, the reference's own scheduler and
+cpu_selftest."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import gmr
+from helpers import fixture, grammar_dict, x_for
+from oracle import oracle as O
+
+REF_SELFTEST = os.path.join(os.path.dirname(O.__file__), "_ref", "cpu_selftest")
+ARRAY_KEYS = ("left", "right", "coeff", "level_offsets", "level_widths", "emit_offsets", "emit_pos",
+              "emit_row", "term_sym", "term_row", "term_orig")
+
+
+def _small_matrices():
+    yield "haplotype", gmr.Matrix.haplotype(7, 512, 256, founders=16, seg_mean=32.0)
+    yield "genotype", gmr.Matrix.genotype(8, 800, 300, founders=32, classes=4, seg_mean=40.0)
+    yield "rmat", gmr.Matrix.rmat(9, 9, 4000, nweights=1)
+    yield "rmat_w", gmr.Matrix.rmat(10, 9, 4000, nweights=100)
+    f = fixture("integration_50x50")
+    yield "ci50", gmr.Matrix.from_dense(np.array(f["M"]))
+    f = fixture("readme_8x7")
+    yield "readme", gmr.Matrix.from_dense(np.array(f["M"]))
+
+
+def test_generators_are_seeded_and_thread_independent():
+    a = gmr.Matrix.haplotype(1, 300, 128, founders=8, seg_mean=16.0).arrays()[0]
+    b = gmr.Matrix.haplotype(1, 300, 128, founders=8, seg_mean=16.0).arrays()[0]
+    c = gmr.Matrix.haplotype(2, 300, 128, founders=8, seg_mean=16.0).arrays()[0]
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    old = os.environ.get("OMP_NUM_THREADS")
+    r = subprocess.run(
+        ["python", "-c",
+         "import sys;sys.path.insert(0,'g-mm-repair_amd');import gmr,hashlib;"
+         "s=gmr.Matrix.genotype(5,2000,500,founders=32,classes=4).arrays()[0];"
+         "print(hashlib.sha1(s.tobytes()).hexdigest())"],
+        capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"),
+        cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import hashlib
+    s = gmr.Matrix.genotype(5, 2000, 500, founders=32, classes=4).arrays()[0]
+    assert r.stdout.strip() == hashlib.sha1(s.tobytes()).hexdigest()
+    del old
+
+
+def test_haplotype_shape_and_density():
+    m = gmr.Matrix.haplotype(20261204, 4096, 2048)
+    seq, V = m.arrays()
+    assert m.rows == 4096 and list(V) == [1.0]
+    assert (seq == 0).sum() == 4096
+    dens = m.nnz() / (4096 * 2048)
+    # 0.08 * 0.99 + 0.92 * 0.01 = 0.0884 (founder density with 1% flips)
+    assert abs(dens - 0.0884) < 0.01
+    # each row strictly increasing columns
+    rows = np.split(seq, np.nonzero(seq == 0)[0] + 1)[:-1]
+    assert all(np.all(np.diff(r[:-1]) > 0) for r in rows)
+
+
+def test_genotype_row_counts_follow_one_over_k():
+    m = gmr.Matrix.genotype(3, 20000, 1000, founders=64, classes=8, seg_mean=64.0)
+    seq, _ = m.arrays()
+    k = np.diff(np.nonzero(seq == 0)[0], prepend=-1) - 1
+    assert k.min() >= 1 and k.max() <= 999
+    # P(k) ~ 1/k: P(k=1) = 1/H_999 ~ 0.1333
+    assert abs((k == 1).mean() - 1 / np.sum(1.0 / np.arange(1, 1000))) < 0.01
+
+
+def test_rmat_dedup_sorted_and_weighted():
+    m = gmr.Matrix.rmat(11, 10, 20000, nweights=100)
+    seq, V = m.arrays()
+    assert m.rows == 1024 and len(V) == 100
+    assert m.nnz() <= 20000
+    rows = np.split(seq, np.nonzero(seq == 0)[0] + 1)[:-1]
+    for r in rows:
+        col = (r[:-1] - 1) % 1024
+        assert np.all(np.diff(col) > 0)  # sorted, no duplicates
+
+
+@pytest.mark.parametrize("name,m", list(_small_matrices()))
+def test_repair_round_trip_and_layouts(name, m):
+    seq, V = m.arrays()
+    g = m.repair()
+    ga = g.arrays()
+    assert np.array_equal(g.expand(), seq)
+    assert np.array_equal(O.expand(ga), seq)
+    # topological numbering, no delimiter inside rules
+    idx = np.arange(len(ga["left"])) + ga["alpha"]
+    assert np.all(ga["left"] < idx) and np.all(ga["right"] < idx)
+    assert np.all(ga["left"] > 0) and np.all(ga["right"] > 0)
+    rows, cols = m.rows, m.cols
+    mine = g.schedule(rows, cols, "refcompat").arrays()
+    orc = O.layer_refcompat(ga)
+    for k in ARRAY_KEYS:
+        np.testing.assert_array_equal(mine[k], orc[k], err_msg=k)
+    ref = O.reference_schedule(ga, rows, cols)
+    if ref is not None:
+        for k in ARRAY_KEYS:
+            np.testing.assert_array_equal(mine[k], ref[k], err_msg=k)
+        for k in ("max_depth", "max_width", "num_layered", "num_passthrough", "emit_total", "num_term_emit", "U"):
+            assert mine["info"][k] == ref["info"][k], k
+    # every layout computes the same products (oracle sweep for double-buffered ones)
+    for sr in (0, 1, 2):
+        x = x_for(sr, cols, 5)
+        y_bf, den = O.brute_force(sr, rows, cols, seq, V, x)
+        for lay in ("refcompat", "wave64"):
+            s = g.schedule(rows, cols, lay).arrays()
+            y = O.sweep(sr, s, rows, cols, V, x)
+            if sr == 0:
+                np.testing.assert_allclose(y, y_bf, rtol=1e-9, atol=1e-9 * (1 + den.max()))
+            else:
+                np.testing.assert_array_equal(y, y_bf)
+
+
+def test_layout_invariants():
+    m = gmr.Matrix.haplotype(21, 1024, 512, founders=16, seg_mean=64.0)
+    g = m.repair()
+    ga = g.arrays()
+    R = len(ga["left"])
+    info = {lay: g.schedule(1024, 512, lay).info() for lay in ("refcompat", "wave64", "resident")}
+    assert info["resident"]["num_layered"] == R and info["resident"]["num_passthrough"] == 0
+    assert info["wave64"]["num_layered"] <= info["refcompat"]["num_layered"]
+    for lay in ("refcompat", "wave64", "resident"):
+        s = g.schedule(1024, 512, lay).arrays()
+        lo = s["level_offsets"]
+        assert lo[0] == 0 and lo[-1] == s["info"]["num_layered"]
+        assert np.all(np.diff(s["emit_offsets"]) >= 0)
+        # emission lists are row-sorted inside each level
+        for k in range(1, len(lo)):
+            r = s["emit_row"][s["emit_offsets"][k - 1]:s["emit_offsets"][k]]
+            assert np.all(np.diff(r) >= 0)
+        for k in range(1, len(lo)):
+            L_ = s["left"][lo[k - 1]:lo[k]]
+            R_ = s["right"][lo[k - 1]:lo[k]]
+            lim = lo[k - 1] if lay == "resident" else (lo[k - 1] - lo[k - 2] if k > 1 else 0)
+            for c in (L_, R_):
+                assert np.all((c < 0) | (c < lim))
+                assert np.all(-c[c < 0] - 1 < s["info"]["U"])
+        # the emission count equals the occurrences in C
+        Cs = ga["C"]
+        assert s["info"]["emit_total"] == int((Cs >= ga["alpha"]).sum())
+        assert s["info"]["num_term_emit"] == int(((Cs > 0) & (Cs < ga["alpha"])).sum())
+    # Wave64 keeps only readable copies: refcompat minus one copy per carried chain
+    lvl = np.zeros(R, np.int64)
+    for i in range(R):
+        a = lvl[ga["left"][i] - ga["alpha"]] if ga["left"][i] >= ga["alpha"] else 0
+        b = lvl[ga["right"][i] - ga["alpha"]] if ga["right"][i] >= ga["alpha"] else 0
+        lvl[i] = 1 + max(a, b)
+    mref = lvl.copy()
+    for i in range(R):
+        for c in (ga["left"][i], ga["right"][i]):
+            if c >= ga["alpha"]:
+                mref[c - ga["alpha"]] = max(mref[c - ga["alpha"]], lvl[i])
+    assert info["refcompat"]["num_passthrough"] == int((mref - lvl).sum())
+    assert info["wave64"]["num_passthrough"] == int(np.maximum(mref - lvl - 1, 0).sum())
+
+
+def test_unit_grammar_through_the_engine_scheduler():
+    f = fixture("unit_grammar")
+    g = gmr.Grammar.from_arrays(f["alpha"], f["left"], f["right"], f["C"], f["V"], kind=f["kind"], rep=f["rep"])
+    s = g.schedule(f["rows"], f["cols"], "refcompat")
+    i = s.info()
+    assert i["max_depth"] == 4 and i["num_passthrough"] == 5
+    y = O.sweep(0, s.arrays(), f["rows"], f["cols"], np.array(f["V"]), np.array(f["x"], np.float32))
+    np.testing.assert_allclose(y, f["expect_y"], atol=1e-12)
+    assert g.schedule(2, 3, "wave64").info()["num_passthrough"] == 2
+
+
+def test_files_round_trip_and_reference_selftest(tmp_path):
+    m = gmr.Matrix.haplotype(31, 700, 300, founders=12, seg_mean=30.0)
+    g = m.repair()
+    base = str(tmp_path / "hap")
+    g.write(base)
+    h = gmr.Grammar.read(base)
+    a, b = g.arrays(), h.arrays()
+    for k in ("left", "right", "C", "V"):
+        np.testing.assert_array_equal(a[k], b[k])
+    assert a["alpha"] == b["alpha"]
+    if not os.path.exists(REF_SELFTEST):
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    # The reference's own CUDA-free selftest reads our files, builds its own
+    # schedule and checks its sweep against its brute force for 3 semirings.
+    r = subprocess.run([REF_SELFTEST, base, "700", "300"], capture_output=True, text=True)
+    assert r.returncode == 0 and "REPAIR FILE-TEST PASSED" in r.stdout, r.stdout + r.stderr
+
+
+def test_bad_inputs_are_rejected(tmp_path):
+    with pytest.raises(gmr.GmrError):
+        gmr.Grammar.read(str(tmp_path / "missing"))
+    m = gmr.Matrix.haplotype(1, 50, 40, founders=4, seg_mean=8.0)
+    g = m.repair()
+    base = str(tmp_path / "t")
+    g.write(base)
+    with open(base + ".vc.R", "ab") as fh:
+        fh.write(b"\x01\x00\x00\x00")   # half a rule
+    with pytest.raises(gmr.GmrError, match="truncated"):
+        gmr.Grammar.read(base)
+    # a rule referencing itself (not topological)
+    bad = gmr.Grammar.from_arrays(3, [1, 4], [2, 2], [3, 0], [1.0])
+    with pytest.raises(gmr.GmrError, match="out of range"):
+        bad.schedule(1, 2)
+    # more rows in C than declared
+    g2 = gmr.Grammar.from_arrays(3, [1], [2], [3, 0, 1, 0], [1.0])
+    with pytest.raises(gmr.GmrError, match="more rows"):
+        g2.schedule(1, 2)
+    with pytest.raises(gmr.GmrError):
+        gmr.Matrix.from_csrv(2, 3, [1, 0], [1.0])   # 1 delimiter for 2 rows
+
+
+def test_empty_and_degenerate_grammars():
+    # no rules at all: every C symbol is a terminal
+    g = gmr.Grammar.from_arrays(4, [], [], [1, 2, 0, 0, 3, 0], [1.0])
+    s = g.schedule(3, 3, "refcompat")
+    i = s.info()
+    assert i["max_depth"] == 0 and i["num_layered"] == 0 and i["num_term_emit"] == 3
+    x = np.array([1.0, 2.0, 4.0], np.float32)
+    y = O.sweep(0, s.arrays(), 3, 3, np.array([1.0]), x)
+    np.testing.assert_array_equal(y, [3.0, 0.0, 4.0])
+    # empty matrix
+    m = gmr.Matrix.from_dense(np.zeros((4, 5)))
+    g = m.repair()
+    assert g.info()["nrules"] == 0 and g.schedule(4, 5).info()["num_term_emit"] == 0
