@@ -47,7 +47,7 @@ struct DeviceError : std::runtime_error {
     do {                                                                                    \
         hipError_t e_ = (call);                                                             \
         if (e_ != hipSuccess)                                                               \
-            throw DeviceError(std::string(#call) + ": " + hipGetErrorString(e_));          \
+            throw ::gmr::DeviceError(std::string(#call) + ": " + hipGetErrorString(e_));   \
     } while (0)
 
 template <class T>
@@ -58,12 +58,6 @@ T* dev_upload(const T* h, std::size_t n, std::size_t* acc) {
     GMR_HIP(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
     if (acc) *acc += n * sizeof(T);
     return d;
-}
-
-inline int pow2_at_least(int v) {
-    int g = 1;
-    while (g < v && g < 64) g <<= 1;
-    return g;
 }
 
 }  // namespace
@@ -190,6 +184,12 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
         for (const int2& p : em) csym[fill[p.y]++] = p.x;
     }
 
+    // Single-vector gather: lanes per row ~ a quarter of the mean row length.
+    {
+        const double mean = rows > 0 ? static_cast<double>(E + Et) / rows : 0.0;
+        gather_lanes = 1;
+        while (gather_lanes < 64 && gather_lanes * 4 < mean) gather_lanes <<= 1;
+    }
     struct_bytes = 0;
     d_kids = dev_upload(kids.data(), kids.size(), &struct_bytes);
     d_coeff = has_runlen ? dev_upload(coeff.data(), coeff.size(), &struct_bytes) : nullptr;
@@ -293,8 +293,10 @@ void launch_levels(const Engine& en, const SweepArgs& a, int block, hipStream_t 
             hipLaunchKernelGGL(dev::k_emit<S>, dim3(grid_for(work, block)), dim3(block), 0, st, a, k);
         }
     }
-    if (a.gather)
-        hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(static_cast<long>(en.rows), gpb)), dim3(block), 0, st, a);
+    if (a.gather) {
+        const long threads = (a.Bt == 1 && a.B == 1) ? static_cast<long>(en.rows) * a.GR : static_cast<long>(en.rows) * G;
+        hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(threads, block)), dim3(block), 0, st, a);
+    }
     GMR_HIP(hipGetLastError());
 }
 
@@ -303,13 +305,19 @@ void launch_levels(const Engine& en, const SweepArgs& a, int block, hipStream_t 
 // barrier waits; a wait that outlives its deadline ends with the error word set.
 template <class S>
 void launch_persistent(Engine& en, const SweepArgs& a, int block, int grid_cap, hipStream_t st) {
-    int per_cu = 0;
-    GMR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dev::k_persistent<S>, block, 0));
-    if (per_cu < 1) throw DeviceError("persistent kernel does not fit on a CU");
-    per_cu = std::min(per_cu, 2048 / block);
+    // The barrier needs every workgroup resident at once. The occupancy query
+    // can promise one workgroup per CU more than the dispatcher actually keeps
+    // resident (measured: 7/CU reported, fewer held, at 71 VGPRs x 256 lanes),
+    // so keep one slot of headroom and aim for 16 waves per CU.
+    int occ = 0;
+    GMR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, dev::k_persistent<S>, block, 0));
+    if (occ < 1) throw DeviceError("persistent kernel does not fit on a CU");
+    const int per_cu = std::max(1, std::min(occ - 1, std::max(1, 1024 / block)));
     int grid = per_cu * en.num_cus;
     if (grid_cap > 0) grid = std::min(grid, grid_cap);
     const unsigned long long deadline = static_cast<unsigned long long>(en.wall_khz) * 1000ULL * 2ULL;  // 2 s
+    en.last_grid = grid;
+    en.last_per_cu = per_cu;
     hipLaunchKernelGGL(dev::k_persistent<S>, dim3(grid), dim3(block), 0, st, a, en.d_barrier, en.barrier_base,
                        deadline);
     GMR_HIP(hipGetLastError());
@@ -379,9 +387,11 @@ void run_all(Engine& en, int semiring, const float* X, int B, void* Y, const gmr
         a.n_term = en.Et;
         a.B = B;
         a.Bt = std::min(Bt, B - b0);
-        a.G = pow2_at_least(a.Bt);
+        a.vec = (a.Bt % 4 == 0 && B % 4 == 0) ? 1 : 0;
+        a.G = dev::lanes_for(a.Bt, a.vec);
         a.resident = en.resident ? 1 : 0;
         a.gather = gather ? 1 : 0;
+        a.GR = en.gather_lanes;
         run_tile<S>(en, semiring, a, o, st);
     }
 }
@@ -501,6 +511,18 @@ int gmr_engine_run(gmr_engine* h, int semiring, const void* d_X, int B, void* d_
 int gmr_engine_check(gmr_engine* h, int* barrier_error) {
     if (!h || !barrier_error) return dev_fail(GMR_ERR_INVALID, "NULL argument");
     return dev_guarded([&] { *barrier_error = h->e.check_barrier(); });
+}
+
+int gmr_engine_debug(gmr_engine* h, long long out[8]) {
+    if (!h || !out) return dev_fail(GMR_ERR_INVALID, "NULL argument");
+    return dev_guarded([&] {
+        gmr::dev::GridBarrier b{};
+        GMR_HIP(hipDeviceSynchronize());
+        GMR_HIP(hipMemcpy(&b, h->e.d_barrier, sizeof(b), hipMemcpyDeviceToHost));
+        long long v[8] = {static_cast<long long>(b.arrived), b.error, static_cast<long long>(h->e.barrier_base),
+                          h->e.wall_khz, h->e.num_cus, h->e.last_grid, h->e.last_per_cu, 0};
+        std::memcpy(out, v, sizeof(v));
+    });
 }
 
 int gmr_engine_last_kernel_ms(const gmr_engine* h, double* ms) {

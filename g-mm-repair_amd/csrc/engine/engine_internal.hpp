@@ -20,6 +20,7 @@ struct Engine {
     int rows = 0, cols = 0, U = 0, NL = 0, L = 0, nvals = 0, layout = 0;
     bool resident = false, has_runlen = false;
     int max_width = 0, E = 0, Et = 0;
+    int gather_lanes = 1;  // lanes per row of the single-vector gather
     std::vector<int> level_off, emit_off;
     std::size_t struct_bytes = 0;
 
@@ -45,6 +46,7 @@ struct Engine {
     int num_cus = 0, wall_khz = 0;
     ::gmr::dev::GridBarrier* d_barrier = nullptr;
     unsigned long long barrier_base = 0;  // arrivals recorded by earlier launches
+    int last_grid = 0, last_per_cu = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipStream_t capture_stream = nullptr;
     double last_ms = 0.0;

@@ -83,11 +83,64 @@ struct SweepArgs {
     int G;                   // lanes per item
     int resident;            // children index one resident array
     int gather;              // emit by a final row gather
+    int vec;                 // 1: four vectors per lane (Bt % 4 == 0 and B % 4 == 0)
+    int GR;                  // lanes per row in the single-vector gather
 };
 
 template <class S>
 __device__ __forceinline__ typename S::T* tp(void* p) {
     return static_cast<typename S::T*>(p);
+}
+
+// Four vectors per lane: float4 / int4 loads and stores (dwordx4). Used when
+// the tile width Bt and the row stride B are multiples of 4, so every row of
+// T, W, X and Y starts 16-byte aligned.
+template <class T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<int> { using type = int4; };
+
+template <class S>
+struct Op4 {
+    using T = typename S::T;
+    using T4 = typename Vec4<T>::type;
+    static __device__ __forceinline__ T4 zero() {
+        T4 r;
+        r.x = r.y = r.z = r.w = S::zero();
+        return r;
+    }
+    static __device__ __forceinline__ T4 oplus(T4 a, T4 b) {
+        T4 r;
+        r.x = S::oplus(a.x, b.x);
+        r.y = S::oplus(a.y, b.y);
+        r.z = S::oplus(a.z, b.z);
+        r.w = S::oplus(a.w, b.w);
+        return r;
+    }
+    static __device__ __forceinline__ T4 apply(T4 a, float c) {
+        T4 r;
+        r.x = S::apply(a.x, c);
+        r.y = S::apply(a.y, c);
+        r.z = S::apply(a.z, c);
+        r.w = S::apply(a.w, c);
+        return r;
+    }
+    static __device__ __forceinline__ T4 ld(const T* p) { return *reinterpret_cast<const T4*>(p); }
+    static __device__ __forceinline__ void st(T* p, T4 v) { *reinterpret_cast<T4*>(p) = v; }
+    static __device__ __forceinline__ void emit(T* y, T4 v) {
+        if (!S::skip(v.x)) S::emit(y + 0, v.x);
+        if (!S::skip(v.y)) S::emit(y + 1, v.y);
+        if (!S::skip(v.z)) S::emit(y + 2, v.z);
+        if (!S::skip(v.w)) S::emit(y + 3, v.w);
+    }
+};
+
+// Lanes per item for a tile of Bt vectors: one lane per vector (vec = 0) or per
+// four vectors (vec = 1), capped at the wave width.
+__host__ __device__ __forceinline__ int lanes_for(int Bt, int vec) {
+    int need = vec ? Bt / 4 : Bt;
+    int g = 1;
+    while (g < need && g < 64) g <<= 1;
+    return g;
 }
 
 // T[p*Bt + b] = leaf(V, X[col*B + b]) (T[0..Bt) = zero); Y = zero for spot emits.
@@ -126,6 +179,40 @@ __device__ void ph_eval(const SweepArgs& a, int i0, int i1, const typename S::T*
         const T* rp = c.y >= 0 ? src + static_cast<long>(c.y) * Bt : Tt + static_cast<long>(-c.y - 1) * Bt;
         T* o = dst + (i - i0) * Bt;
         for (int b = lane; b < Bt; b += G) o[b] = S::oplus(S::apply(lp[b], cf), rp[b]);
+    }
+}
+
+// Same, four vectors per lane; two rules per group per step so a wave keeps
+// four dwordx4 child gathers in flight.
+template <class S>
+__device__ void ph_eval_v4(const SweepArgs& a, int i0, int i1, const typename S::T* src,
+                           typename S::T* dst, long gid, long ng, int lane) {
+    using T = typename S::T;
+    using O = Op4<S>;
+    const T* Tt = tp<S>(a.Tv);
+    const int Bt = a.Bt, G = a.G;
+    auto row = [&](int c) -> const T* {
+        return c >= 0 ? src + static_cast<long>(c) * Bt : Tt + static_cast<long>(-c - 1) * Bt;
+    };
+    long i = i0 + gid;
+    for (; i + ng < i1; i += 2 * ng) {
+        const int2 c0 = a.kids[i], c1 = a.kids[i + ng];
+        const float f0 = a.coeff ? a.coeff[i] : 1.0f, f1 = a.coeff ? a.coeff[i + ng] : 1.0f;
+        const T *l0 = row(c0.x), *r0 = row(c0.y), *l1 = row(c1.x), *r1 = row(c1.y);
+        T* o0 = dst + (i - i0) * Bt;
+        T* o1 = dst + (i + ng - i0) * Bt;
+        for (int b = 4 * lane; b < Bt; b += 4 * G) {
+            auto x0 = O::ld(l0 + b), y0 = O::ld(r0 + b), x1 = O::ld(l1 + b), y1 = O::ld(r1 + b);
+            O::st(o0 + b, O::oplus(O::apply(x0, f0), y0));
+            O::st(o1 + b, O::oplus(O::apply(x1, f1), y1));
+        }
+    }
+    if (i < i1) {
+        const int2 c0 = a.kids[i];
+        const float f0 = a.coeff ? a.coeff[i] : 1.0f;
+        const T *l0 = row(c0.x), *r0 = row(c0.y);
+        T* o0 = dst + (i - i0) * Bt;
+        for (int b = 4 * lane; b < Bt; b += 4 * G) O::st(o0 + b, O::oplus(O::apply(O::ld(l0 + b), f0), O::ld(r0 + b)));
     }
 }
 
@@ -186,6 +273,34 @@ __device__ void ph_emit_chunk(const int2* em, int n, const typename S::T* src, t
     }
 }
 
+template <class S, int CH>
+__device__ void ph_emit_chunk_v4(const int2* em, int n, const typename S::T* src, typename S::T* Y,
+                                 int B, int Bt, int G, long gid, long ng, int lane) {
+    using O = Op4<S>;
+    using T4 = typename O::T4;
+    const long nch = (static_cast<long>(n) + CH - 1) / CH;
+    for (long ch = gid; ch < nch; ch += ng) {
+        const int e0 = static_cast<int>(ch * CH);
+        const int e1 = min(n, e0 + CH);
+        for (int b = 4 * lane; b < Bt; b += 4 * G) {
+            int row = -1;
+            T4 acc = O::zero();
+            for (int e = e0; e < e1; ++e) {
+                int2 pr = em[e];
+                T4 v = O::ld(src + static_cast<long>(pr.x) * Bt + b);
+                if (pr.y != row) {
+                    if (row >= 0) O::emit(Y + static_cast<long>(row) * B + b, acc);
+                    row = pr.y;
+                    acc = v;
+                } else {
+                    acc = O::oplus(acc, v);
+                }
+            }
+            if (row >= 0) O::emit(Y + static_cast<long>(row) * B + b, acc);
+        }
+    }
+}
+
 constexpr int kEmitChunk = 8;
 
 template <class S>
@@ -193,14 +308,17 @@ __device__ __forceinline__ void ph_emit(const SweepArgs& a, const int2* em, int 
                                         const typename S::T* src, long tid, long nth, int lane) {
     if (a.Bt == 1 && a.B == 1) {
         ph_emit_wave<S>(em, n, src, tp<S>(a.Y), tid / 64, nth / 64, static_cast<int>(tid & 63));
+    } else if (a.vec) {
+        ph_emit_chunk_v4<S, kEmitChunk>(em, n, src, tp<S>(a.Y), a.B, a.Bt, a.G, tid / a.G, nth / a.G, lane);
     } else {
         ph_emit_chunk<S, kEmitChunk>(em, n, src, tp<S>(a.Y), a.B, a.Bt, a.G, tid / a.G, nth / a.G, lane);
     }
 }
 
 // Resident layout, gather emission: Y[r] = (+) over row r's top-sequence symbols.
+// Scalar B-wide form: a group per row, lanes over vectors.
 template <class S>
-__device__ void ph_gather(const SweepArgs& a, long gid, long ng, int lane) {
+__device__ void ph_gather_rows(const SweepArgs& a, long gid, long ng, int lane) {
     using T = typename S::T;
     const T* W = tp<S>(a.W0);
     const T* Tt = tp<S>(a.Tv);
@@ -220,16 +338,89 @@ __device__ void ph_gather(const SweepArgs& a, long gid, long ng, int lane) {
     }
 }
 
+// Four vectors per lane, symbols unrolled by 4 so each lane has four dwordx4
+// gathers in flight.
+template <class S>
+__device__ void ph_gather_rows_v4(const SweepArgs& a, long gid, long ng, int lane) {
+    using T = typename S::T;
+    using O = Op4<S>;
+    using T4 = typename O::T4;
+    const T* W = tp<S>(a.W0);
+    const T* Tt = tp<S>(a.Tv);
+    T* Y = tp<S>(a.Y);
+    const int Bt = a.Bt;
+    auto row = [&](int c) -> const T* {
+        return c >= 0 ? W + static_cast<long>(c) * Bt : Tt + static_cast<long>(-c - 1) * Bt;
+    };
+    for (long r = gid; r < a.rows; r += ng) {
+        const int s0 = a.crow[r], s1 = a.crow[r + 1];
+        for (int b = 4 * lane; b < Bt; b += 4 * a.G) {
+            T4 acc0 = O::zero(), acc1 = O::zero();
+            int j = s0;
+            for (; j + 3 < s1; j += 4) {
+                const int c0 = a.csym[j], c1 = a.csym[j + 1], c2 = a.csym[j + 2], c3 = a.csym[j + 3];
+                T4 v0 = O::ld(row(c0) + b), v1 = O::ld(row(c1) + b), v2 = O::ld(row(c2) + b), v3 = O::ld(row(c3) + b);
+                acc0 = O::oplus(acc0, O::oplus(v0, v1));
+                acc1 = O::oplus(acc1, O::oplus(v2, v3));
+            }
+            for (; j < s1; ++j) acc0 = O::oplus(acc0, O::ld(row(a.csym[j]) + b));
+            O::st(Y + r * a.B + b, O::oplus(acc0, acc1));
+        }
+    }
+}
+
+// One vector: GR lanes per row stride over the row's symbols, then a
+// shuffle tree reduces the GR partial results.
+template <class S>
+__device__ void ph_gather_rows_1(const SweepArgs& a, long tid, long nth) {
+    using T = typename S::T;
+    const T* W = tp<S>(a.W0);
+    const T* Tt = tp<S>(a.Tv);
+    T* Y = tp<S>(a.Y);
+    const int GR = a.GR;
+    const int sub = static_cast<int>(tid % GR);
+    // All lanes of a wave run the same number of iterations (shuffles below).
+    const long nrows_pad = ((static_cast<long>(a.rows) + (nth / GR) - 1) / (nth / GR)) * (nth / GR);
+    for (long r = tid / GR; r < nrows_pad; r += nth / GR) {
+        T acc = S::zero();
+        if (r < a.rows) {
+            const int s0 = a.crow[r], s1 = a.crow[r + 1];
+            for (int j = s0 + sub; j < s1; j += GR) {
+                int c = a.csym[j];
+                acc = S::oplus(acc, c >= 0 ? W[c] : Tt[-c - 1]);
+            }
+        }
+        for (int d = GR / 2; d >= 1; d >>= 1) acc = S::oplus(acc, __shfl_xor(acc, d, 64));
+        if (sub == 0 && r < a.rows) Y[r] = acc;
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void ph_gather(const SweepArgs& a, long tid, long nth) {
+    if (a.Bt == 1 && a.B == 1) {
+        ph_gather_rows_1<S>(a, tid, nth);
+    } else if (a.vec) {
+        ph_gather_rows_v4<S>(a, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+    } else {
+        ph_gather_rows<S>(a, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void ph_eval_any(const SweepArgs& a, int i0, int i1, const typename S::T* src,
+                                            typename S::T* dst, long gid, long ng, int lane) {
+    if (a.vec) ph_eval_v4<S>(a, i0, i1, src, dst, gid, ng, lane);
+    else ph_eval<S>(a, i0, i1, src, dst, gid, ng, lane);
+}
+
 // Frontier that level k writes (k >= 1) and the source it reads.
 template <class S>
 __device__ __forceinline__ typename S::T* level_dst(const SweepArgs& a, int k) {
-    using T = typename S::T;
     if (a.resident) return tp<S>(a.W0) + static_cast<long>(a.level_off[k - 1]) * a.Bt;
     return (k & 1) ? tp<S>(a.W1) : tp<S>(a.W0);
 }
 template <class S>
 __device__ __forceinline__ const typename S::T* level_src(const SweepArgs& a, int k) {
-    using T = typename S::T;
     if (a.resident) return tp<S>(a.W0);
     return (k & 1) ? tp<S>(a.W0) : tp<S>(a.W1);  // frontier k-1
 }
@@ -246,8 +437,8 @@ template <class S>
 __global__ void k_eval(SweepArgs a, int k) {
     long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     long nth = static_cast<long>(gridDim.x) * blockDim.x;
-    ph_eval<S>(a, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k),
-               tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+    ph_eval_any<S>(a, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k),
+                   tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
 }
 
 // k = 0: terminal emissions; k >= 1: level k's emissions.
@@ -270,7 +461,7 @@ template <class S>
 __global__ void k_gather(SweepArgs a) {
     long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     long nth = static_cast<long>(gridDim.x) * blockDim.x;
-    ph_gather<S>(a, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+    ph_gather<S>(a, tid, nth);
 }
 
 // ---- persistent variant: the whole sweep in one launch ----
@@ -333,12 +524,12 @@ __global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long ba
                 ph_emit<S>(a, a.emits + e0, e1 - e0, src, tid, nth, lane);
             }
         }
-        ph_eval<S>(a, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k), gid, ng,
-                   lane);
+        ph_eval_any<S>(a, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k), gid,
+                       ng, lane);
         grid_barrier(gb, base + nb * static_cast<unsigned long long>(k + 1), deadline);
     }
     if (a.gather) {
-        ph_gather<S>(a, gid, ng, lane);
+        ph_gather<S>(a, tid, nth);
     } else if (a.L == 0) {
         ph_emit<S>(a, a.term_emits, a.n_term, tp<S>(a.Tv), tid, nth, lane);
     } else {

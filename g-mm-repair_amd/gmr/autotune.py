@@ -17,21 +17,26 @@ VARIANT_NAMES = {VARIANT_LEVELS: "levels", VARIANT_GRAPH: "graph", VARIANT_PERSI
 EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
 
 
-def candidates(B, layout, blocks=(256,)):
-    """The search space for one engine and batch width."""
+def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), (1024, 0), (256, 256))):
+    """The search space for one engine and batch width: variant x emission x
+    vector tile x launch shape. Persistent shapes are (block, grid cap); a cap
+    of 0 means the engine's default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
     tiles = sorted({B} | {t for t in (16, 32, 64) if t < B})
     out = []
     for v in (VARIANT_LEVELS, VARIANT_GRAPH, VARIANT_PERSISTENT):
         for e in emits:
             for t in tiles:
-                for blk in blocks:
-                    out.append(dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk))
+                shapes = persistent_shapes if v == VARIANT_PERSISTENT else [(b, 0) for b in blocks]
+                for blk, cap in shapes:
+                    out.append(dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap))
     return out
 
 
 def describe(c):
-    return f"{VARIANT_NAMES[c['variant']]}/{EMIT_NAMES[c['emit']]}/tile={c['b_tile'] or 'B'}/blk={c['block']}"
+    cap = c.get("grid_cap", 0)
+    return (f"{VARIANT_NAMES[c['variant']]}/{EMIT_NAMES[c['emit']]}/tile={c['b_tile'] or 'B'}/blk={c['block']}"
+            + (f"/grid<={cap}" if cap else ""))
 
 
 def shape_key(info, B, semiring):

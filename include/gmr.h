@@ -73,6 +73,11 @@ int gmr_engine_run(gmr_engine* e, int semiring, const void* d_X, int B, void* d_
  * of an earlier persistent launch on this engine timed out (its results are
  * invalid; the barrier has been reset), else 0. Synchronises the device. */
 int gmr_engine_check(gmr_engine* e, int* barrier_error);
+/* Diagnostics of the persistent variant: out[0..6] = barrier arrivals on the
+ * device, barrier error word, arrivals the host expects, wall-clock rate (kHz),
+ * CU count, workgroups of the last persistent launch, workgroups per CU.
+ * Synchronises the device. */
+int gmr_engine_debug(gmr_engine* e, long long out[8]);
 /* Device time of the last gmr_engine_run on this engine, when timing was
  * requested with opts->reserved[0] = 1 (synchronises the stream). */
 int gmr_engine_last_kernel_ms(const gmr_engine* e, double* ms);
