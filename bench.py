@@ -74,6 +74,14 @@ def main():
     ap.add_argument("--variant", default=None, help="force a variant (levels|graph|persistent)")
     ap.add_argument("--emit", default=None, help="force an emission strategy (spot|gather)")
     ap.add_argument("--b-tile", type=int, default=None)
+    ap.add_argument("--profile", action="store_true",
+                    help="only the timed batched region (for rocprofv3); no plain sweep, CPU baseline or check")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run a 1 GiB device-to-device copy first (known bytes, to scale HBM counters)")
+    ap.add_argument("--layout", default="resident", choices=["resident", "wave64", "refcompat"],
+                    help="schedule layout of the timed engine")
+    ap.add_argument("--grammar-cache", default=None,
+                    help="directory to save/load the RePair grammar files between runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this engine's sweep runs on a single MI355X "
@@ -93,15 +101,25 @@ def main():
     log("generating", args.rows, "x", args.cols)
     m = gmr.Matrix.genotype(args.seed, args.rows, args.cols)
     nnz = m.nnz()
-    log("nnz", nnz, "; RePair ...")
-    g = m.repair()
-    log(f"RePair {g.build_seconds:.1f}s rounds={g.build_rounds}", g.info())
+    base = None
+    if args.grammar_cache:
+        os.makedirs(args.grammar_cache, exist_ok=True)
+        base = os.path.join(args.grammar_cache, f"genotype_{args.rows}x{args.cols}_{args.seed}")
+    if base and os.path.exists(base + ".vc.C"):
+        g = gmr.Grammar.read(base)
+        log("grammar loaded from", base)
+    else:
+        log("nnz", nnz, "; RePair ...")
+        g = m.repair()
+        log(f"RePair {g.build_seconds:.1f}s rounds={g.build_rounds}", g.info())
+        if base:
+            g.write(base)
     t = time.time()
-    s_res = g.schedule(args.rows, args.cols, "resident")
-    s_ref = g.schedule(args.rows, args.cols, "refcompat")
+    s_res = g.schedule(args.rows, args.cols, args.layout)
+    s_ref = None if args.profile else g.schedule(args.rows, args.cols, "refcompat")
     sched_s = time.time() - t
     info = s_res.info()
-    info_ref = s_ref.info()
+    info_ref = s_ref.info() if s_ref else {"num_passthrough": None}
     log("schedules", f"{sched_s:.1f}s", "resident", info, "refcompat", info_ref)
     eng = s_res.engine()
     log("engine uploaded", eng.info())
@@ -109,6 +127,8 @@ def main():
     gen = torch.Generator(device="cuda").manual_seed(args.seed)
     X = torch.randn(args.cols, B, device="cuda", generator=gen, dtype=torch.float32)
     Y = torch.empty(args.rows, B, device="cuda", dtype=torch.float32)
+    if B == 1:
+        X, Y = X[:, 0].contiguous(), Y[:, 0].contiguous()
 
     # ---- schedule choice
     if args.no_tune or args.variant:
@@ -116,9 +136,20 @@ def main():
                     b_tile=args.b_tile or 0, block=256)
         tuned = []
     else:
-        cands = autotune.candidates(B, gmr.RESIDENT)
+        cands = autotune.candidates(B, gmr.LAYOUTS[args.layout])
         best, tuned = autotune.tune(eng, 0, X, Y, cands, reps=2, log=log)
     log("schedule:", autotune.describe(best))
+
+    if args.calibrate:
+        # 1 GiB read + 1 GiB written by one copy kernel: the HBM counters of this
+        # dispatch give the bytes-per-count scale for the sweep's dispatches.
+        a_ = torch.empty(1 << 28, device="cuda", dtype=torch.float32).fill_(1.0)
+        b_ = torch.empty_like(a_)
+        torch.cuda.synchronize()
+        b_.copy_(a_)
+        torch.cuda.synchronize()
+        del a_, b_
+        log("calibration copy: 1073741824 bytes read, 1073741824 bytes written")
 
     # ---- timed region: W warm-up steps, then K steps, one event pair per step
     for _ in range(args.warmup):
@@ -143,6 +174,11 @@ def main():
     kern_ms = float(np.mean(per))
     value = nnz * B / (ms_step * 1e-3)
     log(f"batched B={B}: {ms_step:.3f} ms/step, launch avg {kern_ms:.3f} ms, {value / 1e9:.1f} G nnz*vec/s")
+
+    if args.profile:
+        print(json.dumps({"profile": True, "ms_per_step": ms_step, "launch_ms": kern_ms,
+                          "schedule": autotune.describe(best), "bytes_dag": dag_bytes(info, B)}), flush=True)
+        return
 
     # ---- FP32 error against the FP64 oracle on a sample of columns
     from oracle import oracle as O
@@ -184,10 +220,17 @@ def main():
     achieved = bytes_alg / (kern_ms * 1e-3) / 1e9
     workload = f"genotype{args.rows}x{args.cols}_B{B}"
     traffic = read_traffic(workload)
+    launch = {gmr.VARIANT_PERSISTENT: "one k_persistent launch per sweep",
+              gmr.VARIANT_GRAPH: "one hipGraph launch per sweep (k_init, one k_eval per level, k_gather or k_emit)",
+              gmr.VARIANT_LEVELS: "the sweep's per-level launches"}[best["variant"]]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_persistent<PlusTimesOp>" if best["variant"] == gmr.VARIANT_PERSISTENT else "sweep",
-                "bytes_per_launch": bytes_alg}
+                "kernel": launch, "bytes_per_launch": bytes_alg,
+                "note": "achieved = DAG-model algorithmic bytes / launch time; the model charges a y "
+                        "read-modify-write per emission that row-gather emission does not perform"}
+    if traffic:
+        roofline["achieved_measured"] = round(traffic / (kern_ms * 1e-3) / 1e9, 1)
+        roofline["frac_measured"] = round(traffic / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
 
     # ---- CPU baseline: the reference's OpenMP FP32 sweep, bounded sample
     cpu = None
