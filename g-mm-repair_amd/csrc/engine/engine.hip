@@ -68,7 +68,7 @@ Engine::~Engine() {
         if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
     void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_level_off,
-                    d_emit_off, ws_T, ws_W0, ws_W1, d_barrier};
+                    d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -272,10 +272,31 @@ inline unsigned grid_for(long work, int block, int cap = 1 << 20) {
 
 // The plain sweep: one launch per phase and level, in the reference's order
 // (init, terminal emits, then per level the node update and its emits).
+inline int kinds_hash(const Engine& en, const gmr_run_opts& o) {
+    unsigned h = 2166136261u ^ static_cast<unsigned>(o.eval_kind);
+    if (o.level_kind)
+        for (int k = 1; k <= en.L; ++k) h = (h ^ static_cast<unsigned>(o.level_kind[k])) * 16777619u;
+    return static_cast<int>(h & 0x7fffff);
+}
+
+// Grid for one level's node-update launch of the given kind.
+inline unsigned eval_grid(const SweepArgs& a, int kind, long w, int block) {
+    if (kind == dev::EV_LDS && a.Bt != 1) kind = dev::EV_GROUP;
+    switch (kind) {
+        case dev::EV_THREAD: return grid_for(w * (a.vec ? a.Bt / 4 : a.Bt), block);
+        case dev::EV_WAVE: return grid_for(w, block / 64);
+        case dev::EV_LDS: return grid_for(w, block);
+        default: return grid_for(w, block / a.G);
+    }
+}
+
+inline int host_kind(const gmr_run_opts& o, int k) { return o.level_kind ? o.level_kind[k] : o.eval_kind; }
+
 template <class S>
-void launch_levels(const Engine& en, const SweepArgs& a, int block, hipStream_t st) {
+void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, int block, hipStream_t st,
+                   hipEvent_t* level_ev = nullptr) {
     const int G = a.G;
-    const int gpb = block / G;  // groups per block
+    if (level_ev) GMR_HIP(hipEventRecord(level_ev[0], st));
     hipLaunchKernelGGL(dev::k_init<S>, dim3(grid_for(static_cast<long>(en.U) * a.Bt +
                                                       static_cast<long>(en.rows) * a.Bt, block, 4096)),
                        dim3(block), 0, st, a);
@@ -285,8 +306,11 @@ void launch_levels(const Engine& en, const SweepArgs& a, int block, hipStream_t 
     }
     for (int k = 1; k <= en.L; ++k) {
         const int w = en.level_off[k] - en.level_off[k - 1];
+        const int kind = host_kind(o, k);
+        if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * k - 1], st));
         if (w > 0)
-            hipLaunchKernelGGL(dev::k_eval<S>, dim3(grid_for(static_cast<long>(w), gpb)), dim3(block), 0, st, a, k);
+            hipLaunchKernelGGL(dev::k_eval<S>, dim3(eval_grid(a, kind, w, block)), dim3(block), 0, st, a, k, kind);
+        if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * k], st));
         const int ne = en.emit_off[k] - en.emit_off[k - 1];
         if (!a.gather && ne > 0) {
             long work = (a.Bt == 1 && a.B == 1) ? ne : ((static_cast<long>(ne) + dev::kEmitChunk - 1) / dev::kEmitChunk) * G;
@@ -297,6 +321,7 @@ void launch_levels(const Engine& en, const SweepArgs& a, int block, hipStream_t 
         const long threads = (a.Bt == 1 && a.B == 1) ? static_cast<long>(en.rows) * a.GR : static_cast<long>(en.rows) * G;
         hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(threads, block)), dim3(block), 0, st, a);
     }
+    (void)o;
     GMR_HIP(hipGetLastError());
 }
 
@@ -330,7 +355,7 @@ void run_tile(Engine& en, int semiring, SweepArgs a, const gmr_run_opts& o, hipS
     if (o.variant == GMR_VARIANT_PERSISTENT) {
         launch_persistent<S>(en, a, block, o.grid_cap, st);
     } else if (o.variant == GMR_VARIANT_GRAPH) {
-        auto key = std::make_tuple(semiring, reinterpret_cast<uintptr_t>(a.X), reinterpret_cast<uintptr_t>(a.Y), a.B, a.Bt,
+        auto key = std::make_tuple(semiring * 1000003 + kinds_hash(en, o), reinterpret_cast<uintptr_t>(a.X), reinterpret_cast<uintptr_t>(a.Y), a.B, a.Bt,
                                    a.gather, block, reinterpret_cast<uintptr_t>(st));
         auto it = en.graphs.find(key);
         if (it == en.graphs.end()) {
@@ -340,7 +365,7 @@ void run_tile(Engine& en, int semiring, SweepArgs a, const gmr_run_opts& o, hipS
             Engine::Graph g;
             GMR_HIP(hipStreamBeginCapture(en.capture_stream, hipStreamCaptureModeThreadLocal));
             try {
-                launch_levels<S>(en, a, block, en.capture_stream);
+                launch_levels<S>(en, a, o, block, en.capture_stream);
             } catch (...) {
                 hipGraph_t dummy;
                 (void)hipStreamEndCapture(en.capture_stream, &dummy);
@@ -352,7 +377,7 @@ void run_tile(Engine& en, int semiring, SweepArgs a, const gmr_run_opts& o, hipS
         }
         GMR_HIP(hipGraphLaunch(it->second.exec, st));
     } else {
-        launch_levels<S>(en, a, block, st);
+        launch_levels<S>(en, a, o, block, st);
     }
 }
 
@@ -392,6 +417,8 @@ void run_all(Engine& en, int semiring, const float* X, int B, void* Y, const gmr
         a.resident = en.resident ? 1 : 0;
         a.gather = gather ? 1 : 0;
         a.GR = en.gather_lanes;
+        a.eval_kind = o.eval_kind;
+        a.level_kind = o.level_kind ? en.device_kinds(o.level_kind) : nullptr;
         run_tile<S>(en, semiring, a, o, st);
     }
 }
@@ -404,7 +431,11 @@ void Engine::run(int semiring, const float* X, int B, void* Y, const gmr_run_opt
     if (o.block != 0 && (o.block < 64 || o.block > 1024 || o.block % 64))
         throw std::invalid_argument("block must be a multiple of 64 in [64, 1024]");
     if (o.variant < 0 || o.variant > 2) throw std::invalid_argument("unknown variant");
-    const bool timed = o.reserved[0] == 1;
+    if (o.eval_kind < 0 || o.eval_kind >= dev::kEvalKinds) throw std::invalid_argument("unknown eval kind");
+    if (o.level_kind)
+        for (int k = 1; k <= L; ++k)
+            if (o.level_kind[k] < 0 || o.level_kind[k] >= dev::kEvalKinds) throw std::invalid_argument("unknown eval kind");
+    const bool timed = (o.flags & GMR_RUN_TIMED) != 0;
     if (timed) GMR_HIP(hipEventRecord(ev0, st));
     switch (semiring) {
         case GMR_PLUS_TIMES: run_all<dev::PlusTimesOp>(*this, semiring, X, B, Y, o, st); break;
@@ -418,6 +449,87 @@ void Engine::run(int semiring, const float* X, int B, void* Y, const gmr_run_opt
         float ms = 0;
         GMR_HIP(hipEventElapsedTime(&ms, ev0, ev1));
         last_ms = ms;
+    }
+}
+
+const int* Engine::device_kinds(const int* host_kinds) {
+    std::vector<int> v(host_kinds, host_kinds + L + 1);
+    if (v != kinds_host || !d_kinds) {
+        if (!d_kinds) GMR_HIP(hipMalloc(&d_kinds, sizeof(int) * (L + 1)));
+        // The previous table may still be read by queued work.
+        GMR_HIP(hipDeviceSynchronize());
+        GMR_HIP(hipMemcpy(d_kinds, v.data(), sizeof(int) * (L + 1), hipMemcpyHostToDevice));
+        kinds_host = v;
+    }
+    return d_kinds;
+}
+
+namespace {
+template <class S>
+void time_levels_t(Engine& en, const float* X, int B, void* Y, const gmr_run_opts& o, int reps, float* out,
+                   hipStream_t st) {
+    using T = typename S::T;
+    en.ensure_workspace(B, static_cast<int>(sizeof(T)));
+    SweepArgs a{};
+    a.kids = en.d_kids;
+    a.coeff = en.d_coeff;
+    a.tcol = en.d_tcol;
+    a.tval = en.d_tval;
+    a.emits = en.d_emits;
+    a.term_emits = en.d_term;
+    a.crow = en.d_crow;
+    a.csym = en.d_csym;
+    a.level_off = en.d_level_off;
+    a.emit_off = en.d_emit_off;
+    a.X = X;
+    a.Y = Y;
+    a.Tv = en.ws_T;
+    a.W0 = en.ws_W0;
+    a.W1 = en.ws_W1;
+    a.rows = en.rows;
+    a.cols = en.cols;
+    a.U = en.U;
+    a.L = en.L;
+    a.n_term = en.Et;
+    a.B = B;
+    a.Bt = B;
+    a.vec = (B % 4 == 0) ? 1 : 0;
+    a.G = dev::lanes_for(B, a.vec);
+    a.resident = en.resident ? 1 : 0;
+    a.gather = (o.emit == GMR_EMIT_GATHER) ? 1 : 0;
+    a.GR = en.gather_lanes;
+    a.eval_kind = o.eval_kind;
+    a.level_kind = o.level_kind ? en.device_kinds(o.level_kind) : nullptr;
+    const int block = o.block > 0 ? o.block : 256;
+    std::vector<hipEvent_t> ev(2 * en.L + 1);
+    for (auto& e : ev) GMR_HIP(hipEventCreate(&e));
+    std::vector<double> acc(en.L + 1, 0.0);
+    for (int r = 0; r < reps; ++r) {
+        launch_levels<S>(en, a, o, block, st, ev.data());
+        GMR_HIP(hipStreamSynchronize(st));
+        float ms = 0;
+        if (en.L > 0) {
+            GMR_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            acc[0] += ms;
+        }
+        for (int k = 1; k <= en.L; ++k) {
+            GMR_HIP(hipEventElapsedTime(&ms, ev[2 * k - 1], ev[2 * k]));
+            acc[k] += ms;
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k <= en.L; ++k) out[k] = static_cast<float>(acc[k] / std::max(reps, 1));
+}
+}  // namespace
+
+void Engine::time_levels(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, int reps,
+                         float* out, hipStream_t st) {
+    if (B < 1 || !X || !Y || !out || reps < 1) throw std::invalid_argument("bad time_levels arguments");
+    switch (semiring) {
+        case GMR_PLUS_TIMES: time_levels_t<dev::PlusTimesOp>(*this, X, B, Y, o, reps, out, st); break;
+        case GMR_BOOLEAN: time_levels_t<dev::BooleanOp>(*this, X, B, Y, o, reps, out, st); break;
+        case GMR_TROPICAL: time_levels_t<dev::TropicalOp>(*this, X, B, Y, o, reps, out, st); break;
+        default: throw std::invalid_argument("unknown semiring");
     }
 }
 
@@ -505,6 +617,17 @@ int gmr_engine_run(gmr_engine* h, int semiring, const void* d_X, int B, void* d_
     if (opts) o = *opts;
     return dev_guarded([&] {
         h->e.run(semiring, static_cast<const float*>(d_X), B, d_Y, o, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gmr_engine_time_levels(gmr_engine* h, int semiring, const void* d_X, int B, void* d_Y,
+                           const gmr_run_opts* opts, int reps, float* level_ms, void* stream) {
+    if (!h) return dev_fail(GMR_ERR_INVALID, "engine is NULL");
+    gmr_run_opts o{};
+    if (opts) o = *opts;
+    return dev_guarded([&] {
+        h->e.time_levels(semiring, static_cast<const float*>(d_X), B, d_Y, o, reps, level_ms,
+                         static_cast<hipStream_t>(stream));
     });
 }
 

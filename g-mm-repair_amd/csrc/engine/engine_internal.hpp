@@ -47,6 +47,8 @@ struct Engine {
     ::gmr::dev::GridBarrier* d_barrier = nullptr;
     unsigned long long barrier_base = 0;  // arrivals recorded by earlier launches
     int last_grid = 0, last_per_cu = 0;
+    int* d_kinds = nullptr;          // per-level eval kinds of the latest run
+    std::vector<int> kinds_host;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipStream_t capture_stream = nullptr;
     double last_ms = 0.0;
@@ -71,6 +73,9 @@ struct Engine {
     // 1 if a persistent launch's barrier timed out since the last check (and
     // the barrier state was reset), else 0. Synchronises the device.
     int check_barrier();
+    const int* device_kinds(const int* host_kinds);
+    void time_levels(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, int reps, float* out,
+                     hipStream_t st);
 };
 
 }  // namespace gmr

@@ -85,7 +85,22 @@ struct SweepArgs {
     int gather;              // emit by a final row gather
     int vec;                 // 1: four vectors per lane (Bt % 4 == 0 and B % 4 == 0)
     int GR;                  // lanes per row in the single-vector gather
+    int eval_kind;           // EvalKind for levels without an entry in level_kind
+    const int* level_kind;   // device, max_depth + 1 entries (or nullptr)
 };
+
+// Per-level node-update kernels (one is chosen for each level):
+//   EV_GROUP   a group of G lanes per rule, lanes over vectors (dwordx4 when
+//              possible), two rules per group in flight
+//   EV_THREAD  one lane per (rule, 4-vector chunk): no idle lanes for narrow B
+//   EV_WAVE    a whole wave per rule, lanes over single vectors
+//   EV_LDS     single vector only: a workgroup stages the window of the
+//              previous frontier its rules' left children fall in into LDS
+//              (the Wave64/Resident layouts sort a level's rules by left
+//              child), then gathers left children from LDS
+enum EvalKind : int { EV_GROUP = 0, EV_THREAD = 1, EV_WAVE = 2, EV_LDS = 3 };
+constexpr int kEvalKinds = 4;
+constexpr int kLdsWindow = 4096;  // values staged per workgroup (16 KiB)
 
 template <class S>
 __device__ __forceinline__ typename S::T* tp(void* p) {
@@ -406,11 +421,123 @@ __device__ __forceinline__ void ph_gather(const SweepArgs& a, long tid, long nth
     }
 }
 
+// EV_THREAD: flattened (rule, chunk) items, chunk = 4 vectors (vec) or 1.
 template <class S>
-__device__ __forceinline__ void ph_eval_any(const SweepArgs& a, int i0, int i1, const typename S::T* src,
-                                            typename S::T* dst, long gid, long ng, int lane) {
-    if (a.vec) ph_eval_v4<S>(a, i0, i1, src, dst, gid, ng, lane);
-    else ph_eval<S>(a, i0, i1, src, dst, gid, ng, lane);
+__device__ void ph_eval_thread(const SweepArgs& a, int i0, int i1, const typename S::T* src,
+                               typename S::T* dst, long tid, long nth) {
+    using T = typename S::T;
+    using O = Op4<S>;
+    const T* Tt = tp<S>(a.Tv);
+    const int Bt = a.Bt;
+    const int c = a.vec ? Bt / 4 : Bt;
+    const long n = static_cast<long>(i1 - i0) * c;
+    for (long j = tid; j < n; j += nth) {
+        const long r = j / c;
+        const int ch = static_cast<int>(j - r * c);
+        const long i = i0 + r;
+        const int2 k = a.kids[i];
+        const float cf = a.coeff ? a.coeff[i] : 1.0f;
+        const T* lp = k.x >= 0 ? src + static_cast<long>(k.x) * Bt : Tt + static_cast<long>(-k.x - 1) * Bt;
+        const T* rp = k.y >= 0 ? src + static_cast<long>(k.y) * Bt : Tt + static_cast<long>(-k.y - 1) * Bt;
+        T* o = dst + r * Bt;
+        if (a.vec) {
+            const int b = 4 * ch;
+            O::st(o + b, O::oplus(O::apply(O::ld(lp + b), cf), O::ld(rp + b)));
+        } else {
+            o[ch] = S::oplus(S::apply(lp[ch], cf), rp[ch]);
+        }
+    }
+}
+
+// EV_WAVE: 64 lanes per rule.
+template <class S>
+__device__ void ph_eval_wave(const SweepArgs& a, int i0, int i1, const typename S::T* src,
+                             typename S::T* dst, long wid, long nw, int lane) {
+    using T = typename S::T;
+    const T* Tt = tp<S>(a.Tv);
+    const int Bt = a.Bt;
+    for (long i = i0 + wid; i < i1; i += nw) {
+        const int2 k = a.kids[i];
+        const float cf = a.coeff ? a.coeff[i] : 1.0f;
+        const T* lp = k.x >= 0 ? src + static_cast<long>(k.x) * Bt : Tt + static_cast<long>(-k.x - 1) * Bt;
+        const T* rp = k.y >= 0 ? src + static_cast<long>(k.y) * Bt : Tt + static_cast<long>(-k.y - 1) * Bt;
+        T* o = dst + (i - i0) * Bt;
+        for (int b = lane; b < Bt; b += 64) o[b] = S::oplus(S::apply(lp[b], cf), rp[b]);
+    }
+}
+
+// Workgroup-wide min and max (every thread of the workgroup must call it).
+__device__ __forceinline__ void block_minmax(int& lo, int& hi) {
+    __shared__ int red[2][16];
+    for (int d = 32; d >= 1; d >>= 1) {
+        lo = min(lo, __shfl_xor(lo, d, 64));
+        hi = max(hi, __shfl_xor(hi, d, 64));
+    }
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = lo;
+        red[1][w] = hi;
+    }
+    __syncthreads();
+    lo = red[0][0];
+    hi = red[1][0];
+    for (int i = 1; i < nw; ++i) {
+        lo = min(lo, red[0][i]);
+        hi = max(hi, red[1][i]);
+    }
+}
+
+// EV_LDS (single vector): one rule per thread; blocks walk the level in
+// blockDim-sized chunks, all threads of a block run the same iterations.
+template <class S>
+__device__ void ph_eval_lds(const SweepArgs& a, int i0, int i1, const typename S::T* src,
+                            typename S::T* dst, long bid, long nb) {
+    using T = typename S::T;
+    __shared__ T win[kLdsWindow];
+    const T* Tt = tp<S>(a.Tv);
+    for (long base = i0 + bid * blockDim.x; base < i1; base += nb * blockDim.x) {
+        const long i = base + threadIdx.x;
+        int2 k = make_int2(-1, -1);
+        float cf = 1.0f;
+        if (i < i1) {
+            k = a.kids[i];
+            if (a.coeff) cf = a.coeff[i];
+        }
+        int lo = k.x >= 0 ? k.x : 0x7fffffff, hi = k.x >= 0 ? k.x : -1;
+        block_minmax(lo, hi);
+        const bool staged = hi >= lo && hi - lo < kLdsWindow;
+        if (staged)
+            for (int t = threadIdx.x; t <= hi - lo; t += blockDim.x) win[t] = src[lo + t];
+        __syncthreads();
+        if (i < i1) {
+            T lv = k.x >= 0 ? (staged ? win[k.x - lo] : src[k.x]) : Tt[-k.x - 1];
+            T rv = k.y >= 0 ? src[k.y] : Tt[-k.y - 1];
+            dst[i - i0] = S::oplus(S::apply(lv, cf), rv);
+        }
+        __syncthreads();
+    }
+}
+
+// The level's kernel: kind as given, EV_LDS only for a single vector.
+template <class S>
+__device__ __forceinline__ void ph_eval_kind(const SweepArgs& a, int kind, int i0, int i1,
+                                             const typename S::T* src, typename S::T* dst, long tid,
+                                             long nth) {
+    if (kind == EV_LDS && a.Bt != 1) kind = EV_GROUP;
+    switch (kind) {
+        case EV_THREAD: ph_eval_thread<S>(a, i0, i1, src, dst, tid, nth); break;
+        case EV_WAVE: ph_eval_wave<S>(a, i0, i1, src, dst, tid / 64, nth / 64, static_cast<int>(tid & 63)); break;
+        case EV_LDS: ph_eval_lds<S>(a, i0, i1, src, dst, tid / blockDim.x, nth / blockDim.x); break;
+        default:
+            if (a.vec) ph_eval_v4<S>(a, i0, i1, src, dst, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+            else ph_eval<S>(a, i0, i1, src, dst, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+    }
+}
+
+template <class S>
+__device__ __forceinline__ int level_kind(const SweepArgs& a, int k) {
+    return a.level_kind ? a.level_kind[k] : a.eval_kind;
 }
 
 // Frontier that level k writes (k >= 1) and the source it reads.
@@ -434,11 +561,11 @@ __global__ void k_init(SweepArgs a) {
 }
 
 template <class S>
-__global__ void k_eval(SweepArgs a, int k) {
+__global__ void k_eval(SweepArgs a, int k, int kind) {
     long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     long nth = static_cast<long>(gridDim.x) * blockDim.x;
-    ph_eval_any<S>(a, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k),
-                   tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+    ph_eval_kind<S>(a, kind, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k), tid,
+                    nth);
 }
 
 // k = 0: terminal emissions; k >= 1: level k's emissions.
@@ -510,7 +637,6 @@ __global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long ba
     const long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long nth = static_cast<long>(gridDim.x) * blockDim.x;
     const int lane = static_cast<int>(tid % a.G);
-    const long gid = tid / a.G, ng = nth / a.G;
     const unsigned long long nb = gridDim.x;
     ph_init<S>(a, tid, nth);
     grid_barrier(gb, base + nb, deadline);
@@ -524,8 +650,8 @@ __global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long ba
                 ph_emit<S>(a, a.emits + e0, e1 - e0, src, tid, nth, lane);
             }
         }
-        ph_eval_any<S>(a, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k), gid,
-                       ng, lane);
+        ph_eval_kind<S>(a, level_kind<S>(a, k), a.level_off[k - 1], a.level_off[k], level_src<S>(a, k),
+                        level_dst<S>(a, k), tid, nth);
         grid_barrier(gb, base + nb * static_cast<unsigned long long>(k + 1), deadline);
     }
     if (a.gather) {

@@ -31,6 +31,9 @@ VARIANT_LEVELS, VARIANT_GRAPH, VARIANT_PERSISTENT = 0, 1, 2
 EMIT_SPOT, EMIT_GATHER = 0, 1
 VARIANTS = {"levels": VARIANT_LEVELS, "graph": VARIANT_GRAPH, "persistent": VARIANT_PERSISTENT}
 EMITS = {"spot": EMIT_SPOT, "gather": EMIT_GATHER}
+# Per-level node-update kernels (gmr.h gmr_eval_kind).
+EVAL_GROUP, EVAL_THREAD, EVAL_WAVE, EVAL_LDS = 0, 1, 2, 3
+EVAL_KINDS = {"group": EVAL_GROUP, "thread": EVAL_THREAD, "wave": EVAL_WAVE, "lds": EVAL_LDS}
 
 TROPICAL_INF = 1000000000
 
@@ -262,17 +265,45 @@ class Engine:
                 "clen_rows", "r0", "r1", "r2"]
         return dict(zip(keys, list(a)))
 
-    def run(self, semiring, X_ptr, B, Y_ptr, variant=VARIANT_LEVELS, emit=EMIT_SPOT, b_tile=0,
-            block=0, grid_cap=0, stream=0):
+    @staticmethod
+    def _opts(variant=VARIANT_LEVELS, emit=EMIT_SPOT, b_tile=0, block=0, grid_cap=0, eval_kind=0,
+              level_kind=None, flags=0):
+        o = RunOpts(variant, emit, b_tile, block, grid_cap, eval_kind, flags, 0, None)
+        keep = None
+        if level_kind is not None:
+            keep = np.ascontiguousarray(level_kind, dtype=np.int32)
+            o.level_kind = keep.ctypes.data_as(C.POINTER(C.c_int))
+        return o, keep
+
+    def run(self, semiring, X_ptr, B, Y_ptr, stream=0, **kw):
         """Y = M X on device pointers: X is cols x B (X[c*B+b]), Y is rows x B.
 
         semiring: 0 (+,x) float, 1 Boolean int32, 2 Tropical int32 (Y holds int
-        bits; X is float as in the reference API)."""
+        bits; X is float as in the reference API). Keywords: variant, emit,
+        b_tile, block, grid_cap, eval_kind, level_kind (max_depth+1 ints)."""
         if isinstance(semiring, str):
             semiring = SEMIRINGS[semiring]
-        o = RunOpts(variant, emit, b_tile, block, grid_cap)
+        o, keep = self._opts(**kw)
         _check(_lib.dev().gmr_engine_run(self._h, semiring, C.c_void_p(X_ptr), B, C.c_void_p(Y_ptr),
                                          C.byref(o), C.c_void_p(stream)))
+        del keep
+
+    def time_levels(self, semiring, X, Y, reps=3, **kw):
+        """Per-level node-update device times (ms), index 0 = init + terminal
+        emits; X, Y torch device tensors."""
+        import torch
+        if isinstance(semiring, str):
+            semiring = SEMIRINGS[semiring]
+        B = 1 if X.dim() == 1 else X.shape[1]
+        L = self.info()["max_depth"]
+        out = np.zeros(L + 1, np.float32)
+        o, keep = self._opts(**kw)
+        stream = torch.cuda.current_stream(X.device).cuda_stream
+        _check(_lib.dev().gmr_engine_time_levels(self._h, semiring, C.c_void_p(X.data_ptr()), B,
+                                                 C.c_void_p(Y.data_ptr()), C.byref(o), reps,
+                                                 out.ctypes.data_as(C.POINTER(C.c_float)), C.c_void_p(stream)))
+        del keep
+        return out
 
     def check(self):
         """Raise if a persistent launch's grid barrier timed out since the last

@@ -37,8 +37,9 @@ class GPUSchedule(C.Structure):
 class RunOpts(C.Structure):
     """gmr.h gmr_run_opts."""
     _fields_ = [
-        ("variant", C.c_int), ("emit", C.c_int), ("b_tile", C.c_int),
-        ("block", C.c_int), ("grid_cap", C.c_int), ("reserved", C.c_int * 3),
+        ("variant", C.c_int), ("emit", C.c_int), ("b_tile", C.c_int), ("block", C.c_int),
+        ("grid_cap", C.c_int), ("eval_kind", C.c_int), ("flags", C.c_int), ("reserved", C.c_int),
+        ("level_kind", c_int_p),
     ]
 
 
@@ -105,6 +106,7 @@ def dev():
     _sig(L, "gmr_engine_last_kernel_ms", C.c_int, [vp, c_double_p])
     _sig(L, "gmr_engine_check", C.c_int, [vp, c_int_p])
     _sig(L, "gmr_engine_debug", C.c_int, [vp, c_ll_p])
+    _sig(L, "gmr_engine_time_levels", C.c_int, [vp, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(RunOpts), C.c_int, c_float_p, C.c_void_p])
     _sig(L, "gmr_engine_destroy", None, [vp])
     _sig(L, "gmr_device_sync", C.c_int, [])
     _dev = L

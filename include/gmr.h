@@ -41,13 +41,26 @@ enum gmr_emit {
     GMR_EMIT_GATHER = 1  /* Resident layout only: one row-ordered gather after the sweep */
 };
 
+/* Node-update kernel of a level. */
+enum gmr_eval_kind {
+    GMR_EVAL_GROUP = 0,   /* a lane group per rule, lanes over vectors (dwordx4) */
+    GMR_EVAL_THREAD = 1,  /* one lane per (rule, 4-vector chunk) */
+    GMR_EVAL_WAVE = 2,    /* one wave per rule */
+    GMR_EVAL_LDS = 3      /* single vector: left-child window staged in LDS */
+};
+
+enum gmr_run_flags { GMR_RUN_TIMED = 1 /* record device time (see gmr_engine_last_kernel_ms) */ };
+
 typedef struct gmr_run_opts {
-    int variant;   /* gmr_variant */
-    int emit;      /* gmr_emit */
-    int b_tile;    /* vectors per pass over the grammar (0: all B at once) */
-    int block;     /* threads per workgroup (0: 256) */
-    int grid_cap;  /* persistent variant: max workgroups (0: full occupancy) */
-    int reserved[3];
+    int variant;            /* gmr_variant */
+    int emit;               /* gmr_emit */
+    int b_tile;             /* vectors per pass over the grammar (0: all B at once) */
+    int block;              /* threads per workgroup (0: 256) */
+    int grid_cap;           /* persistent variant: max workgroups (0: 16 waves per CU) */
+    int eval_kind;          /* gmr_eval_kind of every level not listed in level_kind */
+    int flags;              /* gmr_run_flags */
+    int reserved;
+    const int* level_kind;  /* optional host array, max_depth + 1 entries ([0] unused) */
 } gmr_run_opts;
 
 typedef struct gmr_engine gmr_engine;
@@ -79,8 +92,14 @@ int gmr_engine_check(gmr_engine* e, int* barrier_error);
  * Synchronises the device. */
 int gmr_engine_debug(gmr_engine* e, long long out[8]);
 /* Device time of the last gmr_engine_run on this engine, when timing was
- * requested with opts->reserved[0] = 1 (synchronises the stream). */
+ * requested with GMR_RUN_TIMED in opts->flags (that run synchronises its stream). */
 int gmr_engine_last_kernel_ms(const gmr_engine* e, double* ms);
+/* Per-level device times of the node-update launches: runs the levels
+ * variant `reps` times with events around every level's eval launch, using the
+ * eval kinds in opts; level_ms has max_depth + 1 entries ([0] = terminal
+ * initialisation and terminal emissions). Synchronises the stream. */
+int gmr_engine_time_levels(gmr_engine* e, int semiring, const void* d_X, int B, void* d_Y,
+                           const gmr_run_opts* opts, int reps, float* level_ms, void* stream);
 void gmr_engine_destroy(gmr_engine* e);
 int gmr_device_sync(void);
 

@@ -242,3 +242,54 @@ def test_dropin_entry_points(hap, monkeypatch):
     _say("dropin batched")
     Y = _dropin(True, s, 4096, 2048, V, X.reshape(-1), 8, "plustimes", monkeypatch).reshape(4096, 8)
     check(0, Y, Y_star, den, "dropin batched")
+
+
+@pytest.mark.parametrize("B", [1, 3, 4, 64])
+def test_eval_kernels_per_level(hap, B):
+    """Every node-update kernel, uniform and mixed per level, in every launch
+    variant, gives the oracle's product."""
+    m, g, seq, V = hap
+    X = x_for(0, 2048, 900 + B, B=B) if B > 1 else x_for(0, 2048, 900)
+    if B > 1:
+        Y_star, den = O.brute_force_batched(0, 4096, 2048, seq, V, X)
+    else:
+        Y_star, den = O.brute_force(0, 4096, 2048, seq, V, X)
+    rng = np.random.default_rng(B)
+    for lay in ("wave64", "resident"):
+        eng = g.schedule(4096, 2048, lay).engine()
+        L = eng.info()["max_depth"]
+        mixed = rng.integers(0, 4, L + 1).astype(np.int32)
+        for var in ("levels", "graph", "persistent"):
+            for kind in (0, 1, 2, 3, None):
+                kw = dict(variant=gmr.VARIANTS[var], emit=gmr.EMIT_SPOT)
+                if kind is None:
+                    kw["level_kind"] = mixed
+                else:
+                    kw["eval_kind"] = kind
+                _say("eval kinds", B, lay, var, kind)
+                Y = run_engine(eng, 0, X, **kw)
+                check(0, Y, Y_star, den, f"B={B} {lay}/{var}/kind={kind}")
+
+
+def test_eval_kernels_integer_semirings(hap):
+    m, g, seq, V = hap
+    eng = g.schedule(4096, 2048, "wave64").engine()
+    for sr in (1, 2):
+        x = x_for(sr, 2048, 70 + sr)
+        y_star, _ = O.brute_force(sr, 4096, 2048, seq, V, x)
+        for kind in range(4):
+            for var in ("levels", "persistent"):
+                _say("int eval kinds", sr, kind, var)
+                y = run_engine(eng, sr, x, variant=gmr.VARIANTS[var], eval_kind=kind)
+                check(sr, y, y_star, None, f"sr={sr} kind={kind} {var}")
+
+
+def test_time_levels_reports_every_level(hap):
+    m, g, seq, V = hap
+    eng = g.schedule(4096, 2048, "resident").engine()
+    X = torch.from_numpy(x_for(0, 2048, 5, B=16)).cuda()
+    Y = torch.empty(4096, 16, device="cuda")
+    for kind in range(4):
+        t = eng.time_levels(0, X, Y, reps=2, eval_kind=kind)
+        assert len(t) == eng.info()["max_depth"] + 1
+        assert np.all(t >= 0) and t[1:].sum() > 0
