@@ -381,6 +381,42 @@ void run_tile(Engine& en, int semiring, SweepArgs a, const gmr_run_opts& o, hipS
     }
 }
 
+// Arguments of one tile of vectors [b0, b0 + Bt) of one engine.
+SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int b0, int Bt, bool gather,
+                    const gmr_run_opts& o) {
+    SweepArgs a{};
+    a.kids = en.d_kids;
+    a.coeff = en.d_coeff;
+    a.tcol = en.d_tcol;
+    a.tval = en.d_tval;
+    a.emits = en.d_emits;
+    a.term_emits = en.d_term;
+    a.crow = en.d_crow;
+    a.csym = en.d_csym;
+    a.level_off = en.d_level_off;
+    a.emit_off = en.d_emit_off;
+    a.X = X + b0;
+    a.Y = static_cast<char*>(Y) + static_cast<std::size_t>(b0) * vbytes;
+    a.Tv = en.ws_T;
+    a.W0 = en.ws_W0;
+    a.W1 = en.ws_W1;
+    a.rows = en.rows;
+    a.cols = en.cols;
+    a.U = en.U;
+    a.L = en.L;
+    a.n_term = en.Et;
+    a.B = B;
+    a.Bt = std::min(Bt, B - b0);
+    a.vec = (a.Bt % 4 == 0 && B % 4 == 0) ? 1 : 0;
+    a.G = dev::lanes_for(a.Bt, a.vec);
+    a.resident = en.resident ? 1 : 0;
+    a.gather = gather ? 1 : 0;
+    a.GR = en.gather_lanes;
+    a.eval_kind = o.eval_kind;
+    a.level_kind = o.level_kind ? en.device_kinds(o.level_kind) : nullptr;
+    return a;
+}
+
 template <class S>
 void run_all(Engine& en, int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st) {
     using T = typename S::T;
@@ -389,36 +425,7 @@ void run_all(Engine& en, int semiring, const float* X, int B, void* Y, const gmr
     if (gather && !en.resident) throw std::invalid_argument("gather emission needs the Resident layout");
     en.ensure_workspace(Bt, static_cast<int>(sizeof(T)));
     for (int b0 = 0; b0 < B; b0 += Bt) {
-        SweepArgs a{};
-        a.kids = en.d_kids;
-        a.coeff = en.d_coeff;
-        a.tcol = en.d_tcol;
-        a.tval = en.d_tval;
-        a.emits = en.d_emits;
-        a.term_emits = en.d_term;
-        a.crow = en.d_crow;
-        a.csym = en.d_csym;
-        a.level_off = en.d_level_off;
-        a.emit_off = en.d_emit_off;
-        a.X = X + b0;
-        a.Y = static_cast<T*>(Y) + b0;
-        a.Tv = en.ws_T;
-        a.W0 = en.ws_W0;
-        a.W1 = en.ws_W1;
-        a.rows = en.rows;
-        a.cols = en.cols;
-        a.U = en.U;
-        a.L = en.L;
-        a.n_term = en.Et;
-        a.B = B;
-        a.Bt = std::min(Bt, B - b0);
-        a.vec = (a.Bt % 4 == 0 && B % 4 == 0) ? 1 : 0;
-        a.G = dev::lanes_for(a.Bt, a.vec);
-        a.resident = en.resident ? 1 : 0;
-        a.gather = gather ? 1 : 0;
-        a.GR = en.gather_lanes;
-        a.eval_kind = o.eval_kind;
-        a.level_kind = o.level_kind ? en.device_kinds(o.level_kind) : nullptr;
+        SweepArgs a = make_args(en, X, B, Y, static_cast<int>(sizeof(T)), b0, Bt, gather, o);
         run_tile<S>(en, semiring, a, o, st);
     }
 }
@@ -533,6 +540,218 @@ void Engine::time_levels(int semiring, const float* X, int B, void* Y, const gmr
     }
 }
 
+
+// ---------------------------------------------------------------- batches
+// A fixed set of engines swept together (Y_g = M_g X_g, B_g vectors each).
+struct Batch {
+    std::vector<Engine*> eng;
+    dev::SweepArgs* d_args = nullptr;
+    dev::Tile* d_tiles = nullptr;
+    long* d_phase_off = nullptr;
+    std::size_t cap_args = 0, cap_tiles = 0, cap_phase = 0;
+    dev::GridBarrier* d_barrier = nullptr;
+    unsigned long long barrier_base = 0;
+    int num_cus = 0, wall_khz = 100000;
+    hipStream_t capture_stream = nullptr;
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> branch_ev;
+    hipEvent_t fork_ev = nullptr;
+    std::map<std::size_t, Engine::Graph> graphs;
+    std::size_t tiles_key = 0;
+    long last_tiles = 0;
+    int last_phases = 0;
+    ~Batch();
+};
+
+Batch::~Batch() {
+    for (auto& kv : graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    for (void* p : {static_cast<void*>(d_args), static_cast<void*>(d_tiles), static_cast<void*>(d_phase_off),
+                    static_cast<void*>(d_barrier)})
+        if (p) (void)hipFree(p);
+    for (auto st : branch) (void)hipStreamDestroy(st);
+    for (auto ev : branch_ev) (void)hipEventDestroy(ev);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+}
+
+namespace {
+
+template <class T>
+void grow_upload(T** d, std::size_t* cap, const std::vector<T>& h) {
+    if (h.size() > *cap) {
+        if (*d) GMR_HIP(hipFree(*d));
+        *d = nullptr;
+        GMR_HIP(hipMalloc(d, std::max<std::size_t>(h.size(), 1) * sizeof(T)));
+        *cap = h.size();
+    }
+    if (!h.empty()) GMR_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+inline std::size_t mix(std::size_t h, std::size_t v) { return (h ^ v) * 1099511628211ULL; }
+
+// Tile sizes: enough work per workgroup visit to amortise the tile fetch.
+inline long eval_tile_rules(const dev::SweepArgs& a, int kind, int block) {
+    if (kind == dev::EV_LDS && a.Bt != 1) kind = dev::EV_GROUP;
+    switch (kind) {
+        case dev::EV_THREAD: return std::max(1L, static_cast<long>(block) * 4 / (a.vec ? a.Bt / 4 : a.Bt));
+        case dev::EV_WAVE: return static_cast<long>(block / 64) * 4;
+        case dev::EV_LDS: return block;
+        default: return static_cast<long>(block / a.G) * 4;
+    }
+}
+inline long emit_tile(const dev::SweepArgs& a, int block) {
+    return (a.Bt == 1 && a.B == 1) ? static_cast<long>(block) * 4 : static_cast<long>(block / a.G) * dev::kEmitChunk * 2;
+}
+inline long gather_tile(const dev::SweepArgs& a, int block) {
+    return (a.Bt == 1 && a.B == 1) ? std::max(1L, static_cast<long>(block / a.GR) * 2) : static_cast<long>(block / a.G) * 2;
+}
+
+template <class S>
+void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, void* const* Y, int variant,
+                 const gmr_run_opts* opts, hipStream_t st) {
+    using T = typename S::T;
+    const int n = static_cast<int>(bt.eng.size());
+    std::vector<gmr_run_opts> o(n);
+    for (int g = 0; g < n; ++g) o[g] = opts ? opts[g] : gmr_run_opts{};
+    for (int g = 0; g < n; ++g) {
+        if (B[g] < 1 || !X[g] || !Y[g]) throw std::invalid_argument("batch: bad X, Y or B");
+        if (o[g].emit == GMR_EMIT_GATHER && !bt.eng[g]->resident)
+            throw std::invalid_argument("batch: gather emission needs the Resident layout");
+        if (o[g].eval_kind < 0 || o[g].eval_kind >= dev::kEvalKinds) throw std::invalid_argument("unknown eval kind");
+        if (o[g].level_kind)
+            for (int k = 1; k <= bt.eng[g]->L; ++k)
+                if (o[g].level_kind[k] < 0 || o[g].level_kind[k] >= dev::kEvalKinds)
+                    throw std::invalid_argument("unknown eval kind");
+    }
+    if (variant == GMR_VARIANT_LEVELS) {
+        for (int g = 0; g < n; ++g) {
+            gmr_run_opts og = o[g];
+            og.variant = GMR_VARIANT_LEVELS;
+            bt.eng[g]->run(semiring, X[g], B[g], Y[g], og, st);
+        }
+        return;
+    }
+    const int block = o[0].block > 0 ? o[0].block : 256;
+    std::vector<dev::SweepArgs> args(n);
+    for (int g = 0; g < n; ++g) {
+        bt.eng[g]->ensure_workspace(B[g], static_cast<int>(sizeof(T)));
+        args[g] = make_args(*bt.eng[g], X[g], B[g], Y[g], static_cast<int>(sizeof(T)), 0, B[g],
+                            o[g].emit == GMR_EMIT_GATHER, o[g]);
+    }
+    if (variant == GMR_VARIANT_GRAPH) {
+        std::size_t key = mix(static_cast<std::size_t>(semiring), static_cast<std::size_t>(block));
+        for (int g = 0; g < n; ++g) {
+            key = mix(key, reinterpret_cast<uintptr_t>(X[g]));
+            key = mix(key, reinterpret_cast<uintptr_t>(Y[g]));
+            key = mix(key, static_cast<std::size_t>(B[g]) * 4 + static_cast<std::size_t>(o[g].emit));
+            key = mix(key, static_cast<std::size_t>(kinds_hash(*bt.eng[g], o[g])));
+        }
+        auto it = bt.graphs.find(key);
+        if (it == bt.graphs.end()) {
+            if (!bt.capture_stream) GMR_HIP(hipStreamCreateWithFlags(&bt.capture_stream, hipStreamNonBlocking));
+            if (!bt.fork_ev) GMR_HIP(hipEventCreateWithFlags(&bt.fork_ev, hipEventDisableTiming));
+            while (static_cast<int>(bt.branch.size()) < n) {
+                hipStream_t s2;
+                hipEvent_t e2;
+                GMR_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+                GMR_HIP(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+                bt.branch.push_back(s2);
+                bt.branch_ev.push_back(e2);
+            }
+            // One branch per grammar, forked from and joined into the capture stream.
+            Engine::Graph gr;
+            GMR_HIP(hipStreamBeginCapture(bt.capture_stream, hipStreamCaptureModeThreadLocal));
+            try {
+                GMR_HIP(hipEventRecord(bt.fork_ev, bt.capture_stream));
+                for (int g = 0; g < n; ++g) {
+                    GMR_HIP(hipStreamWaitEvent(bt.branch[g], bt.fork_ev, 0));
+                    launch_levels<S>(*bt.eng[g], args[g], o[g], block, bt.branch[g]);
+                    GMR_HIP(hipEventRecord(bt.branch_ev[g], bt.branch[g]));
+                    GMR_HIP(hipStreamWaitEvent(bt.capture_stream, bt.branch_ev[g], 0));
+                }
+            } catch (...) {
+                hipGraph_t dummy;
+                (void)hipStreamEndCapture(bt.capture_stream, &dummy);
+                throw;
+            }
+            GMR_HIP(hipStreamEndCapture(bt.capture_stream, &gr.graph));
+            GMR_HIP(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
+            it = bt.graphs.emplace(key, gr).first;
+        }
+        GMR_HIP(hipGraphLaunch(it->second.exec, st));
+        return;
+    }
+    // Persistent: tile lists per phase.
+    int Lmax = 0;
+    for (int g = 0; g < n; ++g) Lmax = std::max(Lmax, bt.eng[g]->L);
+    const int phases = Lmax + 2;
+    std::size_t key = mix(static_cast<std::size_t>(block), static_cast<std::size_t>(semiring));
+    for (int g = 0; g < n; ++g) {
+        key = mix(key, reinterpret_cast<uintptr_t>(X[g]));
+        key = mix(key, reinterpret_cast<uintptr_t>(Y[g]));
+        key = mix(key, static_cast<std::size_t>(B[g]) * 4 + static_cast<std::size_t>(o[g].emit));
+        key = mix(key, static_cast<std::size_t>(kinds_hash(*bt.eng[g], o[g])));
+        key = mix(key, reinterpret_cast<uintptr_t>(args[g].Tv));
+        key = mix(key, reinterpret_cast<uintptr_t>(args[g].W0));
+    }
+    if (key != bt.tiles_key) {
+        std::vector<std::vector<dev::Tile>> per(phases);
+        for (int g = 0; g < n; ++g) {
+            const Engine& en = *bt.eng[g];
+            const dev::SweepArgs& a = args[g];
+            const long ni = dev::init_items(a);
+            for (long b0 = 0; b0 < ni; b0 += 8192) per[0].push_back({g, dev::TT_INIT, 0, 0, b0, std::min(ni, b0 + 8192)});
+            for (int p = 1; p <= en.L + 1; ++p) {
+                if (p <= en.L) {
+                    const int kind = host_kind(o[g], p);
+                    const long lo = en.level_off[p - 1], hi = en.level_off[p];
+                    const long step = eval_tile_rules(a, kind, block);
+                    for (long r = lo; r < hi; r += step) per[p].push_back({g, dev::TT_EVAL, p, kind, r, std::min(hi, r + step)});
+                }
+                if (!a.gather) {
+                    const long ne = p == 1 ? en.Et : en.emit_off[p - 1] - en.emit_off[p - 2];
+                    const long step = emit_tile(a, block);
+                    for (long e = 0; e < ne; e += step) per[p].push_back({g, dev::TT_EMIT, p - 1, 0, e, std::min(ne, e + step)});
+                } else if (p == en.L + 1) {
+                    const long step = gather_tile(a, block);
+                    for (long r = 0; r < en.rows; r += step)
+                        per[p].push_back({g, dev::TT_GATHER, 0, 0, r, std::min<long>(en.rows, r + step)});
+                }
+            }
+        }
+        std::vector<dev::Tile> tiles;
+        std::vector<long> off(phases + 1, 0);
+        for (int p = 0; p < phases; ++p) {
+            off[p] = static_cast<long>(tiles.size());
+            tiles.insert(tiles.end(), per[p].begin(), per[p].end());
+        }
+        off[phases] = static_cast<long>(tiles.size());
+        GMR_HIP(hipDeviceSynchronize());  // the old tables may still be in use
+        grow_upload(&bt.d_tiles, &bt.cap_tiles, tiles);
+        grow_upload(&bt.d_phase_off, &bt.cap_phase, off);
+        bt.tiles_key = key;
+        bt.last_tiles = static_cast<long>(tiles.size());
+        bt.last_phases = phases;
+    }
+    GMR_HIP(hipDeviceSynchronize());  // d_args may still be read by a queued launch
+    grow_upload(&bt.d_args, &bt.cap_args, args);
+    int occ = 0;
+    GMR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, dev::k_multi<S>, block, 0));
+    if (occ < 1) throw DeviceError("multi-grammar kernel does not fit on a CU");
+    const int per_cu = std::max(1, std::min(occ - 1, std::max(1, 1024 / block)));
+    int grid = per_cu * bt.num_cus;
+    if (o[0].grid_cap > 0) grid = std::min(grid, o[0].grid_cap);
+    const unsigned long long deadline = static_cast<unsigned long long>(bt.wall_khz) * 2000ULL;
+    hipLaunchKernelGGL(dev::k_multi<S>, dim3(grid), dim3(block), 0, st, bt.d_args, bt.d_tiles, bt.d_phase_off,
+                       bt.last_phases, bt.d_barrier, bt.barrier_base, deadline);
+    GMR_HIP(hipGetLastError());
+    bt.barrier_base += static_cast<unsigned long long>(grid) * static_cast<unsigned long long>(bt.last_phases - 1);
+}
+
+}  // namespace
 }  // namespace gmr
 
 // ---------------------------------------------------------------- C ABI
@@ -655,6 +874,72 @@ int gmr_engine_last_kernel_ms(const gmr_engine* h, double* ms) {
 }
 
 void gmr_engine_destroy(gmr_engine* h) { delete h; }
+
+struct gmr_batch {
+    gmr::Batch b;
+};
+
+int gmr_batch_create(gmr_engine* const* engines, int n, gmr_batch** out) {
+    if (!engines || n < 1 || !out) return dev_fail(GMR_ERR_INVALID, "batch: need engines");
+    for (int i = 0; i < n; ++i) {
+        if (!engines[i]) return dev_fail(GMR_ERR_INVALID, "batch: NULL engine");
+        for (int j = 0; j < i; ++j)
+            if (engines[j] == engines[i]) return dev_fail(GMR_ERR_INVALID, "batch: an engine appears twice");
+    }
+    gmr_batch* h = new (std::nothrow) gmr_batch;
+    if (!h) return GMR_ERR_NOMEM;
+    int rc = dev_guarded([&] {
+        for (int i = 0; i < n; ++i) h->b.eng.push_back(&engines[i]->e);
+        int dev = 0;
+        GMR_HIP(hipGetDevice(&dev));
+        GMR_HIP(hipDeviceGetAttribute(&h->b.num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+        GMR_HIP(hipDeviceGetAttribute(&h->b.wall_khz, hipDeviceAttributeWallClockRate, dev));
+        if (h->b.wall_khz <= 0) h->b.wall_khz = 100000;
+        GMR_HIP(hipMalloc(&h->b.d_barrier, sizeof(gmr::dev::GridBarrier)));
+        GMR_HIP(hipMemset(h->b.d_barrier, 0, sizeof(gmr::dev::GridBarrier)));
+        GMR_HIP(hipDeviceSynchronize());
+    });
+    if (rc != GMR_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return GMR_OK;
+}
+
+int gmr_batch_run(gmr_batch* h, int semiring, const void* const* d_X, const int* B, void* const* d_Y, int variant,
+                  const gmr_run_opts* opts, void* stream) {
+    if (!h || !d_X || !B || !d_Y) return dev_fail(GMR_ERR_INVALID, "NULL argument");
+    if (variant < 0 || variant > 2) return dev_fail(GMR_ERR_INVALID, "unknown variant");
+    return dev_guarded([&] {
+        const float* const* X = reinterpret_cast<const float* const*>(d_X);
+        auto st = static_cast<hipStream_t>(stream);
+        switch (semiring) {
+            case GMR_PLUS_TIMES: gmr::batch_run_t<gmr::dev::PlusTimesOp>(h->b, semiring, X, B, d_Y, variant, opts, st); break;
+            case GMR_BOOLEAN: gmr::batch_run_t<gmr::dev::BooleanOp>(h->b, semiring, X, B, d_Y, variant, opts, st); break;
+            case GMR_TROPICAL: gmr::batch_run_t<gmr::dev::TropicalOp>(h->b, semiring, X, B, d_Y, variant, opts, st); break;
+            default: throw std::invalid_argument("unknown semiring");
+        }
+    });
+}
+
+int gmr_batch_check(gmr_batch* h, int* barrier_error) {
+    if (!h || !barrier_error) return dev_fail(GMR_ERR_INVALID, "NULL argument");
+    return dev_guarded([&] {
+        gmr::dev::GridBarrier b{};
+        GMR_HIP(hipDeviceSynchronize());
+        GMR_HIP(hipMemcpy(&b, h->b.d_barrier, sizeof(b), hipMemcpyDeviceToHost));
+        *barrier_error = 0;
+        if (b.error || b.arrived != h->b.barrier_base) {
+            GMR_HIP(hipMemset(h->b.d_barrier, 0, sizeof(b)));
+            GMR_HIP(hipDeviceSynchronize());
+            h->b.barrier_base = 0;
+            *barrier_error = 1;
+        }
+    });
+}
+
+void gmr_batch_destroy(gmr_batch* h) { delete h; }
 
 int gmr_device_sync(void) {
     return hipDeviceSynchronize() == hipSuccess ? GMR_OK : GMR_ERR_DEVICE;

@@ -160,23 +160,32 @@ __host__ __device__ __forceinline__ int lanes_for(int Bt, int vec) {
 
 // T[p*Bt + b] = leaf(V, X[col*B + b]) (T[0..Bt) = zero); Y = zero for spot emits.
 template <class S>
-__device__ void ph_init(const SweepArgs& a, long tid, long nth) {
+__device__ void ph_init_range(const SweepArgs& a, long i0, long i1, long tid, long nth) {
     using T = typename S::T;
     T* Tt = tp<S>(a.Tv);
-    const long n = static_cast<long>(a.U) * a.Bt;
-    for (long i = tid; i < n; i += nth) {
-        int p = static_cast<int>(i / a.Bt);
-        int b = static_cast<int>(i - static_cast<long>(p) * a.Bt);
-        Tt[i] = p == 0 ? S::zero() : S::leaf(a.tval[p], a.X[static_cast<long>(a.tcol[p]) * a.B + b]);
-    }
-    if (!a.gather) {
-        T* Y = tp<S>(a.Y);
-        const long ny = static_cast<long>(a.rows) * a.Bt;
-        for (long i = tid; i < ny; i += nth) {
-            long r = i / a.Bt;
-            Y[r * a.B + (i - r * a.Bt)] = S::zero();
+    T* Y = tp<S>(a.Y);
+    const long nT = static_cast<long>(a.U) * a.Bt;
+    for (long i = i0 + tid; i < i1; i += nth) {
+        if (i < nT) {
+            int p = static_cast<int>(i / a.Bt);
+            int b = static_cast<int>(i - static_cast<long>(p) * a.Bt);
+            Tt[i] = p == 0 ? S::zero() : S::leaf(a.tval[p], a.X[static_cast<long>(a.tcol[p]) * a.B + b]);
+        } else {
+            long j = i - nT;
+            long r = j / a.Bt;
+            Y[r * a.B + (j - r * a.Bt)] = S::zero();
         }
     }
+}
+
+// Items of the init phase: U*Bt terminal values, then (spot emission) rows*Bt y values.
+__host__ __device__ __forceinline__ long init_items(const SweepArgs& a) {
+    return static_cast<long>(a.U) * a.Bt + (a.gather ? 0L : static_cast<long>(a.rows) * a.Bt);
+}
+
+template <class S>
+__device__ void ph_init(const SweepArgs& a, long tid, long nth) {
+    ph_init_range<S>(a, 0, init_items(a), tid, nth);
 }
 
 // Rules [i0, i1) of one level. src is the previous frontier (or the resident
@@ -333,13 +342,13 @@ __device__ __forceinline__ void ph_emit(const SweepArgs& a, const int2* em, int 
 // Resident layout, gather emission: Y[r] = (+) over row r's top-sequence symbols.
 // Scalar B-wide form: a group per row, lanes over vectors.
 template <class S>
-__device__ void ph_gather_rows(const SweepArgs& a, long gid, long ng, int lane) {
+__device__ void ph_gather_rows(const SweepArgs& a, long r0, long r1, long gid, long ng, int lane) {
     using T = typename S::T;
     const T* W = tp<S>(a.W0);
     const T* Tt = tp<S>(a.Tv);
     T* Y = tp<S>(a.Y);
     const int Bt = a.Bt;
-    for (long r = gid; r < a.rows; r += ng) {
+    for (long r = r0 + gid; r < r1; r += ng) {
         const int s0 = a.crow[r], s1 = a.crow[r + 1];
         for (int b = lane; b < Bt; b += a.G) {
             T acc = S::zero();
@@ -356,7 +365,7 @@ __device__ void ph_gather_rows(const SweepArgs& a, long gid, long ng, int lane) 
 // Four vectors per lane, symbols unrolled by 4 so each lane has four dwordx4
 // gathers in flight.
 template <class S>
-__device__ void ph_gather_rows_v4(const SweepArgs& a, long gid, long ng, int lane) {
+__device__ void ph_gather_rows_v4(const SweepArgs& a, long r0, long r1, long gid, long ng, int lane) {
     using T = typename S::T;
     using O = Op4<S>;
     using T4 = typename O::T4;
@@ -367,7 +376,7 @@ __device__ void ph_gather_rows_v4(const SweepArgs& a, long gid, long ng, int lan
     auto row = [&](int c) -> const T* {
         return c >= 0 ? W + static_cast<long>(c) * Bt : Tt + static_cast<long>(-c - 1) * Bt;
     };
-    for (long r = gid; r < a.rows; r += ng) {
+    for (long r = r0 + gid; r < r1; r += ng) {
         const int s0 = a.crow[r], s1 = a.crow[r + 1];
         for (int b = 4 * lane; b < Bt; b += 4 * a.G) {
             T4 acc0 = O::zero(), acc1 = O::zero();
@@ -387,7 +396,7 @@ __device__ void ph_gather_rows_v4(const SweepArgs& a, long gid, long ng, int lan
 // One vector: GR lanes per row stride over the row's symbols, then a
 // shuffle tree reduces the GR partial results.
 template <class S>
-__device__ void ph_gather_rows_1(const SweepArgs& a, long tid, long nth) {
+__device__ void ph_gather_rows_1(const SweepArgs& a, long r0, long r1, long tid, long nth) {
     using T = typename S::T;
     const T* W = tp<S>(a.W0);
     const T* Tt = tp<S>(a.Tv);
@@ -395,10 +404,12 @@ __device__ void ph_gather_rows_1(const SweepArgs& a, long tid, long nth) {
     const int GR = a.GR;
     const int sub = static_cast<int>(tid % GR);
     // All lanes of a wave run the same number of iterations (shuffles below).
-    const long nrows_pad = ((static_cast<long>(a.rows) + (nth / GR) - 1) / (nth / GR)) * (nth / GR);
-    for (long r = tid / GR; r < nrows_pad; r += nth / GR) {
+    const long span = r1 - r0;
+    const long pad = ((span + (nth / GR) - 1) / (nth / GR)) * (nth / GR);
+    for (long q = tid / GR; q < pad; q += nth / GR) {
+        const long r = r0 + q;
         T acc = S::zero();
-        if (r < a.rows) {
+        if (q < span) {
             const int s0 = a.crow[r], s1 = a.crow[r + 1];
             for (int j = s0 + sub; j < s1; j += GR) {
                 int c = a.csym[j];
@@ -406,19 +417,24 @@ __device__ void ph_gather_rows_1(const SweepArgs& a, long tid, long nth) {
             }
         }
         for (int d = GR / 2; d >= 1; d >>= 1) acc = S::oplus(acc, __shfl_xor(acc, d, 64));
-        if (sub == 0 && r < a.rows) Y[r] = acc;
+        if (sub == 0 && q < span) Y[r] = acc;
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void ph_gather_range(const SweepArgs& a, long r0, long r1, long tid, long nth) {
+    if (a.Bt == 1 && a.B == 1) {
+        ph_gather_rows_1<S>(a, r0, r1, tid, nth);
+    } else if (a.vec) {
+        ph_gather_rows_v4<S>(a, r0, r1, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+    } else {
+        ph_gather_rows<S>(a, r0, r1, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
     }
 }
 
 template <class S>
 __device__ __forceinline__ void ph_gather(const SweepArgs& a, long tid, long nth) {
-    if (a.Bt == 1 && a.B == 1) {
-        ph_gather_rows_1<S>(a, tid, nth);
-    } else if (a.vec) {
-        ph_gather_rows_v4<S>(a, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
-    } else {
-        ph_gather_rows<S>(a, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
-    }
+    ph_gather_range<S>(a, 0, a.rows, tid, nth);
 }
 
 // EV_THREAD: flattened (rule, chunk) items, chunk = 4 vectors (vec) or 1.
@@ -662,6 +678,62 @@ __global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long ba
         const int e0 = a.emit_off[a.L - 1], e1 = a.emit_off[a.L];
         const typename S::T* src = a.resident ? tp<S>(a.W0) : level_dst<S>(a, a.L);
         ph_emit<S>(a, a.emits + e0, e1 - e0, src, tid, nth, lane);
+    }
+}
+
+// ---- many grammars in one persistent launch ----
+//
+// The host lists, phase by phase, workgroup-sized tiles of every grammar's
+// work; a workgroup runs whole tiles (so a tile may use the grammar's own
+// per-level kernel, LDS staging included) and the bounded grid barrier
+// separates the phases. Phase 0 initialises every grammar; phase p >= 1 runs
+// level p of every grammar that has one together with the emissions of its
+// level p-1 (terminal emissions for p = 1); phase L_g + 1 holds grammar g's
+// last emissions or its row gather.
+enum TileTask : int { TT_INIT = 0, TT_EVAL = 1, TT_EMIT = 2, TT_GATHER = 3 };
+
+struct Tile {
+    int g;      // grammar
+    int task;   // TileTask
+    int level;  // TT_EVAL: level; TT_EMIT: level whose emissions (0 = terminals)
+    int kind;   // TT_EVAL: EvalKind
+    long begin, end;  // item range: init items, rules (absolute), emissions (within level), rows
+};
+
+template <class S>
+__device__ void run_tile(const SweepArgs& a, const Tile& t) {
+    const long tid = threadIdx.x, nth = blockDim.x;
+    switch (t.task) {
+        case TT_INIT: ph_init_range<S>(a, t.begin, t.end, tid, nth); break;
+        case TT_EVAL:
+            ph_eval_kind<S>(a, t.kind, static_cast<int>(t.begin), static_cast<int>(t.end), level_src<S>(a, t.level),
+                            level_dst<S>(a, t.level) + (t.begin - a.level_off[t.level - 1]) * a.Bt, tid, nth);
+            break;
+        case TT_EMIT: {
+            const int lane = static_cast<int>(tid % a.G);
+            if (t.level == 0) {
+                ph_emit<S>(a, a.term_emits + t.begin, static_cast<int>(t.end - t.begin), tp<S>(a.Tv), tid, nth, lane);
+            } else {
+                const int e0 = a.emit_off[t.level - 1];
+                const typename S::T* src = a.resident ? tp<S>(a.W0) : level_dst<S>(a, t.level);
+                ph_emit<S>(a, a.emits + e0 + t.begin, static_cast<int>(t.end - t.begin), src, tid, nth, lane);
+            }
+            break;
+        }
+        default: ph_gather_range<S>(a, t.begin, t.end, tid, nth);
+    }
+}
+
+template <class S>
+__global__ void k_multi(const SweepArgs* args, const Tile* tiles, const long* phase_off, int phases,
+                        GridBarrier* gb, unsigned long long base, unsigned long long deadline) {
+    const unsigned long long nb = gridDim.x;
+    for (int p = 0; p < phases; ++p) {
+        for (long t = phase_off[p] + blockIdx.x; t < phase_off[p + 1]; t += gridDim.x) {
+            const Tile tile = tiles[t];
+            run_tile<S>(args[tile.g], tile);
+        }
+        if (p + 1 < phases) grid_barrier(gb, base + nb * static_cast<unsigned long long>(p + 1), deadline);
     }
 }
 

@@ -340,3 +340,58 @@ class Engine:
         if getattr(self, "_h", None) and lib is not None:
             lib.gmr_engine_destroy(self._h)
             self._h = None
+
+
+class Batch:
+    """Several engines swept together: Y_g = M_g X_g with B_g vectors each
+    (gmr_batch_*). The engines must outlive the batch."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
+        h = C.c_void_p()
+        _check(_lib.dev().gmr_batch_create(arr, len(self.engines), C.byref(h)))
+        self._h = h
+
+    def run_torch(self, semiring, Xs, Ys=None, variant=VARIANT_PERSISTENT, opts=None, block=0, grid_cap=0):
+        """Xs: list of float32 CUDA tensors (cols_g, B_g); returns the Ys.
+        opts: optional list of per-grammar dicts (emit, eval_kind, level_kind)."""
+        import torch
+        if isinstance(semiring, str):
+            semiring = SEMIRINGS[semiring]
+        n = len(self.engines)
+        assert len(Xs) == n
+        dt = torch.float32 if semiring == PLUS_TIMES else torch.int32
+        if Ys is None:
+            Ys = [torch.empty((e.info()["rows"], X.shape[1]), dtype=dt, device=X.device)
+                  for e, X in zip(self.engines, Xs)]
+        Xp = (C.c_void_p * n)(*[X.data_ptr() for X in Xs])
+        Yp = (C.c_void_p * n)(*[Y.data_ptr() for Y in Ys])
+        Bs = np.ascontiguousarray([X.shape[1] for X in Xs], np.int32)
+        keep = []
+        os_ = (RunOpts * n)()
+        for g in range(n):
+            kw = dict(opts[g]) if opts else {}
+            if g == 0:
+                kw.setdefault("block", block)
+                kw.setdefault("grid_cap", grid_cap)
+            o, k = Engine._opts(**kw)
+            os_[g] = o
+            keep.append(k)
+        stream = torch.cuda.current_stream(Xs[0].device).cuda_stream
+        _check(_lib.dev().gmr_batch_run(self._h, semiring, Xp, Bs.ctypes.data_as(C.POINTER(C.c_int)), Yp, variant,
+                                        os_, C.c_void_p(stream)))
+        del keep
+        return Ys
+
+    def check(self):
+        err = C.c_int()
+        _check(_lib.dev().gmr_batch_check(self._h, C.byref(err)))
+        if err.value:
+            raise GmrError("batched persistent sweep: a grid-barrier wait timed out; results are invalid")
+
+    def __del__(self):
+        lib = getattr(_lib, "_dev", None) if _lib is not None else None
+        if getattr(self, "_h", None) and lib is not None:
+            lib.gmr_batch_destroy(self._h)
+            self._h = None

@@ -109,5 +109,9 @@ def dev():
     _sig(L, "gmr_engine_time_levels", C.c_int, [vp, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(RunOpts), C.c_int, c_float_p, C.c_void_p])
     _sig(L, "gmr_engine_destroy", None, [vp])
     _sig(L, "gmr_device_sync", C.c_int, [])
+    _sig(L, "gmr_batch_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int, vpp])
+    _sig(L, "gmr_batch_run", C.c_int, [vp, C.c_int, C.POINTER(C.c_void_p), c_int_p, C.POINTER(C.c_void_p), C.c_int, C.POINTER(RunOpts), C.c_void_p])
+    _sig(L, "gmr_batch_check", C.c_int, [vp, c_int_p])
+    _sig(L, "gmr_batch_destroy", None, [vp])
     _dev = L
     return L

@@ -103,6 +103,23 @@ int gmr_engine_time_levels(gmr_engine* e, int semiring, const void* d_X, int B, 
 void gmr_engine_destroy(gmr_engine* e);
 int gmr_device_sync(void);
 
+/* ---- many grammars swept together (uneven sizes, own B each) ---- */
+typedef struct gmr_batch gmr_batch;
+/* The engines stay owned by the caller and must outlive the batch; an engine
+ * may appear once. */
+int gmr_batch_create(gmr_engine* const* engines, int n, gmr_batch** out);
+/* Y_g = M_g X_g for every grammar g: d_X[g] is cols_g x B[g], d_Y[g] rows_g x B[g].
+ * variant: GMR_VARIANT_LEVELS runs the grammars one after another;
+ * GMR_VARIANT_GRAPH replays one hipGraph with a branch of per-level launches per
+ * grammar; GMR_VARIANT_PERSISTENT runs every grammar in one launch, level by
+ * level, with a grid barrier between levels. opts: n entries (emission, eval
+ * kinds per grammar; block and grid_cap of entry 0 apply to the launch) or NULL. */
+int gmr_batch_run(gmr_batch* b, int semiring, const void* const* d_X, const int* B, void* const* d_Y,
+                  int variant, const gmr_run_opts* opts, void* stream);
+/* As gmr_engine_check, for the batch's persistent launches. */
+int gmr_batch_check(gmr_batch* b, int* barrier_error);
+void gmr_batch_destroy(gmr_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

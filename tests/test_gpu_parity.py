@@ -293,3 +293,66 @@ def test_time_levels_reports_every_level(hap):
         t = eng.time_levels(0, X, Y, reps=2, eval_kind=kind)
         assert len(t) == eng.info()["max_depth"] + 1
         assert np.all(t >= 0) and t[1:].sum() > 0
+
+
+def test_batch_of_uneven_grammars():
+    """Grammars of different sizes, depths, layouts and B swept together in one
+    launch (persistent), one graph (branches) or one after another: each Y_g
+    equals its oracle product."""
+    rng = np.random.default_rng(11)
+    specs = [("hap", 1, 700, 300, 5), ("hap", 2, 2000, 512, 64), ("geno", 3, 3000, 400, 17),
+             ("rmat", 4, 10, 6000, 1), ("hap", 5, 64, 40, 128), ("geno", 6, 900, 256, 8)]
+    mats, engs, Xs, stars = [], [], [], []
+    for kind, seed, a, b, B in specs:
+        if kind == "hap":
+            m = gmr.Matrix.haplotype(seed, a, b, founders=12, seg_mean=24.0)
+        elif kind == "geno":
+            m = gmr.Matrix.genotype(seed, a, b, founders=32, classes=4, seg_mean=32.0)
+        else:
+            m = gmr.Matrix.rmat(seed, a, b)
+        g = m.repair()
+        seq, V = m.arrays()
+        lay = ["resident", "wave64", "refcompat"][seed % 3]
+        eng = g.schedule(m.rows, m.cols, lay).engine()
+        X = x_for(0, m.cols, seed, B=B)
+        Ys, den = O.brute_force_batched(0, m.rows, m.cols, seq, V, X)
+        mats.append((m, g, lay))
+        engs.append(eng)
+        Xs.append(torch.from_numpy(X).cuda())
+        stars.append((Ys, den))
+    batch = gmr.Batch(engs)
+    for var in ("levels", "graph", "persistent"):
+        for emit_gather in (False, True):
+            opts = []
+            for (m, g, lay), eng in zip(mats, engs):
+                L = eng.info()["max_depth"]
+                o = {"emit": gmr.EMIT_GATHER if (emit_gather and lay == "resident") else gmr.EMIT_SPOT,
+                     "level_kind": rng.integers(0, 4, L + 1).astype(np.int32)}
+                opts.append(o)
+            _say("batch", var, "gather" if emit_gather else "spot")
+            Ys = batch.run_torch(0, Xs, variant=gmr.VARIANTS[var], opts=opts)
+            torch.cuda.synchronize()
+            if var == "persistent":
+                batch.check()
+            for i, (Y, (Ystar, den)) in enumerate(zip(Ys, stars)):
+                check(0, Y.cpu().numpy(), Ystar, den, f"batch {var} grammar {i}")
+
+
+def test_batch_integer_semirings():
+    ms = [gmr.Matrix.rmat(40 + i, 9 + i % 3, 3000 * (i + 1), nweights=1 + 99 * (i % 2)) for i in range(4)]
+    engs, Xs, stars = [], [], []
+    for i, m in enumerate(ms):
+        g = m.repair()
+        seq, V = m.arrays()
+        engs.append(g.schedule(m.rows, m.cols, "resident").engine())
+        X = x_for(2, m.cols, i, B=4 + i)
+        Xs.append(torch.from_numpy(X).cuda())
+        stars.append(O.brute_force_batched(2, m.rows, m.cols, seq, V, X)[0])
+    batch = gmr.Batch(engs)
+    for var in ("graph", "persistent"):
+        Ys = batch.run_torch(2, Xs, variant=gmr.VARIANTS[var], opts=[{"emit": gmr.EMIT_GATHER}] * 4)
+        torch.cuda.synchronize()
+        if var == "persistent":
+            batch.check()
+        for Y, Ys_ in zip(Ys, stars):
+            check(2, Y.cpu().numpy(), Ys_, None, f"batch tropical {var}")
