@@ -321,7 +321,7 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         const long threads = (a.Bt == 1 && a.B == 1) ? static_cast<long>(en.rows) * a.GR : static_cast<long>(en.rows) * G;
         hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(threads, block)), dim3(block), 0, st, a);
     }
-    (void)o;
+    if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * en.L + 1], st));
     GMR_HIP(hipGetLastError());
 }
 
@@ -508,24 +508,29 @@ void time_levels_t(Engine& en, const float* X, int B, void* Y, const gmr_run_opt
     a.eval_kind = o.eval_kind;
     a.level_kind = o.level_kind ? en.device_kinds(o.level_kind) : nullptr;
     const int block = o.block > 0 ? o.block : 256;
-    std::vector<hipEvent_t> ev(2 * en.L + 1);
+    std::vector<hipEvent_t> ev(2 * en.L + 2);
     for (auto& e : ev) GMR_HIP(hipEventCreate(&e));
-    std::vector<double> acc(en.L + 1, 0.0);
+    // out[0]: start to the first level (init, terminal emits); out[k]: level k's
+    // node update; out[L+1]: after the last node update to the end (the last
+    // level's emits, or the row gather).
+    std::vector<double> acc(en.L + 2, 0.0);
     for (int r = 0; r < reps; ++r) {
         launch_levels<S>(en, a, o, block, st, ev.data());
         GMR_HIP(hipStreamSynchronize(st));
         float ms = 0;
-        if (en.L > 0) {
-            GMR_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-            acc[0] += ms;
-        }
+        GMR_HIP(hipEventElapsedTime(&ms, ev[0], en.L > 0 ? ev[1] : ev[2 * en.L + 1]));
+        acc[0] += ms;
         for (int k = 1; k <= en.L; ++k) {
             GMR_HIP(hipEventElapsedTime(&ms, ev[2 * k - 1], ev[2 * k]));
             acc[k] += ms;
         }
+        if (en.L > 0) {
+            GMR_HIP(hipEventElapsedTime(&ms, ev[2 * en.L], ev[2 * en.L + 1]));
+            acc[en.L + 1] += ms;
+        }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    for (int k = 0; k <= en.L; ++k) out[k] = static_cast<float>(acc[k] / std::max(reps, 1));
+    for (int k = 0; k <= en.L + 1; ++k) out[k] = static_cast<float>(acc[k] / std::max(reps, 1));
 }
 }  // namespace
 

@@ -289,14 +289,14 @@ class Engine:
         del keep
 
     def time_levels(self, semiring, X, Y, reps=3, **kw):
-        """Per-level node-update device times (ms), index 0 = init + terminal
-        emits; X, Y torch device tensors."""
+        """Device times (ms): [0] init and terminal emits, [k] level k's node
+        update, [L+1] the last emits or the row gather. X, Y torch tensors."""
         import torch
         if isinstance(semiring, str):
             semiring = SEMIRINGS[semiring]
         B = 1 if X.dim() == 1 else X.shape[1]
         L = self.info()["max_depth"]
-        out = np.zeros(L + 1, np.float32)
+        out = np.zeros(L + 2, np.float32)
         o, keep = self._opts(**kw)
         stream = torch.cuda.current_stream(X.device).cuda_stream
         _check(_lib.dev().gmr_engine_time_levels(self._h, semiring, C.c_void_p(X.data_ptr()), B,

@@ -113,5 +113,13 @@ def dev():
     _sig(L, "gmr_batch_run", C.c_int, [vp, C.c_int, C.POINTER(C.c_void_p), c_int_p, C.POINTER(C.c_void_p), C.c_int, C.POINTER(RunOpts), C.c_void_p])
     _sig(L, "gmr_batch_check", C.c_int, [vp, c_int_p])
     _sig(L, "gmr_batch_destroy", None, [vp])
+    _sig(L, "gmr_mlp_create", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, vpp])
+    _sig(L, "gmr_mlp_num_params", C.c_longlong, [vp])
+    _sig(L, "gmr_mlp_get_params", C.c_int, [vp, c_float_p])
+    _sig(L, "gmr_mlp_set_params", C.c_int, [vp, c_float_p])
+    _sig(L, "gmr_mlp_forward", C.c_int, [vp, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p])
+    _sig(L, "gmr_mlp_loss_grad", C.c_int, [vp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, c_float_p, c_float_p])
+    _sig(L, "gmr_mlp_train", C.c_int, [vp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_float, c_float_p])
+    _sig(L, "gmr_mlp_destroy", None, [vp])
     _dev = L
     return L

@@ -19,6 +19,8 @@
 #ifndef GMR_H
 #define GMR_H
 
+#include <stdint.h>
+
 #include "gmr_host.h"
 #include "layered_types.h"
 
@@ -94,10 +96,12 @@ int gmr_engine_debug(gmr_engine* e, long long out[8]);
 /* Device time of the last gmr_engine_run on this engine, when timing was
  * requested with GMR_RUN_TIMED in opts->flags (that run synchronises its stream). */
 int gmr_engine_last_kernel_ms(const gmr_engine* e, double* ms);
-/* Per-level device times of the node-update launches: runs the levels
- * variant `reps` times with events around every level's eval launch, using the
- * eval kinds in opts; level_ms has max_depth + 1 entries ([0] = terminal
- * initialisation and terminal emissions). Synchronises the stream. */
+/* Per-level device times: runs the levels variant `reps` times with events
+ * around every level's node-update launch, using the eval kinds in opts (no
+ * vector tiling: B is one tile). level_ms has max_depth + 2 entries: [0] from
+ * the start to the first level (terminal values, terminal emits), [k] level k's
+ * node update, [max_depth+1] from the last node update to the end (the last
+ * level's emits, or the row gather). Synchronises the stream. */
 int gmr_engine_time_levels(gmr_engine* e, int semiring, const void* d_X, int B, void* d_Y,
                            const gmr_run_opts* opts, int reps, float* level_ms, void* stream);
 void gmr_engine_destroy(gmr_engine* e);
@@ -119,6 +123,26 @@ int gmr_batch_run(gmr_batch* b, int semiring, const void* const* d_X, const int*
 /* As gmr_engine_check, for the batch's persistent launches. */
 int gmr_batch_check(gmr_batch* b, int* barrier_error);
 void gmr_batch_destroy(gmr_batch* b);
+
+/* ---- the schedule proposer's network, trained on the GPU by its own kernels ----
+ * in -> H -> H -> out, ReLU on both hidden layers, masked mean-squared error,
+ * Adam. Parameters are one FP32 array: W1 [H x in], b1 [H], W2 [H x H], b2 [H],
+ * W3 [out x H], b3 [out], row-major. d_X is N x in, d_T and d_M (0/1 mask) are
+ * N x out, all device pointers; count = number of unmasked targets. No
+ * counterpart in the reference: it chooses launch schedules only. */
+typedef struct gmr_mlp gmr_mlp;
+int gmr_mlp_create(int n_in, int n_hidden, int n_out, uint64_t seed, gmr_mlp** out);
+long long gmr_mlp_num_params(const gmr_mlp* m);
+int gmr_mlp_get_params(const gmr_mlp* m, float* host_params);
+int gmr_mlp_set_params(gmr_mlp* m, const float* host_params);
+int gmr_mlp_forward(gmr_mlp* m, const float* d_X, int N, float* d_out, void* stream);
+/* Loss (mean over unmasked targets) and its gradient (host copy, may be NULL). */
+int gmr_mlp_loss_grad(gmr_mlp* m, const float* d_X, const float* d_T, const float* d_M, int N, float count,
+                      float* loss, float* host_grad);
+/* `epochs` full-batch Adam steps at learning rate lr; losses[e] (may be NULL). */
+int gmr_mlp_train(gmr_mlp* m, const float* d_X, const float* d_T, const float* d_M, int N, float count,
+                  int epochs, float lr, float* losses);
+void gmr_mlp_destroy(gmr_mlp* m);
 
 #ifdef __cplusplus
 }

@@ -20,6 +20,8 @@
  *   orc_sweep            orc_brute_force on the same inputs (exact for Boolean and
  *                        Tropical, 1e-12 relative for FP64)
  *   orc_cpu_layered_f32  orc_sweep on the same schedule (FP32 tolerance)
+ *   orc_mlp_loss_grad    central finite differences of its own loss (the
+ *                        proposer network has no reference counterpart)
  */
 #include <math.h>
 #include <stdint.h>
@@ -475,4 +477,78 @@ void orc_cpu_layered_f32(int semiring, int rows, int U, const int* val_idx, cons
     free(T);
     free(rd);
     free(wr);
+}
+
+/* The schedule proposer's network in FP64, for checking the GPU kernels:
+ * in -> H -> H -> out, ReLU on the hidden layers, loss = sum of
+ * mask*(out - target)^2 / count. Parameters as in the engine (W1 [H x in], b1,
+ * W2 [H x H], b2, W3 [out x H], b3). Writes the loss and, if grad != NULL,
+ * d loss / d params. There is no reference counterpart; pinned by central
+ * finite differences in the tests. */
+double orc_mlp_loss_grad(int nin, int nh, int nout, const double* p, const double* X, const double* T,
+                         const double* M, int N, double count, double* grad, double* out) {
+    const double *W1 = p, *b1 = W1 + (long)nh * nin, *W2 = b1 + nh, *b2 = W2 + (long)nh * nh,
+                 *W3 = b2 + nh, *b3 = W3 + (long)nout * nh;
+    double* z1 = (double*)malloc(sizeof(double) * nh);
+    double* a1 = (double*)malloc(sizeof(double) * nh);
+    double* z2 = (double*)malloc(sizeof(double) * nh);
+    double* a2 = (double*)malloc(sizeof(double) * nh);
+    double* o = (double*)malloc(sizeof(double) * nout);
+    double* dO = (double*)malloc(sizeof(double) * nout);
+    double* d2 = (double*)malloc(sizeof(double) * nh);
+    double* d1 = (double*)malloc(sizeof(double) * nh);
+    long np = (long)nh * nin + nh + (long)nh * nh + nh + (long)nout * nh + nout;
+    if (grad) memset(grad, 0, sizeof(double) * np);
+    double *gW1 = grad, *gb1 = grad ? gW1 + (long)nh * nin : 0, *gW2 = grad ? gb1 + nh : 0,
+           *gb2 = grad ? gW2 + (long)nh * nh : 0, *gW3 = grad ? gb2 + nh : 0, *gb3 = grad ? gW3 + (long)nout * nh : 0;
+    double loss = 0.0;
+    for (int n = 0; n < N; ++n) {
+        const double* x = X + (long)n * nin;
+        for (int h = 0; h < nh; ++h) {
+            double z = b1[h];
+            for (int i = 0; i < nin; ++i) z += W1[(long)h * nin + i] * x[i];
+            z1[h] = z;
+            a1[h] = z > 0 ? z : 0;
+        }
+        for (int h = 0; h < nh; ++h) {
+            double z = b2[h];
+            for (int i = 0; i < nh; ++i) z += W2[(long)h * nh + i] * a1[i];
+            z2[h] = z;
+            a2[h] = z > 0 ? z : 0;
+        }
+        for (int k = 0; k < nout; ++k) {
+            double z = b3[k];
+            for (int i = 0; i < nh; ++i) z += W3[(long)k * nh + i] * a2[i];
+            o[k] = z;
+            if (out) out[(long)n * nout + k] = z;
+            double d = (z - T[(long)n * nout + k]) * M[(long)n * nout + k];
+            loss += d * d;
+            dO[k] = 2.0 * d / count;
+        }
+        if (!grad) continue;
+        for (int k = 0; k < nout; ++k) {
+            gb3[k] += dO[k];
+            for (int i = 0; i < nh; ++i) gW3[(long)k * nh + i] += dO[k] * a2[i];
+        }
+        for (int i = 0; i < nh; ++i) {
+            double g = 0;
+            for (int k = 0; k < nout; ++k) g += dO[k] * W3[(long)k * nh + i];
+            d2[i] = z2[i] > 0 ? g : 0;
+        }
+        for (int h = 0; h < nh; ++h) {
+            gb2[h] += d2[h];
+            for (int i = 0; i < nh; ++i) gW2[(long)h * nh + i] += d2[h] * a1[i];
+        }
+        for (int i = 0; i < nh; ++i) {
+            double g = 0;
+            for (int h = 0; h < nh; ++h) g += d2[h] * W2[(long)h * nh + i];
+            d1[i] = z1[i] > 0 ? g : 0;
+        }
+        for (int h = 0; h < nh; ++h) {
+            gb1[h] += d1[h];
+            for (int i = 0; i < nin; ++i) gW1[(long)h * nin + i] += d1[h] * x[i];
+        }
+    }
+    free(z1); free(a1); free(z2); free(a2); free(o); free(dO); free(d2); free(d1);
+    return loss / count;
 }

@@ -179,3 +179,20 @@ def cpu_layered_f32(semiring, s, rows, cols, V, x):
                               C.c_longlong(len(s["term_sym"])), _p(s["term_sym"], C.c_int),
                               _p(s["term_row"], C.c_int), y.ctypes.data_as(C.c_void_p))
     return y
+
+
+def mlp_loss_grad(nin, nh, nout, params, X, T, M, count=None):
+    """FP64 loss, gradient and outputs of the proposer network (see oracle.c)."""
+    p = np.ascontiguousarray(params, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    T = np.ascontiguousarray(T, np.float64)
+    M = np.ascontiguousarray(M, np.float64)
+    N = X.shape[0]
+    count = float(M.sum()) if count is None else float(count)
+    g = np.zeros_like(p)
+    out = np.zeros((N, nout), np.float64)
+    f = lib().orc_mlp_loss_grad
+    f.restype = C.c_double
+    loss = f(C.c_int(nin), C.c_int(nh), C.c_int(nout), _p(p, C.c_double), _p(X, C.c_double), _p(T, C.c_double),
+             _p(M, C.c_double), C.c_int(N), C.c_double(count), _p(g, C.c_double), _p(out, C.c_double))
+    return loss, g, out

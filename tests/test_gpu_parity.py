@@ -291,8 +291,8 @@ def test_time_levels_reports_every_level(hap):
     Y = torch.empty(4096, 16, device="cuda")
     for kind in range(4):
         t = eng.time_levels(0, X, Y, reps=2, eval_kind=kind)
-        assert len(t) == eng.info()["max_depth"] + 1
-        assert np.all(t >= 0) and t[1:].sum() > 0
+        assert len(t) == eng.info()["max_depth"] + 2
+        assert np.all(t >= 0) and t[1:-1].sum() > 0
 
 
 def test_batch_of_uneven_grammars():

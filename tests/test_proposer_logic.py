@@ -1,0 +1,51 @@
+"""Decision logic of the schedule proposer on synthetic timing tables (CPU)."""
+import math
+
+import numpy as np
+
+from gmr import proposer as P
+
+
+def _table():
+    # two tiles, 3 levels; plan 3 (graph/gather) is fastest at tile 64
+    ev64 = np.array([[1.0, 0.9, 3.0, np.nan], [2.0, 2.1, 5.0, np.nan], [0.5, 0.4, 0.6, np.nan]])
+    ev16 = ev64 * 0.3
+    return {64: {"eval": ev64, "plan": np.array([10.0, 6.0, 9.0, 5.0, 8.0, 5.5])},
+            16: {"eval": ev16, "plan": np.array([3.0, 1.9, 2.8, 1.7, 2.5, 1.8])}}
+
+
+def test_sweep_cost_counts_passes_and_kernel_deltas():
+    t = _table()
+    c = P.sweep_cost(t, 64, 16, 3, [1, 0, 1])
+    delta = (0.9 - 1.0) * 0.3 + 0.0 + (0.4 - 0.5) * 0.3
+    assert math.isclose(c, 4 * (1.7 + delta))
+
+
+def test_exhaustive_and_heuristic():
+    t = _table()
+    cost, Bt, plan, kinds = P.exhaustive(t, 64)
+    assert (Bt, plan, kinds) == (64, 3, [1, 0, 1])
+    assert math.isclose(cost, 5.0 - 0.1 - 0.1)
+    h = P.heuristic(t, 64)
+    assert h[1] == 64 and h[2] == P.DEFAULT_PLAN and h[3] == [P.EVAL_THREAD] * 3
+    assert math.isclose(h[0], 10.0 + (0.9 - 1.0) + (2.1 - 2.0) + (0.4 - 0.5))
+
+
+def test_regret_of_the_optimum_is_zero():
+    t = _table()
+    c, Bt, plan, kinds = P.exhaustive(t, 64)
+    assert P.regret(t, 64, (Bt, plan, kinds, 0.0)) == 0.0
+    assert P.regret(t, 64, (64, 0, kinds, 0.0)) > 0.0
+
+
+def test_features_shape_and_tiles():
+    assert P.tile_candidates(100) == [16, 32, 64, 100]
+    assert P.tile_candidates(1) == [1]
+    import gmr
+    m = gmr.Matrix.haplotype(3, 300, 128, founders=8, seg_mean=16.0)
+    g = m.repair()
+    arr = g.schedule(300, 128, "resident").arrays()
+    F, gs = P.features(arr, 16)
+    assert F.shape == (arr["info"]["max_depth"], P.N_FEAT) and gs.shape == (P.N_FEAT,)
+    assert np.all(np.isfinite(F)) and np.all(np.isfinite(gs))
+    assert gs[0] == 1.0 and np.all(F[:, 0] == 0.0)
