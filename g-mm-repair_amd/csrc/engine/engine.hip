@@ -67,7 +67,7 @@ Engine::~Engine() {
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
         if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
-    void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_level_off,
+    void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_level_off,
                     d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -172,6 +172,8 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
 
     // Resident layout: the row-ordered top sequence for gather emission.
     std::vector<int> crow, csym;
+    std::vector<int4> gseg;
+    gather_split = 0;
     if (resident) {
         crow.assign(static_cast<std::size_t>(rows) + 1, 0);
         for (const int2& p : em) ++crow[p.y + 1];
@@ -182,11 +184,22 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
         // Terminals first, then levels in order (C order is not needed).
         for (const int2& p : te) csym[fill[p.y]++] = -p.x - 1;
         for (const int2& p : em) csym[fill[p.y]++] = p.x;
+        // Gather work items: rows cut into segments of at most kGatherSeg
+        // symbols, so a few long rows do not serialise the gather. A segment
+        // that covers its whole row stores its sum; the pieces of a split row
+        // combine atomically into a zeroed y row.
+        for (int r = 0; r < rows; ++r) {
+            const int a0 = crow[r], a1 = crow[r + 1];
+            const int split = (a1 - a0) > kGatherSeg ? 1 : 0;
+            gather_split |= split;
+            if (a1 == a0) gseg.push_back(make_int4(r, a0, a1, 0));
+            for (int q = a0; q < a1; q += kGatherSeg) gseg.push_back(make_int4(r, q, std::min(a1, q + kGatherSeg), split));
+        }
     }
 
     // Single-vector gather: lanes per row ~ a quarter of the mean row length.
     {
-        const double mean = rows > 0 ? static_cast<double>(E + Et) / rows : 0.0;
+        const double mean = !gseg.empty() ? static_cast<double>(E + Et) / gseg.size() : 0.0;
         gather_lanes = 1;
         while (gather_lanes < 64 && gather_lanes * 4 < mean) gather_lanes <<= 1;
     }
@@ -200,6 +213,8 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
     if (resident) {
         d_crow = dev_upload(crow.data(), crow.size(), &struct_bytes);
         d_csym = dev_upload(csym.data(), csym.size(), &struct_bytes);
+        d_gseg = dev_upload(gseg.data(), gseg.size(), &struct_bytes);
+        n_gseg = static_cast<int>(gseg.size());
     }
     d_level_off = dev_upload(level_off.data(), level_off.size(), &struct_bytes);
     d_emit_off = dev_upload(emit_off.data(), emit_off.size(), &struct_bytes);
@@ -318,7 +333,7 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         }
     }
     if (a.gather) {
-        const long threads = (a.Bt == 1 && a.B == 1) ? static_cast<long>(en.rows) * a.GR : static_cast<long>(en.rows) * G;
+        const long threads = (a.Bt == 1 && a.B == 1) ? static_cast<long>(en.n_gseg) * a.GR : static_cast<long>(en.n_gseg) * G;
         hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(threads, block)), dim3(block), 0, st, a);
     }
     if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * en.L + 1], st));
@@ -393,6 +408,9 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
     a.term_emits = en.d_term;
     a.crow = en.d_crow;
     a.csym = en.d_csym;
+    a.gseg = en.d_gseg;
+    a.n_gseg = en.n_gseg;
+    a.gsplit = en.gather_split;
     a.level_off = en.d_level_off;
     a.emit_off = en.d_emit_off;
     a.X = X + b0;
@@ -486,6 +504,9 @@ void time_levels_t(Engine& en, const float* X, int B, void* Y, const gmr_run_opt
     a.term_emits = en.d_term;
     a.crow = en.d_crow;
     a.csym = en.d_csym;
+    a.gseg = en.d_gseg;
+    a.n_gseg = en.n_gseg;
+    a.gsplit = en.gather_split;
     a.level_off = en.d_level_off;
     a.emit_off = en.d_emit_off;
     a.X = X;
@@ -722,8 +743,8 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
                     for (long e = 0; e < ne; e += step) per[p].push_back({g, dev::TT_EMIT, p - 1, 0, e, std::min(ne, e + step)});
                 } else if (p == en.L + 1) {
                     const long step = gather_tile(a, block);
-                    for (long r = 0; r < en.rows; r += step)
-                        per[p].push_back({g, dev::TT_GATHER, 0, 0, r, std::min<long>(en.rows, r + step)});
+                    for (long r = 0; r < en.n_gseg; r += step)
+                        per[p].push_back({g, dev::TT_GATHER, 0, 0, r, std::min<long>(en.n_gseg, r + step)});
                 }
             }
         }

@@ -33,6 +33,10 @@ struct Engine {
     int2* d_term = nullptr;
     int* d_crow = nullptr;
     int* d_csym = nullptr;
+    int4* d_gseg = nullptr;          // gather segments (row, first, end, split)
+    int n_gseg = 0;
+    int gather_split = 0;            // some row is split over several segments
+    static constexpr int kGatherSeg = 128;
     int* d_level_off = nullptr;
     int* d_emit_off = nullptr;
 

@@ -67,6 +67,9 @@ struct SweepArgs {
     const int2* term_emits;  // (sym, row) per terminal occurrence, row-sorted
     const int* crow;         // gather mode: row r's symbols are csym[crow[r], crow[r+1])
     const int* csym;         // >= 0 global slot of a rule, < 0 terminal -c-1
+    const int4* gseg;        // gather segments: (row, first symbol, end, row is split)
+    int n_gseg;
+    int gsplit;              // some row is split: y is zeroed and pieces add atomically
     const int* level_off;    // max_depth + 1
     const int* emit_off;     // max_depth + 1
     // values (device)
@@ -178,9 +181,10 @@ __device__ void ph_init_range(const SweepArgs& a, long i0, long i1, long tid, lo
     }
 }
 
-// Items of the init phase: U*Bt terminal values, then (spot emission) rows*Bt y values.
+// Items of the init phase: U*Bt terminal values, then (spot emission, or a
+// gather with split rows) rows*Bt y values.
 __host__ __device__ __forceinline__ long init_items(const SweepArgs& a) {
-    return static_cast<long>(a.U) * a.Bt + (a.gather ? 0L : static_cast<long>(a.rows) * a.Bt);
+    return static_cast<long>(a.U) * a.Bt + ((a.gather && !a.gsplit) ? 0L : static_cast<long>(a.rows) * a.Bt);
 }
 
 template <class S>
@@ -339,25 +343,34 @@ __device__ __forceinline__ void ph_emit(const SweepArgs& a, const int2* em, int 
     }
 }
 
-// Resident layout, gather emission: Y[r] = (+) over row r's top-sequence symbols.
-// Scalar B-wide form: a group per row, lanes over vectors.
+// Resident layout, gather emission: Y[r] = (+) over row r's top-sequence
+// symbols, computed per segment (a run of at most 128 symbols of one row). A
+// segment that is its row's only one stores the sum; the segments of a split
+// row add into the zeroed row with the semiring's atomic.
 template <class S>
-__device__ void ph_gather_rows(const SweepArgs& a, long r0, long r1, long gid, long ng, int lane) {
+__device__ __forceinline__ void gather_out(typename S::T* y, typename S::T v, int split) {
+    if (!split) *y = v;
+    else if (!S::skip(v)) S::emit(y, v);
+}
+
+// Scalar B-wide form: a lane group per segment, lanes over vectors.
+template <class S>
+__device__ void ph_gather_rows(const SweepArgs& a, long q0, long q1, long gid, long ng, int lane) {
     using T = typename S::T;
     const T* W = tp<S>(a.W0);
     const T* Tt = tp<S>(a.Tv);
     T* Y = tp<S>(a.Y);
     const int Bt = a.Bt;
-    for (long r = r0 + gid; r < r1; r += ng) {
-        const int s0 = a.crow[r], s1 = a.crow[r + 1];
+    for (long q = q0 + gid; q < q1; q += ng) {
+        const int4 sg = a.gseg[q];
         for (int b = lane; b < Bt; b += a.G) {
             T acc = S::zero();
-            for (int j = s0; j < s1; ++j) {
+            for (int j = sg.y; j < sg.z; ++j) {
                 int c = a.csym[j];
                 T v = c >= 0 ? W[static_cast<long>(c) * Bt + b] : Tt[static_cast<long>(-c - 1) * Bt + b];
                 acc = S::oplus(acc, v);
             }
-            Y[r * a.B + b] = acc;
+            gather_out<S>(Y + static_cast<long>(sg.x) * a.B + b, acc, sg.w);
         }
     }
 }
@@ -365,7 +378,7 @@ __device__ void ph_gather_rows(const SweepArgs& a, long r0, long r1, long gid, l
 // Four vectors per lane, symbols unrolled by 4 so each lane has four dwordx4
 // gathers in flight.
 template <class S>
-__device__ void ph_gather_rows_v4(const SweepArgs& a, long r0, long r1, long gid, long ng, int lane) {
+__device__ void ph_gather_rows_v4(const SweepArgs& a, long q0, long q1, long gid, long ng, int lane) {
     using T = typename S::T;
     using O = Op4<S>;
     using T4 = typename O::T4;
@@ -376,27 +389,30 @@ __device__ void ph_gather_rows_v4(const SweepArgs& a, long r0, long r1, long gid
     auto row = [&](int c) -> const T* {
         return c >= 0 ? W + static_cast<long>(c) * Bt : Tt + static_cast<long>(-c - 1) * Bt;
     };
-    for (long r = r0 + gid; r < r1; r += ng) {
-        const int s0 = a.crow[r], s1 = a.crow[r + 1];
+    for (long q = q0 + gid; q < q1; q += ng) {
+        const int4 sg = a.gseg[q];
         for (int b = 4 * lane; b < Bt; b += 4 * a.G) {
             T4 acc0 = O::zero(), acc1 = O::zero();
-            int j = s0;
-            for (; j + 3 < s1; j += 4) {
+            int j = sg.y;
+            for (; j + 3 < sg.z; j += 4) {
                 const int c0 = a.csym[j], c1 = a.csym[j + 1], c2 = a.csym[j + 2], c3 = a.csym[j + 3];
                 T4 v0 = O::ld(row(c0) + b), v1 = O::ld(row(c1) + b), v2 = O::ld(row(c2) + b), v3 = O::ld(row(c3) + b);
                 acc0 = O::oplus(acc0, O::oplus(v0, v1));
                 acc1 = O::oplus(acc1, O::oplus(v2, v3));
             }
-            for (; j < s1; ++j) acc0 = O::oplus(acc0, O::ld(row(a.csym[j]) + b));
-            O::st(Y + r * a.B + b, O::oplus(acc0, acc1));
+            for (; j < sg.z; ++j) acc0 = O::oplus(acc0, O::ld(row(a.csym[j]) + b));
+            T* y = Y + static_cast<long>(sg.x) * a.B + b;
+            const T4 v = O::oplus(acc0, acc1);
+            if (!sg.w) O::st(y, v);
+            else O::emit(y, v);
         }
     }
 }
 
-// One vector: GR lanes per row stride over the row's symbols, then a
-// shuffle tree reduces the GR partial results.
+// One vector: GR lanes per segment stride over its symbols, then a shuffle
+// tree reduces the GR partial results.
 template <class S>
-__device__ void ph_gather_rows_1(const SweepArgs& a, long r0, long r1, long tid, long nth) {
+__device__ void ph_gather_rows_1(const SweepArgs& a, long q0, long q1, long tid, long nth) {
     using T = typename S::T;
     const T* W = tp<S>(a.W0);
     const T* Tt = tp<S>(a.Tv);
@@ -404,20 +420,20 @@ __device__ void ph_gather_rows_1(const SweepArgs& a, long r0, long r1, long tid,
     const int GR = a.GR;
     const int sub = static_cast<int>(tid % GR);
     // All lanes of a wave run the same number of iterations (shuffles below).
-    const long span = r1 - r0;
+    const long span = q1 - q0;
     const long pad = ((span + (nth / GR) - 1) / (nth / GR)) * (nth / GR);
-    for (long q = tid / GR; q < pad; q += nth / GR) {
-        const long r = r0 + q;
+    for (long i = tid / GR; i < pad; i += nth / GR) {
         T acc = S::zero();
-        if (q < span) {
-            const int s0 = a.crow[r], s1 = a.crow[r + 1];
-            for (int j = s0 + sub; j < s1; j += GR) {
+        int4 sg = make_int4(0, 0, 0, 0);
+        if (i < span) {
+            sg = a.gseg[q0 + i];
+            for (int j = sg.y + sub; j < sg.z; j += GR) {
                 int c = a.csym[j];
                 acc = S::oplus(acc, c >= 0 ? W[c] : Tt[-c - 1]);
             }
         }
         for (int d = GR / 2; d >= 1; d >>= 1) acc = S::oplus(acc, __shfl_xor(acc, d, 64));
-        if (sub == 0 && q < span) Y[r] = acc;
+        if (sub == 0 && i < span) gather_out<S>(Y + sg.x, acc, sg.w);
     }
 }
 
@@ -434,7 +450,7 @@ __device__ __forceinline__ void ph_gather_range(const SweepArgs& a, long r0, lon
 
 template <class S>
 __device__ __forceinline__ void ph_gather(const SweepArgs& a, long tid, long nth) {
-    ph_gather_range<S>(a, 0, a.rows, tid, nth);
+    ph_gather_range<S>(a, 0, a.n_gseg, tid, nth);
 }
 
 // EV_THREAD: flattened (rule, chunk) items, chunk = 4 vectors (vec) or 1.
@@ -697,7 +713,7 @@ struct Tile {
     int task;   // TileTask
     int level;  // TT_EVAL: level; TT_EMIT: level whose emissions (0 = terminals)
     int kind;   // TT_EVAL: EvalKind
-    long begin, end;  // item range: init items, rules (absolute), emissions (within level), rows
+    long begin, end;  // item range: init items, rules (absolute), emissions (within level), gather segments
 };
 
 template <class S>

@@ -34,16 +34,25 @@ def log(*a):
     print(f"[proposer {time.time() - T0:6.1f}s]", *a, file=sys.stderr, flush=True)
 
 
-def training_set(quick):
+def training_set(quick, graphs=False):
+    """Matrices of the haplotype (config 1) and genotype (config 2) families,
+    the genotype ones at row counts from 64 to 500,000 so the set spans the
+    shapes the decisions depend on (row-parallel work, level widths); with
+    `graphs`, also R-MAT graphs (the config 4 family) at several scales."""
     import gmr
+    if graphs:
+        for scale, edges, seed in ((12, 60000, 31), (15, 450000, 32), (18, 3500000, 33), (21, 28000000, 34)):
+            yield f"rmat{scale}", gmr.Matrix.rmat(seed, scale, edges)
     if quick:
-        yield "hap1024", gmr.Matrix.haplotype(20261204, 1024, 512, founders=16, seg_mean=32.0)
-        yield "geno20k", gmr.Matrix.genotype(20261205, 20000, 2000, founders=64, classes=8, seg_mean=128.0)
-        yield "hap2048", gmr.Matrix.haplotype(7, 2048, 1024, founders=32, seg_mean=64.0)
+        yield "hap1024x512", gmr.Matrix.haplotype(20261204, 1024, 512, founders=16, seg_mean=32.0)
+        yield "hap2048x1024", gmr.Matrix.haplotype(7, 2048, 1024, founders=32, seg_mean=64.0)
+        for rows, seed in ((64, 11), (300, 12), (3000, 13)):
+            yield f"geno{rows}x10000", gmr.Matrix.genotype(seed, rows, 10000)
         return
     yield "hap4096x2048", gmr.Matrix.haplotype(20261204, 4096, 2048)
     yield "hap4096x2048_s2", gmr.Matrix.haplotype(20261214, 4096, 2048)
-    for rows, seed in ((500000, 20261205), (150000, 20261215), (40000, 20261225)):
+    for rows, seed in ((500000, 20261205), (150000, 20261215), (40000, 20261225), (10000, 20261235),
+                       (2000, 20261245), (300, 20261255), (64, 20261265)):
         yield f"geno{rows}x10000", gmr.Matrix.genotype(seed, rows, 10000)
 
 
@@ -73,6 +82,7 @@ def main(argv=None):
     ap.add_argument("--nnz-lo", type=float, default=1e5)
     ap.add_argument("--nnz-hi", type=float, default=1e8)
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--graphs", action="store_true", help="also train on R-MAT graphs (config 4 family)")
     args = ap.parse_args(argv)
     import torch
 
@@ -80,10 +90,10 @@ def main(argv=None):
     from gmr import proposer as P
 
     os.makedirs(args.out, exist_ok=True)
-    train_B = [1, 3, 4, 16, 29, 64, 100, 256] if not args.quick else [1, 4, 16, 64]
+    train_B = [1, 3, 4, 16, 29, 64, 100, 256] if not args.quick else [1, 4, 16, 37, 64]
     Xs, Ts, Ms = [], [], []
     t_collect = time.time()
-    for name, m in training_set(args.quick):
+    for name, m in training_set(args.quick, args.graphs):
         g = m.repair()
         s = g.schedule(m.rows, m.cols, "resident")
         arr = s.arrays()
@@ -160,7 +170,7 @@ def main(argv=None):
         fused[vname] = e0.elapsed_time(e1) / 3
     nnz_vec = sum(r["nnz"] * r["B"] for r in rows_out)
     report = dict(
-        training=dict(samples=int(len(X)), epochs=args.epochs, loss_first=float(losses[0]),
+        training=dict(samples=int(len(X)), epochs=args.epochs, graphs=bool(args.graphs), loss_first=float(losses[0]),
                       loss_last=float(losses[-1]), collect_s=t_collect, train_s=t_train, B=train_B),
         heldout=dict(n=len(rows_out), regret_mean=float(reg.mean()), regret_median=float(np.median(reg)),
                      regret_max=float(reg.max()), regret_of_totals=tot_mine / tot_ex - 1.0,
