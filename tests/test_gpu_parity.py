@@ -356,3 +356,33 @@ def test_batch_integer_semirings():
             batch.check()
         for Y, Ys_ in zip(Ys, stars):
             check(2, Y.cpu().numpy(), Ys_, None, f"batch tropical {var}")
+
+
+def test_gpu_test_driver_crosscheck_files(tmp_path):
+    """The reference-compatible driver: XVEC in, CROSSCHECK out (float32 x,
+    float64 y), for every semiring, and its batched self-check."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "g-mm-repair_amd", "bin",
+                       "gpu_test")
+    m = gmr.Matrix.haplotype(77, 3000, 1024, founders=24, seg_mean=48.0)
+    g = m.repair()
+    seq, V = m.arrays()
+    base = str(tmp_path / "g")
+    g.write(base)
+    for sr, name in ((0, "plustimes"), (1, "boolean"), (2, "tropical")):
+        x = x_for(sr, 1024, 500 + sr)
+        xf = tmp_path / f"x{sr}.bin"
+        x.tofile(xf)
+        prefix = str(tmp_path / "out")
+        env = dict(os.environ, SEMIRING=name, XVEC=str(xf), CROSSCHECK=prefix)
+        _say("gpu_test", name)
+        r = subprocess.run([exe, base, "3000", "1024", "2", "repair", "4"], capture_output=True, text=True, env=env,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "GPU kernel:" in r.stdout and "Batched verification" in r.stdout and "SUCCESS" in r.stdout
+        xd = np.fromfile(prefix + f".{name}.x", np.float32)
+        yd = np.fromfile(prefix + f".{name}.y", np.float64)
+        np.testing.assert_array_equal(xd, x)
+        y_star, den = O.brute_force(sr, 3000, 1024, seq, V, x)
+        check(sr, yd, y_star, den, f"gpu_test {name}")
