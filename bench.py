@@ -65,9 +65,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--B", type=int, default=64)
-    ap.add_argument("--rows", type=int, default=500000)
-    ap.add_argument("--cols", type=int, default=10000)
-    ap.add_argument("--seed", type=int, default=20261205)
+    ap.add_argument("--workload", default="genotype", choices=["genotype", "haplotype", "rmat"],
+                    help="genotype: 500,000 x 10,000 (the headline); haplotype: 4,096 x 2,048; "
+                         "rmat: 2^23-vertex graph, Boolean and min/plus (own JSON line)")
+    ap.add_argument("--sweep-B", default=None,
+                    help="comma-separated vector counts: throughput per B on this workload (own JSON line)")
+    ap.add_argument("--rows", type=int, default=None)
+    ap.add_argument("--cols", type=int, default=None)
+    ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="use the default schedule instead of autotuning")
@@ -97,14 +102,23 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "rmat":
+        return run_rmat(args)
+    dflt = {"genotype": (500000, 10000, 20261205), "haplotype": (4096, 2048, 20261204)}[args.workload]
+    args.rows = args.rows or dflt[0]
+    args.cols = args.cols or dflt[1]
+    args.seed = args.seed or dflt[2]
     B = args.B
-    log("generating", args.rows, "x", args.cols)
-    m = gmr.Matrix.genotype(args.seed, args.rows, args.cols)
+    log("generating", args.workload, args.rows, "x", args.cols)
+    if args.workload == "genotype":
+        m = gmr.Matrix.genotype(args.seed, args.rows, args.cols)
+    else:
+        m = gmr.Matrix.haplotype(args.seed, args.rows, args.cols)
     nnz = m.nnz()
     base = None
     if args.grammar_cache:
         os.makedirs(args.grammar_cache, exist_ok=True)
-        base = os.path.join(args.grammar_cache, f"genotype_{args.rows}x{args.cols}_{args.seed}")
+        base = os.path.join(args.grammar_cache, f"{args.workload}_{args.rows}x{args.cols}_{args.seed}")
     if base and os.path.exists(base + ".vc.C"):
         g = gmr.Grammar.read(base)
         log("grammar loaded from", base)
@@ -124,6 +138,9 @@ def main():
     eng = s_res.engine()
     log("engine uploaded", eng.info())
 
+    if args.sweep_B:
+        return run_sweep_B(args, m, g, s_res, eng)
+
     gen = torch.Generator(device="cuda").manual_seed(args.seed)
     X = torch.randn(args.cols, B, device="cuda", generator=gen, dtype=torch.float32)
     Y = torch.empty(args.rows, B, device="cuda", dtype=torch.float32)
@@ -139,6 +156,7 @@ def main():
         cands = autotune.candidates(B, gmr.LAYOUTS[args.layout])
         best, tuned = autotune.tune(eng, 0, X, Y, cands, reps=2, log=log)
     log("schedule:", autotune.describe(best))
+    run_kw = autotune.apply(eng, best)
 
     if args.calibrate:
         # 1 GiB read + 1 GiB written by one copy kernel: the HBM counters of this
@@ -153,7 +171,7 @@ def main():
 
     # ---- timed region: W warm-up steps, then K steps, one event pair per step
     for _ in range(args.warmup):
-        eng.run_torch(0, X, Y, **best)
+        eng.run_torch(0, X, Y, **run_kw)
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     start = torch.cuda.Event(enable_timing=True)
@@ -162,7 +180,7 @@ def main():
     start.record()
     for i in range(args.steps):
         ev[i][0].record()
-        eng.run_torch(0, X, Y, **best)
+        eng.run_torch(0, X, Y, **run_kw)
         ev[i][1].record()
     stop.record()
     torch.cuda.synchronize()
@@ -218,7 +236,7 @@ def main():
     # persistent variant; otherwise the sweep's launches together)
     bytes_alg = dag_bytes(info, B)
     achieved = bytes_alg / (kern_ms * 1e-3) / 1e9
-    workload = f"genotype{args.rows}x{args.cols}_B{B}"
+    workload = f"{args.workload}{args.rows}x{args.cols}_B{B}"
     traffic = read_traffic(workload)
     launch = {gmr.VARIANT_PERSISTENT: "one k_persistent launch per sweep",
               gmr.VARIANT_GRAPH: "one hipGraph launch per sweep (k_init, one k_eval per level, k_gather or k_emit)",
@@ -262,7 +280,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded genotype mosaic, RePair-compressed by the engine's builder; X ~ N(0,1))",
+        "data": f"synthetic (seeded {args.workload} mosaic, RePair-compressed by the engine's builder; X ~ N(0,1))",
         "config": {"workload": workload, "rows": args.rows, "cols": args.cols, "nnz": nnz, "B": B,
                    "grammar": {"rules": info["num_raw_nt"], "C": g.info()["clen"], "depth": info["max_depth"],
                                "emissions": info["emit_total"], "term_emissions": info["num_term_emit"],
@@ -279,6 +297,116 @@ def main():
         "tuned": [(autotune.describe(c), round(ms, 4)) for c, ms in tuned],
     }
     print(json.dumps(out), flush=True)
+
+
+def _time_steps(eng, semiring, X, Y, cand, steps, warmup):
+    import torch
+
+    from gmr import autotune
+    cand = autotune.apply(eng, cand)
+    for _ in range(warmup):
+        eng.run_torch(semiring, X, Y, **cand)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        eng.run_torch(semiring, X, Y, **cand)
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def run_sweep_B(args, m, g, s_res, eng):
+    """Throughput per vector count B on one matrix (each B autotuned), with the
+    FP32 error of two columns against the FP64 oracle."""
+    import numpy as np
+    import torch
+
+    import gmr
+    from gmr import autotune
+    from oracle import oracle as O
+    seq, V = m.arrays()
+    nnz = m.nnz()
+    info = s_res.info()
+    out = []
+    for B in [int(b) for b in args.sweep_B.split(",")]:
+        gen = torch.Generator(device="cuda").manual_seed(args.seed + B)
+        X = torch.randn(args.cols, B, device="cuda", generator=gen)
+        Y = torch.empty(args.rows, B, device="cuda")
+        if B == 1:
+            X, Y = X[:, 0].contiguous(), Y[:, 0].contiguous()
+        best, _ = autotune.tune(eng, 0, X, Y, autotune.candidates(B, gmr.RESIDENT), reps=2)
+        ms = _time_steps(eng, 0, X, Y, best, args.steps, args.warmup)
+        if best["variant"] == gmr.VARIANT_PERSISTENT:
+            eng.check()
+        cols = [0] if B == 1 else [0, B - 1]
+        Xs = X.reshape(args.cols, -1)[:, cols].contiguous().cpu().numpy()
+        Ys, den = O.brute_force_batched(0, args.rows, args.cols, seq, V, Xs)
+        Yg = Y.reshape(args.rows, -1)[:, cols].cpu().numpy().astype(np.float64)
+        d = np.abs(Yg - Ys)
+        err = float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1), np.where(d == 0, 0, np.inf))))
+        r = {"B": B, "ms": ms, "nnz_vec_per_s": nnz * B / (ms * 1e-3), "gbs_dag": dag_bytes(info, B) / ms / 1e6,
+             "schedule": autotune.describe(best), "fp32_err": err}
+        log("B", B, r)
+        out.append(r)
+        del X, Y
+    print(json.dumps({"metric": METRIC, "workload": f"{args.workload}{args.rows}x{args.cols}", "nnz": nnz,
+                      "grammar": {"rules": info["num_raw_nt"], "depth": info["max_depth"],
+                                  "emissions": info["emit_total"], "term_emissions": info["num_term_emit"]},
+                      "per_B": out}), flush=True)
+
+
+def run_rmat(args):
+    """R-MAT graph on 2^23 vertices from 2^27 edge draws (a, b, c, d) = (.57, .19,
+    .19, .05), duplicates removed: (a) Boolean one-step reachability from a 1%
+    frontier; (b) min/plus relaxation with weights U{1..100}, distances 0 on
+    1,024 sources and INT32_MAX/2 elsewhere. Both checked bit-exact against the
+    oracle; throughput of the tuned single-vector sweep and of the plain sweep."""
+    import numpy as np
+    import torch
+
+    import gmr
+    from gmr import autotune
+    from oracle import oracle as O
+    scale, edges = 23, 1 << 27
+    res = {}
+    for tag, nw, sr in (("boolean", 1, gmr.BOOLEAN), ("minplus", 100, gmr.TROPICAL)):
+        m = gmr.Matrix.rmat(20261206 + nw, scale, edges, nweights=nw)
+        nnz, n = m.nnz(), m.rows
+        g = m.repair()
+        log(tag, "nnz", nnz, f"RePair {g.build_seconds:.1f}s", g.info())
+        s_res = g.schedule(n, n, "resident")
+        s_ref = g.schedule(n, n, "refcompat")
+        eng = s_res.engine()
+        rng = np.random.default_rng(2026 + nw)
+        if sr == gmr.BOOLEAN:
+            x = np.zeros(n, np.float32)
+            x[rng.choice(n, n // 100, replace=False)] = 1.0
+        else:
+            x = np.full(n, np.float32(2 ** 31 - 1) // 2, np.float32)
+            x[rng.choice(n, 1024, replace=False)] = 0.0
+        xt = torch.from_numpy(x).cuda()
+        y = torch.empty(n, dtype=torch.int32, device="cuda")
+        best, _ = autotune.tune(eng, sr, xt, y, autotune.candidates(1, gmr.RESIDENT), reps=2)
+        ms = _time_steps(eng, sr, xt, y, best, args.steps, args.warmup)
+        seq, V = m.arrays()
+        y_star, _ = O.brute_force(sr, n, n, seq, V, x)
+        exact = bool(np.array_equal(y.cpu().numpy().astype(np.int64), y_star.astype(np.int64)))
+        eng_ref = s_ref.engine()
+        plain_ms = _time_steps(eng_ref, sr, xt, y, dict(variant=gmr.VARIANT_LEVELS, emit=gmr.EMIT_SPOT),
+                               args.steps, args.warmup)
+        exact_plain = bool(np.array_equal(y.cpu().numpy().astype(np.int64), y_star.astype(np.int64)))
+        info = s_res.info()
+        res[tag] = {"nnz": nnz, "ms": ms, "nnz_per_s": nnz / (ms * 1e-3), "schedule": autotune.describe(best),
+                    "bit_exact": exact, "plain_ms": plain_ms, "plain_nnz_per_s": nnz / (plain_ms * 1e-3),
+                    "plain_bit_exact": exact_plain,
+                    "grammar": {"rules": info["num_raw_nt"], "depth": info["max_depth"], "C": g.info()["clen"],
+                                "U": info["U"], "refcompat_passthrough": s_ref.info()["num_passthrough"]},
+                    "repair_s": g.build_seconds}
+        log(tag, res[tag])
+        del eng, eng_ref, m, g, s_res, s_ref, seq
+        torch.cuda.empty_cache()
+    print(json.dumps({"metric": METRIC, "workload": f"rmat{scale}_{edges}", "results": res}), flush=True)
 
 
 if __name__ == "__main__":

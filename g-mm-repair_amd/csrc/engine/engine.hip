@@ -215,6 +215,7 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
         d_csym = dev_upload(csym.data(), csym.size(), &struct_bytes);
         d_gseg = dev_upload(gseg.data(), gseg.size(), &struct_bytes);
         n_gseg = static_cast<int>(gseg.size());
+        crow_host.assign(1, static_cast<int>(csym.size()));
     }
     d_level_off = dev_upload(level_off.data(), level_off.size(), &struct_bytes);
     d_emit_off = dev_upload(emit_off.data(), emit_off.size(), &struct_bytes);
@@ -278,10 +279,17 @@ namespace {
 
 using dev::SweepArgs;
 
+// Row width of T and W for a tile of Bt vectors: a multiple of 4 so rows are
+// 16-byte aligned and every lane moves four values (one vector stays 1).
+inline int padded_width(int Bt) { return Bt <= 1 ? 1 : (Bt + 3) / 4 * 4; }
+
+// Blocks for `work` threads, rounded up to a multiple of 8 (one share per XCD;
+// the kernels' grid-stride loops leave surplus threads idle).
 inline unsigned grid_for(long work, int block, int cap = 1 << 20) {
     long g = (work + block - 1) / block;
     if (g < 1) g = 1;
     if (g > cap) g = cap;
+    if (g > 8) g = (g + 7) / 8 * 8;
     return static_cast<unsigned>(g);
 }
 
@@ -298,7 +306,7 @@ inline int kinds_hash(const Engine& en, const gmr_run_opts& o) {
 inline unsigned eval_grid(const SweepArgs& a, int kind, long w, int block) {
     if (kind == dev::EV_LDS && a.Bt != 1) kind = dev::EV_GROUP;
     switch (kind) {
-        case dev::EV_THREAD: return grid_for(w * (a.vec ? a.Bt / 4 : a.Bt), block);
+        case dev::EV_THREAD: return grid_for(w * (a.vec ? a.Bs / 4 : a.Bs), block);
         case dev::EV_WAVE: return grid_for(w, block / 64);
         case dev::EV_LDS: return grid_for(w, block);
         default: return grid_for(w, block / a.G);
@@ -312,8 +320,7 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
                    hipEvent_t* level_ev = nullptr) {
     const int G = a.G;
     if (level_ev) GMR_HIP(hipEventRecord(level_ev[0], st));
-    hipLaunchKernelGGL(dev::k_init<S>, dim3(grid_for(static_cast<long>(en.U) * a.Bt +
-                                                      static_cast<long>(en.rows) * a.Bt, block, 4096)),
+    hipLaunchKernelGGL(dev::k_init<S>, dim3(grid_for(dev::init_items(a), block, 4096)),
                        dim3(block), 0, st, a);
     if (!a.gather && en.Et > 0) {
         long work = (a.Bt == 1 && a.B == 1) ? en.Et : ((static_cast<long>(en.Et) + dev::kEmitChunk - 1) / dev::kEmitChunk) * G;
@@ -425,8 +432,10 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
     a.n_term = en.Et;
     a.B = B;
     a.Bt = std::min(Bt, B - b0);
-    a.vec = (a.Bt % 4 == 0 && B % 4 == 0) ? 1 : 0;
-    a.G = dev::lanes_for(a.Bt, a.vec);
+    a.Bs = padded_width(a.Bt);
+    a.vec = a.Bt > 1 ? 1 : 0;
+    a.yvec = (B % 4 == 0 && b0 % 4 == 0) ? 1 : 0;
+    a.G = dev::lanes_for(a.Bs, a.vec);
     a.resident = en.resident ? 1 : 0;
     a.gather = gather ? 1 : 0;
     a.GR = en.gather_lanes;
@@ -441,7 +450,7 @@ void run_all(Engine& en, int semiring, const float* X, int B, void* Y, const gmr
     const int Bt = (o.b_tile > 0 && o.b_tile < B) ? o.b_tile : B;
     const bool gather = o.emit == GMR_EMIT_GATHER;
     if (gather && !en.resident) throw std::invalid_argument("gather emission needs the Resident layout");
-    en.ensure_workspace(Bt, static_cast<int>(sizeof(T)));
+    en.ensure_workspace(padded_width(Bt), static_cast<int>(sizeof(T)));
     for (int b0 = 0; b0 < B; b0 += Bt) {
         SweepArgs a = make_args(en, X, B, Y, static_cast<int>(sizeof(T)), b0, Bt, gather, o);
         run_tile<S>(en, semiring, a, o, st);
@@ -477,6 +486,41 @@ void Engine::run(int semiring, const float* X, int B, void* Y, const gmr_run_opt
     }
 }
 
+void Engine::set_gather_order(int order) {
+    if (!resident || n_gseg == 0) return;
+    std::vector<int4> seg(n_gseg);
+    std::vector<int> sym(crow_host.back());
+    GMR_HIP(hipDeviceSynchronize());
+    GMR_HIP(hipMemcpy(seg.data(), d_gseg, sizeof(int4) * n_gseg, hipMemcpyDeviceToHost));
+    if (!sym.empty()) GMR_HIP(hipMemcpy(sym.data(), d_csym, sizeof(int) * sym.size(), hipMemcpyDeviceToHost));
+    if (order == 0) {
+        std::stable_sort(seg.begin(), seg.end(), [](const int4& a, const int4& b) {
+            return a.x != b.x ? a.x < b.x : a.y < b.y;
+        });
+    } else {
+        // Segments that share their highest rule (the longest substring they
+        // copy) run next to each other, so their value rows are read while
+        // still in the caches.
+        std::vector<std::pair<long long, int>> key(n_gseg);
+        for (int q = 0; q < n_gseg; ++q) {
+            int hi = -1;
+            for (int j = seg[q].y; j < seg[q].z; ++j) hi = std::max(hi, sym[j]);
+            key[q] = {static_cast<long long>(hi) * (1LL << 31) + seg[q].x, q};
+        }
+        std::sort(key.begin(), key.end());
+        std::vector<int4> out(n_gseg);
+        for (int q = 0; q < n_gseg; ++q) out[q] = seg[key[q].second];
+        seg.swap(out);
+    }
+    GMR_HIP(hipMemcpy(d_gseg, seg.data(), sizeof(int4) * n_gseg, hipMemcpyHostToDevice));
+    for (auto& kv : graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    graphs.clear();
+    gather_order = order;
+}
+
 const int* Engine::device_kinds(const int* host_kinds) {
     std::vector<int> v(host_kinds, host_kinds + L + 1);
     if (v != kinds_host || !d_kinds) {
@@ -494,7 +538,7 @@ template <class S>
 void time_levels_t(Engine& en, const float* X, int B, void* Y, const gmr_run_opts& o, int reps, float* out,
                    hipStream_t st) {
     using T = typename S::T;
-    en.ensure_workspace(B, static_cast<int>(sizeof(T)));
+    en.ensure_workspace(padded_width(B), static_cast<int>(sizeof(T)));
     SweepArgs a{};
     a.kids = en.d_kids;
     a.coeff = en.d_coeff;
@@ -521,8 +565,10 @@ void time_levels_t(Engine& en, const float* X, int B, void* Y, const gmr_run_opt
     a.n_term = en.Et;
     a.B = B;
     a.Bt = B;
-    a.vec = (B % 4 == 0) ? 1 : 0;
-    a.G = dev::lanes_for(B, a.vec);
+    a.Bs = padded_width(B);
+    a.vec = B > 1 ? 1 : 0;
+    a.yvec = (B % 4 == 0) ? 1 : 0;
+    a.G = dev::lanes_for(a.Bs, a.vec);
     a.resident = en.resident ? 1 : 0;
     a.gather = (o.emit == GMR_EMIT_GATHER) ? 1 : 0;
     a.GR = en.gather_lanes;
@@ -622,7 +668,7 @@ inline std::size_t mix(std::size_t h, std::size_t v) { return (h ^ v) * 10995116
 inline long eval_tile_rules(const dev::SweepArgs& a, int kind, int block) {
     if (kind == dev::EV_LDS && a.Bt != 1) kind = dev::EV_GROUP;
     switch (kind) {
-        case dev::EV_THREAD: return std::max(1L, static_cast<long>(block) * 4 / (a.vec ? a.Bt / 4 : a.Bt));
+        case dev::EV_THREAD: return std::max(1L, static_cast<long>(block) * 4 / (a.vec ? a.Bs / 4 : a.Bs));
         case dev::EV_WAVE: return static_cast<long>(block / 64) * 4;
         case dev::EV_LDS: return block;
         default: return static_cast<long>(block / a.G) * 4;
@@ -663,7 +709,7 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
     const int block = o[0].block > 0 ? o[0].block : 256;
     std::vector<dev::SweepArgs> args(n);
     for (int g = 0; g < n; ++g) {
-        bt.eng[g]->ensure_workspace(B[g], static_cast<int>(sizeof(T)));
+        bt.eng[g]->ensure_workspace(padded_width(B[g]), static_cast<int>(sizeof(T)));
         args[g] = make_args(*bt.eng[g], X[g], B[g], Y[g], static_cast<int>(sizeof(T)), 0, B[g],
                             o[g].emit == GMR_EMIT_GATHER, o[g]);
     }
@@ -874,6 +920,12 @@ int gmr_engine_time_levels(gmr_engine* h, int semiring, const void* d_X, int B, 
         h->e.time_levels(semiring, static_cast<const float*>(d_X), B, d_Y, o, reps, level_ms,
                          static_cast<hipStream_t>(stream));
     });
+}
+
+int gmr_engine_set_gather_order(gmr_engine* h, int order) {
+    if (!h) return dev_fail(GMR_ERR_INVALID, "engine is NULL");
+    if (order < 0 || order > 1) return dev_fail(GMR_ERR_INVALID, "unknown gather order");
+    return dev_guarded([&] { h->e.set_gather_order(order); });
 }
 
 int gmr_engine_check(gmr_engine* h, int* barrier_error) {

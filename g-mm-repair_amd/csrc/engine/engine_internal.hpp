@@ -37,6 +37,8 @@ struct Engine {
     int n_gseg = 0;
     int gather_split = 0;            // some row is split over several segments
     static constexpr int kGatherSeg = 128;
+    int gather_order = 0;            // 0: row order, 1: by highest rule
+    std::vector<int> crow_host;      // [0] = number of top-sequence symbols
     int* d_level_off = nullptr;
     int* d_emit_off = nullptr;
 
@@ -78,6 +80,7 @@ struct Engine {
     // the barrier state was reset), else 0. Synchronises the device.
     int check_barrier();
     const int* device_kinds(const int* host_kinds);
+    void set_gather_order(int order);
     void time_levels(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, int reps, float* out,
                      hipStream_t st);
 };

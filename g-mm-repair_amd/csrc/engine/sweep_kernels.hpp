@@ -75,18 +75,20 @@ struct SweepArgs {
     // values (device)
     const float* X;          // cols x B (this tile's column base folded in)
     void* Y;                 // rows x B
-    void* Tv;                // U x Bt terminal values
+    void* Tv;                // U x Bs terminal values
     void* W0;                // frontier buffer 0 (or the resident value array)
     void* W1;                // frontier buffer 1 (double-buffered layouts)
     // sizes
     int rows, cols, U, L;
     int n_term;              // terminal emissions
     int B;                   // row stride of X and Y
-    int Bt;                  // vectors in this tile (stride of T and W)
+    int Bt;                  // vectors in this tile
+    int Bs;                  // stride of T and W rows: Bt rounded up to a multiple of 4 (1 for one vector)
     int G;                   // lanes per item
     int resident;            // children index one resident array
     int gather;              // emit by a final row gather
-    int vec;                 // 1: four vectors per lane (Bt % 4 == 0 and B % 4 == 0)
+    int vec;                 // 1: four vectors per lane (Bt > 1; T and W rows padded to Bs)
+    int yvec;                // Y rows are 16-byte aligned (B and the tile's first column multiples of 4)
     int GR;                  // lanes per row in the single-vector gather
     int eval_kind;           // EvalKind for levels without an entry in level_kind
     const int* level_kind;   // device, max_depth + 1 entries (or nullptr)
@@ -110,9 +112,9 @@ __device__ __forceinline__ typename S::T* tp(void* p) {
     return static_cast<typename S::T*>(p);
 }
 
-// Four vectors per lane: float4 / int4 loads and stores (dwordx4). Used when
-// the tile width Bt and the row stride B are multiples of 4, so every row of
-// T, W, X and Y starts 16-byte aligned.
+// Four vectors per lane: float4 / int4 loads and stores (dwordx4). T and W
+// rows are padded to Bs (a multiple of 4), so they are always 16-byte aligned;
+// the padding lanes carry the semiring zero and are never written to Y.
 template <class T> struct Vec4;
 template <> struct Vec4<float> { using type = float4; };
 template <> struct Vec4<int> { using type = int4; };
@@ -144,35 +146,48 @@ struct Op4 {
     }
     static __device__ __forceinline__ T4 ld(const T* p) { return *reinterpret_cast<const T4*>(p); }
     static __device__ __forceinline__ void st(T* p, T4 v) { *reinterpret_cast<T4*>(p) = v; }
-    static __device__ __forceinline__ void emit(T* y, T4 v) {
-        if (!S::skip(v.x)) S::emit(y + 0, v.x);
-        if (!S::skip(v.y)) S::emit(y + 1, v.y);
-        if (!S::skip(v.z)) S::emit(y + 2, v.z);
-        if (!S::skip(v.w)) S::emit(y + 3, v.w);
+    // The first n (<= 4) components into y, atomically / by plain stores.
+    static __device__ __forceinline__ void emit_n(T* y, T4 v, int n) {
+        if (n > 0 && !S::skip(v.x)) S::emit(y + 0, v.x);
+        if (n > 1 && !S::skip(v.y)) S::emit(y + 1, v.y);
+        if (n > 2 && !S::skip(v.z)) S::emit(y + 2, v.z);
+        if (n > 3 && !S::skip(v.w)) S::emit(y + 3, v.w);
+    }
+    static __device__ __forceinline__ void st_n(T* y, T4 v, int n, int aligned) {
+        if (n >= 4 && aligned) {
+            st(y, v);
+            return;
+        }
+        if (n > 0) y[0] = v.x;
+        if (n > 1) y[1] = v.y;
+        if (n > 2) y[2] = v.z;
+        if (n > 3) y[3] = v.w;
     }
 };
 
-// Lanes per item for a tile of Bt vectors: one lane per vector (vec = 0) or per
-// four vectors (vec = 1), capped at the wave width.
-__host__ __device__ __forceinline__ int lanes_for(int Bt, int vec) {
-    int need = vec ? Bt / 4 : Bt;
+// Lanes per item for rows of Bs values: one lane per value (vec = 0) or per
+// four values (vec = 1), capped at the wave width.
+__host__ __device__ __forceinline__ int lanes_for(int Bs, int vec) {
+    int need = vec ? Bs / 4 : Bs;
     int g = 1;
     while (g < need && g < 64) g <<= 1;
     return g;
 }
 
-// T[p*Bt + b] = leaf(V, X[col*B + b]) (T[0..Bt) = zero); Y = zero for spot emits.
+// T[p*Bs + b] = leaf(V, X[col*B + b]) for b < Bt, zero for p = 0 and for the
+// padding lanes; Y = zero for spot emits (and gathers with split rows).
 template <class S>
 __device__ void ph_init_range(const SweepArgs& a, long i0, long i1, long tid, long nth) {
     using T = typename S::T;
     T* Tt = tp<S>(a.Tv);
     T* Y = tp<S>(a.Y);
-    const long nT = static_cast<long>(a.U) * a.Bt;
+    const long nT = static_cast<long>(a.U) * a.Bs;
     for (long i = i0 + tid; i < i1; i += nth) {
         if (i < nT) {
-            int p = static_cast<int>(i / a.Bt);
-            int b = static_cast<int>(i - static_cast<long>(p) * a.Bt);
-            Tt[i] = p == 0 ? S::zero() : S::leaf(a.tval[p], a.X[static_cast<long>(a.tcol[p]) * a.B + b]);
+            int p = static_cast<int>(i / a.Bs);
+            int b = static_cast<int>(i - static_cast<long>(p) * a.Bs);
+            Tt[i] = (p == 0 || b >= a.Bt) ? S::zero()
+                                          : S::leaf(a.tval[p], a.X[static_cast<long>(a.tcol[p]) * a.B + b]);
         } else {
             long j = i - nT;
             long r = j / a.Bt;
@@ -181,10 +196,10 @@ __device__ void ph_init_range(const SweepArgs& a, long i0, long i1, long tid, lo
     }
 }
 
-// Items of the init phase: U*Bt terminal values, then (spot emission, or a
+// Items of the init phase: U*Bs terminal values, then (spot emission, or a
 // gather with split rows) rows*Bt y values.
 __host__ __device__ __forceinline__ long init_items(const SweepArgs& a) {
-    return static_cast<long>(a.U) * a.Bt + ((a.gather && !a.gsplit) ? 0L : static_cast<long>(a.rows) * a.Bt);
+    return static_cast<long>(a.U) * a.Bs + ((a.gather && !a.gsplit) ? 0L : static_cast<long>(a.rows) * a.Bt);
 }
 
 template <class S>
@@ -199,7 +214,7 @@ __device__ void ph_eval(const SweepArgs& a, int i0, int i1, const typename S::T*
                         typename S::T* dst, long gid, long ng, int lane) {
     using T = typename S::T;
     const T* Tt = tp<S>(a.Tv);
-    const int Bt = a.Bt, G = a.G;
+    const int Bt = a.Bs, G = a.G;
     for (long i = i0 + gid; i < i1; i += ng) {
         int2 c = a.kids[i];
         float cf = a.coeff ? a.coeff[i] : 1.0f;
@@ -218,7 +233,7 @@ __device__ void ph_eval_v4(const SweepArgs& a, int i0, int i1, const typename S:
     using T = typename S::T;
     using O = Op4<S>;
     const T* Tt = tp<S>(a.Tv);
-    const int Bt = a.Bt, G = a.G;
+    const int Bt = a.Bs, G = a.G;
     auto row = [&](int c) -> const T* {
         return c >= 0 ? src + static_cast<long>(c) * Bt : Tt + static_cast<long>(-c - 1) * Bt;
     };
@@ -276,7 +291,7 @@ __device__ void ph_emit_wave(const int2* em, int n, const typename S::T* src, ty
 // running sum per vector while the row repeats and issues one atomic per run.
 template <class S, int CH>
 __device__ void ph_emit_chunk(const int2* em, int n, const typename S::T* src, typename S::T* Y,
-                              int B, int Bt, int G, long gid, long ng, int lane) {
+                              int B, int Bt, int Bs, int G, long gid, long ng, int lane) {
     using T = typename S::T;
     const long nch = (static_cast<long>(n) + CH - 1) / CH;
     for (long ch = gid; ch < nch; ch += ng) {
@@ -287,7 +302,7 @@ __device__ void ph_emit_chunk(const int2* em, int n, const typename S::T* src, t
             T acc = S::zero();
             for (int e = e0; e < e1; ++e) {
                 int2 pr = em[e];
-                T v = src[static_cast<long>(pr.x) * Bt + b];
+                T v = src[static_cast<long>(pr.x) * Bs + b];
                 if (pr.y != row) {
                     if (row >= 0 && !S::skip(acc)) S::emit(Y + static_cast<long>(row) * B + b, acc);
                     row = pr.y;
@@ -303,28 +318,30 @@ __device__ void ph_emit_chunk(const int2* em, int n, const typename S::T* src, t
 
 template <class S, int CH>
 __device__ void ph_emit_chunk_v4(const int2* em, int n, const typename S::T* src, typename S::T* Y,
-                                 int B, int Bt, int G, long gid, long ng, int lane) {
+                                 int B, int Bt, int Bs, int G, long gid, long ng, int lane) {
     using O = Op4<S>;
     using T4 = typename O::T4;
     const long nch = (static_cast<long>(n) + CH - 1) / CH;
     for (long ch = gid; ch < nch; ch += ng) {
         const int e0 = static_cast<int>(ch * CH);
         const int e1 = min(n, e0 + CH);
-        for (int b = 4 * lane; b < Bt; b += 4 * G) {
+        for (int b = 4 * lane; b < Bs; b += 4 * G) {
+            const int nb = min(4, Bt - b);
+            if (nb <= 0) break;
             int row = -1;
             T4 acc = O::zero();
             for (int e = e0; e < e1; ++e) {
                 int2 pr = em[e];
-                T4 v = O::ld(src + static_cast<long>(pr.x) * Bt + b);
+                T4 v = O::ld(src + static_cast<long>(pr.x) * Bs + b);
                 if (pr.y != row) {
-                    if (row >= 0) O::emit(Y + static_cast<long>(row) * B + b, acc);
+                    if (row >= 0) O::emit_n(Y + static_cast<long>(row) * B + b, acc, nb);
                     row = pr.y;
                     acc = v;
                 } else {
                     acc = O::oplus(acc, v);
                 }
             }
-            if (row >= 0) O::emit(Y + static_cast<long>(row) * B + b, acc);
+            if (row >= 0) O::emit_n(Y + static_cast<long>(row) * B + b, acc, nb);
         }
     }
 }
@@ -337,9 +354,9 @@ __device__ __forceinline__ void ph_emit(const SweepArgs& a, const int2* em, int 
     if (a.Bt == 1 && a.B == 1) {
         ph_emit_wave<S>(em, n, src, tp<S>(a.Y), tid / 64, nth / 64, static_cast<int>(tid & 63));
     } else if (a.vec) {
-        ph_emit_chunk_v4<S, kEmitChunk>(em, n, src, tp<S>(a.Y), a.B, a.Bt, a.G, tid / a.G, nth / a.G, lane);
+        ph_emit_chunk_v4<S, kEmitChunk>(em, n, src, tp<S>(a.Y), a.B, a.Bt, a.Bs, a.G, tid / a.G, nth / a.G, lane);
     } else {
-        ph_emit_chunk<S, kEmitChunk>(em, n, src, tp<S>(a.Y), a.B, a.Bt, a.G, tid / a.G, nth / a.G, lane);
+        ph_emit_chunk<S, kEmitChunk>(em, n, src, tp<S>(a.Y), a.B, a.Bt, a.Bs, a.G, tid / a.G, nth / a.G, lane);
     }
 }
 
@@ -360,14 +377,14 @@ __device__ void ph_gather_rows(const SweepArgs& a, long q0, long q1, long gid, l
     const T* W = tp<S>(a.W0);
     const T* Tt = tp<S>(a.Tv);
     T* Y = tp<S>(a.Y);
-    const int Bt = a.Bt;
+    const int Bt = a.Bt, Bs = a.Bs;
     for (long q = q0 + gid; q < q1; q += ng) {
         const int4 sg = a.gseg[q];
         for (int b = lane; b < Bt; b += a.G) {
             T acc = S::zero();
             for (int j = sg.y; j < sg.z; ++j) {
                 int c = a.csym[j];
-                T v = c >= 0 ? W[static_cast<long>(c) * Bt + b] : Tt[static_cast<long>(-c - 1) * Bt + b];
+                T v = c >= 0 ? W[static_cast<long>(c) * Bs + b] : Tt[static_cast<long>(-c - 1) * Bs + b];
                 acc = S::oplus(acc, v);
             }
             gather_out<S>(Y + static_cast<long>(sg.x) * a.B + b, acc, sg.w);
@@ -385,13 +402,15 @@ __device__ void ph_gather_rows_v4(const SweepArgs& a, long q0, long q1, long gid
     const T* W = tp<S>(a.W0);
     const T* Tt = tp<S>(a.Tv);
     T* Y = tp<S>(a.Y);
-    const int Bt = a.Bt;
+    const int Bt = a.Bt, Bs = a.Bs;
     auto row = [&](int c) -> const T* {
-        return c >= 0 ? W + static_cast<long>(c) * Bt : Tt + static_cast<long>(-c - 1) * Bt;
+        return c >= 0 ? W + static_cast<long>(c) * Bs : Tt + static_cast<long>(-c - 1) * Bs;
     };
     for (long q = q0 + gid; q < q1; q += ng) {
         const int4 sg = a.gseg[q];
-        for (int b = 4 * lane; b < Bt; b += 4 * a.G) {
+        for (int b = 4 * lane; b < Bs; b += 4 * a.G) {
+            const int nb = min(4, Bt - b);
+            if (nb <= 0) break;
             T4 acc0 = O::zero(), acc1 = O::zero();
             int j = sg.y;
             for (; j + 3 < sg.z; j += 4) {
@@ -403,8 +422,8 @@ __device__ void ph_gather_rows_v4(const SweepArgs& a, long q0, long q1, long gid
             for (; j < sg.z; ++j) acc0 = O::oplus(acc0, O::ld(row(a.csym[j]) + b));
             T* y = Y + static_cast<long>(sg.x) * a.B + b;
             const T4 v = O::oplus(acc0, acc1);
-            if (!sg.w) O::st(y, v);
-            else O::emit(y, v);
+            if (!sg.w) O::st_n(y, v, nb, a.yvec);
+            else O::emit_n(y, v, nb);
         }
     }
 }
@@ -460,7 +479,7 @@ __device__ void ph_eval_thread(const SweepArgs& a, int i0, int i1, const typenam
     using T = typename S::T;
     using O = Op4<S>;
     const T* Tt = tp<S>(a.Tv);
-    const int Bt = a.Bt;
+    const int Bt = a.Bs;
     const int c = a.vec ? Bt / 4 : Bt;
     const long n = static_cast<long>(i1 - i0) * c;
     for (long j = tid; j < n; j += nth) {
@@ -487,7 +506,7 @@ __device__ void ph_eval_wave(const SweepArgs& a, int i0, int i1, const typename 
                              typename S::T* dst, long wid, long nw, int lane) {
     using T = typename S::T;
     const T* Tt = tp<S>(a.Tv);
-    const int Bt = a.Bt;
+    const int Bt = a.Bs;
     for (long i = i0 + wid; i < i1; i += nw) {
         const int2 k = a.kids[i];
         const float cf = a.coeff ? a.coeff[i] : 1.0f;
@@ -575,7 +594,7 @@ __device__ __forceinline__ int level_kind(const SweepArgs& a, int k) {
 // Frontier that level k writes (k >= 1) and the source it reads.
 template <class S>
 __device__ __forceinline__ typename S::T* level_dst(const SweepArgs& a, int k) {
-    if (a.resident) return tp<S>(a.W0) + static_cast<long>(a.level_off[k - 1]) * a.Bt;
+    if (a.resident) return tp<S>(a.W0) + static_cast<long>(a.level_off[k - 1]) * a.Bs;
     return (k & 1) ? tp<S>(a.W1) : tp<S>(a.W0);
 }
 template <class S>
@@ -584,17 +603,28 @@ __device__ __forceinline__ const typename S::T* level_src(const SweepArgs& a, in
     return (k & 1) ? tp<S>(a.W0) : tp<S>(a.W1);  // frontier k-1
 }
 
+// Global thread index with an XCD-aware block order. Workgroups are dealt to
+// the 8 XCDs round-robin (workgroup b runs on XCD b mod 8) and each XCD has
+// its own L2; renumbering so XCD x owns the contiguous block range
+// [x*n/8, (x+1)*n/8) keeps neighbouring items (rules sorted by left child,
+// gather segments sorted by shared rule) on one L2. Grids are multiples of 8.
+__device__ __forceinline__ long xcd_tid() {
+    unsigned nb = gridDim.x, b = blockIdx.x;
+    if ((nb & 7u) == 0u) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    return static_cast<long>(b) * blockDim.x + threadIdx.x;
+}
+
 // ---- one launch per phase (levels / graph variants) ----
 
 template <class S>
 __global__ void k_init(SweepArgs a) {
-    long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    long tid = xcd_tid();
     ph_init<S>(a, tid, static_cast<long>(gridDim.x) * blockDim.x);
 }
 
 template <class S>
 __global__ void k_eval(SweepArgs a, int k, int kind) {
-    long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    long tid = xcd_tid();
     long nth = static_cast<long>(gridDim.x) * blockDim.x;
     ph_eval_kind<S>(a, kind, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k), tid,
                     nth);
@@ -603,7 +633,7 @@ __global__ void k_eval(SweepArgs a, int k, int kind) {
 // k = 0: terminal emissions; k >= 1: level k's emissions.
 template <class S>
 __global__ void k_emit(SweepArgs a, int k) {
-    long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    long tid = xcd_tid();
     long nth = static_cast<long>(gridDim.x) * blockDim.x;
     int lane = static_cast<int>(tid % a.G);
     if (k == 0) {
@@ -618,7 +648,7 @@ __global__ void k_emit(SweepArgs a, int k) {
 
 template <class S>
 __global__ void k_gather(SweepArgs a) {
-    long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    long tid = xcd_tid();
     long nth = static_cast<long>(gridDim.x) * blockDim.x;
     ph_gather<S>(a, tid, nth);
 }
@@ -666,7 +696,7 @@ __device__ __forceinline__ void grid_barrier(GridBarrier* gb, unsigned long long
 template <class S>
 __global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long base,
                              unsigned long long deadline) {
-    const long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long tid = xcd_tid();
     const long nth = static_cast<long>(gridDim.x) * blockDim.x;
     const int lane = static_cast<int>(tid % a.G);
     const unsigned long long nb = gridDim.x;
@@ -723,7 +753,7 @@ __device__ void run_tile(const SweepArgs& a, const Tile& t) {
         case TT_INIT: ph_init_range<S>(a, t.begin, t.end, tid, nth); break;
         case TT_EVAL:
             ph_eval_kind<S>(a, t.kind, static_cast<int>(t.begin), static_cast<int>(t.end), level_src<S>(a, t.level),
-                            level_dst<S>(a, t.level) + (t.begin - a.level_off[t.level - 1]) * a.Bt, tid, nth);
+                            level_dst<S>(a, t.level) + (t.begin - a.level_off[t.level - 1]) * a.Bs, tid, nth);
             break;
         case TT_EMIT: {
             const int lane = static_cast<int>(tid % a.G);

@@ -313,6 +313,10 @@ class Engine:
         if err.value:
             raise GmrError("persistent sweep: a grid-barrier wait timed out; results are invalid")
 
+    def set_gather_order(self, order):
+        """0: row order; 1: segments sharing their highest rule together."""
+        _check(_lib.dev().gmr_engine_set_gather_order(self._h, order))
+
     def debug(self):
         a = (C.c_longlong * 8)()
         _check(_lib.dev().gmr_engine_debug(self._h, a))

@@ -19,8 +19,9 @@ EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
 
 def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), (1024, 0), (256, 256))):
     """The search space for one engine and batch width: variant x emission x
-    vector tile x launch shape. Persistent shapes are (block, grid cap); a cap
-    of 0 means the engine's default of 16 waves per CU."""
+    vector tile x launch shape (x gather order for row-gather emission).
+    Persistent shapes are (block, grid cap); a cap of 0 means the engine's
+    default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
     tiles = sorted({B} | {t for t in (16, 32, 64) if t < B})
     out = []
@@ -29,14 +30,25 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
             for t in tiles:
                 shapes = persistent_shapes if v == VARIANT_PERSISTENT else [(b, 0) for b in blocks]
                 for blk, cap in shapes:
-                    out.append(dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap))
+                    for order in ((0, 1) if e == EMIT_GATHER else (0,)):
+                        out.append(dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap,
+                                        gather_order=order))
     return out
+
+
+def apply(engine, cand):
+    """Set the candidate's engine state; return the run keywords."""
+    kw = dict(cand)
+    order = kw.pop("gather_order", None)
+    if order is not None:
+        engine.set_gather_order(order)
+    return kw
 
 
 def describe(c):
     cap = c.get("grid_cap", 0)
     return (f"{VARIANT_NAMES[c['variant']]}/{EMIT_NAMES[c['emit']]}/tile={c['b_tile'] or 'B'}/blk={c['block']}"
-            + (f"/grid<={cap}" if cap else ""))
+            + (f"/grid<={cap}" if cap else "") + ("/order=rule" if c.get("gather_order") == 1 else ""))
 
 
 def shape_key(info, B, semiring):
@@ -48,6 +60,7 @@ def time_candidate(engine, semiring, X, Y, cand, reps=3, warm=1):
     """Mean device milliseconds of one sweep under `cand` (torch events on the
     current stream, which is where the engine launches)."""
     import torch
+    cand = apply(engine, cand)
     for _ in range(warm):
         engine.run_torch(semiring, X, Y, **cand)
     torch.cuda.synchronize()

@@ -84,6 +84,10 @@ int gmr_engine_info(const gmr_engine* e, long long info[16]);
  * returns after enqueueing. opts may be NULL (levels variant, spot emit). */
 int gmr_engine_run(gmr_engine* e, int semiring, const void* d_X, int B, void* d_Y,
                    const gmr_run_opts* opts, void* stream);
+/* Resident layout: order in which the row gather visits its segments.
+ * 0 = row order (default); 1 = segments sharing their highest rule together
+ * (cache locality for the value rows they read). Results are unchanged. */
+int gmr_engine_set_gather_order(gmr_engine* e, int order);
 /* Persistent variant health check: *barrier_error = 1 if a grid-barrier wait
  * of an earlier persistent launch on this engine timed out (its results are
  * invalid; the barrier has been reset), else 0. Synchronises the device. */

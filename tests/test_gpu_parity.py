@@ -97,7 +97,7 @@ def test_haplotype_single_vector(hap, sr, lay):
         check(sr, y, y_star, den, f"{lay}/{var}/{em}")
 
 
-@pytest.mark.parametrize("B", [4, 16, 64, 256, 3, 100])
+@pytest.mark.parametrize("B", [4, 16, 64, 256, 3, 7, 100, 125])
 def test_haplotype_batched_distinct_columns(hap, B):
     m, g, seq, V = hap
     X = x_for(0, 2048, 7 + B, B=B)
@@ -386,3 +386,15 @@ def test_gpu_test_driver_crosscheck_files(tmp_path):
         np.testing.assert_array_equal(xd, x)
         y_star, den = O.brute_force(sr, 3000, 1024, seq, V, x)
         check(sr, yd, y_star, den, f"gpu_test {name}")
+
+
+def test_gather_orders_agree(hap):
+    m, g, seq, V = hap
+    eng = g.schedule(4096, 2048, "resident").engine()
+    X = x_for(0, 2048, 31, B=16)
+    Y_star, den = O.brute_force_batched(0, 4096, 2048, seq, V, X)
+    for order in (1, 0, 1):
+        eng.set_gather_order(order)
+        for var in ("levels", "graph", "persistent"):
+            Y = run_engine(eng, 0, X, variant=gmr.VARIANTS[var], emit=gmr.EMIT_GATHER)
+            check(0, Y, Y_star, den, f"gather order {order} {var}")
