@@ -15,6 +15,7 @@ judge of the proposer's regret. Every choice computes the same Y.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -291,6 +292,23 @@ class Proposer:
         with np.load(path) as f:
             p.load_state({k: f[k] for k in f.files})
         return p
+
+    @classmethod
+    def default(cls):
+        """The network shipped with the package: trained on MI355X sweep
+        timings of haplotype, genotype and R-MAT grammars (gmr.train_proposer
+        --graphs) at B in {1, 3, 4, 16, 29, 64, 100, 256}."""
+        return cls.load(DEFAULT_WEIGHTS)
+
+
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(__file__), "proposer_mi355x.npz")
+
+
+def run_options(choice):
+    """(Bt, variant, opts) for Engine.run / Batch.run from a choose() result."""
+    Bt, plan, kinds, _ = choice
+    variant, emit = PLANS[plan]
+    return Bt, variant, {"emit": emit, "b_tile": Bt, "level_kind": np.array([0] + list(kinds), np.int32)}
 
 
 def regret(table, B, choice):

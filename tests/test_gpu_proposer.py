@@ -24,3 +24,42 @@ def test_proposer_trains_and_beats_the_plain_schedule(tmp_path):
     # What the network gains over the method without it: the plain schedule.
     assert ho["gain_over_plain_of_totals"] > 1.0
     assert all(r["regret"] >= -1e-9 for r in rep["per_grammar"])
+
+
+def test_shipped_proposer_picks_a_correct_fast_schedule():
+    """The network shipped with the package, on a genotype grammar it was not
+    trained on: the chosen schedule computes Y within the FP32 bound and costs
+    no more than 1.5x the exhaustive optimum over the same timing table."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import numpy as np
+
+    import gmr
+    from gmr import proposer as P
+    from helpers import rel_err
+    from oracle import oracle as O
+
+    m = gmr.Matrix.genotype(424242, 3000, 10000)
+    g = m.repair()
+    s = g.schedule(m.rows, m.cols, "resident")
+    arr = s.arrays()
+    eng = s.engine()
+    B = 48
+    prop = P.Proposer.default()
+    choice = prop.choose(arr, B)
+    Bt, variant, opts = P.run_options(choice)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    X = torch.randn(m.cols, B, device="cuda", generator=gen)
+    print(f"shipped proposer: geno3000x10000 B={B} tile={Bt} plan={P.PLAN_NAMES[choice[1]]}", flush=True)
+    Y = eng.run_torch(0, X, variant=variant, **opts)
+    torch.cuda.synchronize()
+    if variant == gmr.VARIANT_PERSISTENT:
+        eng.check()
+    seq, V = m.arrays()
+    Yr, den = O.brute_force_batched(0, m.rows, m.cols, seq, V, X.cpu().numpy())
+    err = rel_err(Y.cpu().numpy(), Yr, den)
+    table, _ = P.collect(eng, arr, P.tile_candidates(B), reps=2)
+    reg = P.regret(table, B, choice)
+    print(f"choice tile={Bt} plan={P.PLAN_NAMES[choice[1]]} err={err:.3e} regret={reg * 100:.1f}%")
+    assert err <= 1e-5
+    assert reg <= 0.5

@@ -49,3 +49,21 @@ def test_features_shape_and_tiles():
     assert F.shape == (arr["info"]["max_depth"], P.N_FEAT) and gs.shape == (P.N_FEAT,)
     assert np.all(np.isfinite(F)) and np.all(np.isfinite(gs))
     assert gs[0] == 1.0 and np.all(F[:, 0] == 0.0)
+
+
+def test_shipped_weights_match_the_network():
+    """The trained network shipped with the package has one parameter per
+    weight and bias of the 12 -> 64 -> 64 -> 10 MLP and finite standardisation."""
+    with np.load(P.DEFAULT_WEIGHTS) as f:
+        p, mu, sd = f["params"], f["mu"], f["sd"]
+        tsd = float(f["tsd"])
+    H, I, O = P.HIDDEN, P.N_FEAT, P.N_OUT
+    assert p.shape == (I * H + H + H * H + H + H * O + O,)
+    assert mu.shape == sd.shape == (I,)
+    assert np.isfinite(p).all() and (sd > 0).all() and tsd > 0
+
+
+def test_run_options_from_a_choice():
+    Bt, variant, opts = P.run_options((32, 5, [1, 0, 2], 0.1))
+    assert Bt == 32 and variant == P.PLANS[5][0] and opts["emit"] == P.PLANS[5][1]
+    assert opts["level_kind"].tolist() == [0, 1, 0, 2] and opts["b_tile"] == 32
