@@ -45,17 +45,26 @@ def dag_bytes(info, B):
     return 8 * R + 8 * (E + Et) + 8 * (U - 1) + 4 * B * (3 * R + 3 * (E + Et) + 2 * (U - 1) + rows)
 
 
-def read_traffic(workload):
-    """HBM bytes per sweep measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    from profiles/traffic.txt lines `<workload> <bytes>`; None if absent."""
+def gather_bytes(info, B, n_seg=0):
+    """Algorithmic bytes of one k_gather launch over B vectors (Resident layout,
+    row gather): the row offsets, the segment table, one symbol id per top-
+    sequence symbol, the B-wide value row (rule slot or terminal) of every symbol,
+    and Y written once."""
+    rows, syms = info["rows"], info["emit_total"] + info["num_term_emit"]
+    return 4 * (rows + 1) + 16 * n_seg + 4 * syms + 4 * B * syms + 4 * B * rows
+
+
+def read_traffic(workload, kernel):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    from profiles/traffic.txt lines `<workload> <kernel> <bytes>`; None if absent."""
     p = os.path.join(ROOT, "profiles", "traffic.txt")
     if not os.path.exists(p):
         return None
     with open(p) as f:
         for line in f:
             parts = line.split()
-            if len(parts) >= 2 and parts[0] == workload and not line.startswith("#"):
-                return float(parts[1])
+            if len(parts) >= 3 and not line.startswith("#") and parts[0] == workload and parts[1] == kernel:
+                return float(parts[2])
     return None
 
 
@@ -73,7 +82,7 @@ def main():
     ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--cols", type=int, default=None)
     ap.add_argument("--seed", type=int, default=None)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="use the default schedule instead of autotuning")
     ap.add_argument("--variant", default=None, help="force a variant (levels|graph|persistent)")
@@ -193,8 +202,17 @@ def main():
     value = nnz * B / (ms_step * 1e-3)
     log(f"batched B={B}: {ms_step:.3f} ms/step, launch avg {kern_ms:.3f} ms, {value / 1e9:.1f} G nnz*vec/s")
 
+    # ---- the dominant kernel's own launch duration: HIP events around every
+    # phase, on the stream the kernels run on, over as many sweeps as were timed
+    gather_ms = None
+    if run_kw.get("emit") == gmr.EMIT_GATHER:
+        lv = eng.time_levels(0, X, Y, reps=args.steps, emit=gmr.EMIT_GATHER, block=run_kw.get("block", 0))
+        gather_ms = float(lv[info["max_depth"] + 1])
+        log(f"phases (ms, mean of {args.steps}): init {lv[0]:.4f}, node updates {float(lv[1:-1].sum()):.4f}, "
+            f"k_gather {gather_ms:.4f}")
+
     if args.profile:
-        print(json.dumps({"profile": True, "ms_per_step": ms_step, "launch_ms": kern_ms,
+        print(json.dumps({"profile": True, "ms_per_step": ms_step, "launch_ms": kern_ms, "gather_ms": gather_ms,
                           "schedule": autotune.describe(best), "bytes_dag": dag_bytes(info, B)}), flush=True)
         return
 
@@ -232,23 +250,23 @@ def main():
     one_ms = autotune.time_candidate(eng, 0, x1, y1, best1, reps=args.steps)
     del eng_ref
 
-    # ---- roofline of the dominant kernel (one launch = one sweep for the
-    # persistent variant; otherwise the sweep's launches together)
-    bytes_alg = dag_bytes(info, B)
-    achieved = bytes_alg / (kern_ms * 1e-3) / 1e9
+    # ---- roofline of the dominant kernel: k_gather (about 70% of the sweep's
+    # device time at B = 64); the whole sweep is reported beside it
     workload = f"{args.workload}{args.rows}x{args.cols}_B{B}"
-    traffic = read_traffic(workload)
-    launch = {gmr.VARIANT_PERSISTENT: "one k_persistent launch per sweep",
-              gmr.VARIANT_GRAPH: "one hipGraph launch per sweep (k_init, one k_eval per level, k_gather or k_emit)",
-              gmr.VARIANT_LEVELS: "the sweep's per-level launches"}[best["variant"]]
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": launch, "bytes_per_launch": bytes_alg,
-                "note": "achieved = DAG-model algorithmic bytes / launch time; the model charges a y "
-                        "read-modify-write per emission that row-gather emission does not perform"}
-    if traffic:
-        roofline["achieved_measured"] = round(traffic / (kern_ms * 1e-3) / 1e9, 1)
-        roofline["frac_measured"] = round(traffic / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    sweep_traffic = read_traffic(workload, "sweep")
+    sweep = {"ms": ms_step, "bytes_dag": dag_bytes(info, B), "traffic": sweep_traffic,
+             "note": "DAG-model bytes charge every emission a y read-modify-write that row gather does not do"}
+    if sweep_traffic:
+        sweep["achieved_measured"] = round(sweep_traffic / (kern_ms * 1e-3) / 1e9, 1)
+        sweep["frac_measured"] = round(sweep_traffic / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    roofline = None
+    if gather_ms:
+        gb = gather_bytes(info, B, eng.info().get("n_gseg", 0))
+        achieved = gb / (gather_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(workload, "k_gather"),
+                    "kernel": "k_gather<PlusTimesOp>", "launch_ms": round(gather_ms, 4), "bytes_per_launch": gb,
+                    "sweep": sweep}
 
     # ---- CPU baseline: the reference's OpenMP FP32 sweep, bounded sample
     cpu = None
@@ -259,7 +277,7 @@ def main():
         xs = X[:, 0].cpu().numpy()
         O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)  # warm-up
         n, t0 = 0, time.time()
-        while n < 1 or (time.time() - t0 < args.cpu_seconds and n < 50):
+        while n < 1 or time.time() - t0 < args.cpu_seconds:
             O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)
             n += 1
         cpu_s = (time.time() - t0) / n

@@ -896,7 +896,7 @@ int gmr_engine_info(const gmr_engine* h, long long info[16]) {
     const gmr::Engine& e = h->e;
     long long v[16] = {e.rows, e.cols, e.U, e.NL, e.L, e.max_width, e.E, e.Et, e.layout,
                        e.has_runlen ? 1 : 0, static_cast<long long>(e.struct_bytes), e.nvals,
-                       0, 0, 0, 0};
+                       e.n_gseg, 0, 0, 0};
     std::memcpy(info, v, sizeof(v));
     return GMR_OK;
 }

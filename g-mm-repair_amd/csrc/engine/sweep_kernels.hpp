@@ -392,13 +392,15 @@ __device__ void ph_gather_rows(const SweepArgs& a, long q0, long q1, long gid, l
     }
 }
 
-// Four vectors per lane, symbols unrolled by 4 so each lane has four dwordx4
-// gathers in flight.
+// Four vectors per lane, symbols unrolled by 8 so each lane has eight dwordx4
+// gathers in flight; the next eight symbol ids are loaded while those gathers
+// are outstanding, so the id latency stays off the critical path.
 template <class S>
 __device__ void ph_gather_rows_v4(const SweepArgs& a, long q0, long q1, long gid, long ng, int lane) {
     using T = typename S::T;
     using O = Op4<S>;
     using T4 = typename O::T4;
+    constexpr int U = 8;
     const T* W = tp<S>(a.W0);
     const T* Tt = tp<S>(a.Tv);
     T* Y = tp<S>(a.Y);
@@ -413,13 +415,23 @@ __device__ void ph_gather_rows_v4(const SweepArgs& a, long q0, long q1, long gid
             if (nb <= 0) break;
             T4 acc0 = O::zero(), acc1 = O::zero();
             int j = sg.y;
-            for (; j + 3 < sg.z; j += 4) {
-                const int c0 = a.csym[j], c1 = a.csym[j + 1], c2 = a.csym[j + 2], c3 = a.csym[j + 3];
-                T4 v0 = O::ld(row(c0) + b), v1 = O::ld(row(c1) + b), v2 = O::ld(row(c2) + b), v3 = O::ld(row(c3) + b);
-                acc0 = O::oplus(acc0, O::oplus(v0, v1));
-                acc1 = O::oplus(acc1, O::oplus(v2, v3));
+            int c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) c[u] = j + u < sg.z ? a.csym[j + u] : 0;
+            while (j + U <= sg.z) {
+                T4 v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) v[u] = O::ld(row(c[u]) + b);
+                j += U;
+#pragma unroll
+                for (int u = 0; u < U; ++u) c[u] = j + u < sg.z ? a.csym[j + u] : 0;
+                acc0 = O::oplus(acc0, O::oplus(O::oplus(v[0], v[1]), O::oplus(v[2], v[3])));
+                acc1 = O::oplus(acc1, O::oplus(O::oplus(v[4], v[5]), O::oplus(v[6], v[7])));
             }
-            for (; j < sg.z; ++j) acc0 = O::oplus(acc0, O::ld(row(a.csym[j]) + b));
+            // Fewer than U symbols left; their ids are already in c.
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (j + u < sg.z) acc0 = O::oplus(acc0, O::ld(row(c[u]) + b));
             T* y = Y + static_cast<long>(sg.x) * a.B + b;
             const T4 v = O::oplus(acc0, acc1);
             if (!sg.w) O::st_n(y, v, nb, a.yvec);

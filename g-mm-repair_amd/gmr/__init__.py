@@ -262,7 +262,7 @@ class Engine:
         _check(_lib.dev().gmr_engine_info(self._h, a))
         keys = ["rows", "cols", "U", "num_layered", "max_depth", "max_width", "emit_total",
                 "num_term_emit", "layout", "has_runlen", "device_bytes", "nvals",
-                "clen_rows", "r0", "r1", "r2"]
+                "n_gseg", "r0", "r1", "r2"]
         return dict(zip(keys, list(a)))
 
     @staticmethod

@@ -77,8 +77,9 @@ int gmr_engine_create(int rows, int cols, int U, int num_rules, const LayeredRul
                       const GPUSchedule* sched, int layout, gmr_engine** out);
 /* Upload a schedule built by gmr_schedule_build. */
 int gmr_engine_from_schedule(const gmr_schedule* s, const float* V, int nvals, gmr_engine** out);
-/* info[0..11] = rows, cols, U, num_layered, max_depth, max_width, emit_total,
- * num_term_emit, layout, has_runlen, device bytes (structure), nvals. */
+/* info[0..12] = rows, cols, U, num_layered, max_depth, max_width, emit_total,
+ * num_term_emit, layout, has_runlen, device bytes (structure), nvals, row-gather
+ * segments (0 outside the Resident layout). */
 int gmr_engine_info(const gmr_engine* e, long long info[16]);
 /* Y = M X on `stream` (a hipStream_t; NULL = default stream). Asynchronous:
  * returns after enqueueing. opts may be NULL (levels variant, spot emit). */
