@@ -202,6 +202,25 @@ def main():
     value = nnz * B / (ms_step * 1e-3)
     log(f"batched B={B}: {ms_step:.3f} ms/step, launch avg {kern_ms:.3f} ms, {value / 1e9:.1f} G nnz*vec/s")
 
+    # ---- the same step with X arriving from and Y returning to pinned host
+    # memory (what a caller handing over host buffers sees); not the bench value
+    pcie_ms = None
+    if not args.profile:
+        Xh = torch.empty(X.shape, dtype=X.dtype, pin_memory=True).copy_(X)
+        Yh = torch.empty(Y.shape, dtype=Y.dtype, pin_memory=True)
+        torch.cuda.synchronize()
+        h0, h1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        h0.record()
+        for _ in range(args.steps):
+            X.copy_(Xh, non_blocking=True)
+            eng.run_torch(0, X, Y, **run_kw)
+            Yh.copy_(Y, non_blocking=True)
+        h1.record()
+        torch.cuda.synchronize()
+        pcie_ms = h0.elapsed_time(h1) / args.steps
+        log(f"with host X in / Y out over PCIe: {pcie_ms:.3f} ms/step")
+        del Xh, Yh
+
     # ---- the dominant kernel's own launch duration: HIP events around every
     # phase, on the stream the kernels run on, over as many sweeps as were timed
     gather_ms = None
@@ -272,18 +291,28 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        os.environ["OMP_NUM_THREADS"] = str(threads)
         sa = s_ref.arrays()
         xs = X[:, 0].cpu().numpy()
+        O.set_threads(threads)
         O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)  # warm-up
-        n, t0 = 0, time.time()
-        while n < 1 or time.time() - t0 < args.cpu_seconds:
+        times, t0 = [], time.time()
+        while not times or time.time() - t0 < args.cpu_seconds:
+            t1 = time.time()
             O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)
-            n += 1
-        cpu_s = (time.time() - t0) / n
+            times.append(time.time() - t1)
+        cpu_s = float(np.mean(times))
+        O.set_threads(1)
+        one = []
+        for _ in range(3):
+            t1 = time.time()
+            O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)
+            one.append(time.time() - t1)
         cpu = {"value": nnz / cpu_s, "unit": "nnz*vectors/s", "cores": threads, "kind": "port",
-               "sample": f"{n} single-vector sweeps of the same grammar (reference layout, FP32, OpenMP "
-                         f"{threads} threads), {cpu_s * 1e3:.1f} ms each"}
+               "sample": f"{len(times)} single-vector sweeps of the same grammar (reference layout, FP32, "
+                         f"OpenMP {threads} threads): mean {cpu_s * 1e3:.1f} ms, best {min(times) * 1e3:.1f} ms, "
+                         f"median {float(np.median(times)) * 1e3:.1f} ms; schedule build excluded",
+               "one_thread": {"value": nnz / min(one), "best_ms": min(one) * 1e3,
+                              "median_ms": float(np.median(one)) * 1e3, "sweeps": len(one)}}
         log("cpu baseline", cpu)
 
     out = {
@@ -310,6 +339,8 @@ def main():
         "plain_sweep": {"value": plain_val, "unit": "nnz/s", "ms": plain_ms,
                         "schedule": "refcompat/levels/spot B=1"},
         "batched_over_plain": value / plain_val,
+        "pcie_inclusive": {"ms": pcie_ms, "value": nnz * B / (pcie_ms * 1e-3), "unit": "nnz*vectors/s",
+                           "note": "X copied in from and Y copied out to pinned host memory inside every step"},
         "best_single_vector": {"value": nnz / (one_ms * 1e-3), "ms": one_ms, "schedule": autotune.describe(best1)},
         "fp32_err": err,
         "tuned": [(autotune.describe(c), round(ms, 4)) for c, ms in tuned],
@@ -353,7 +384,7 @@ def run_sweep_B(args, m, g, s_res, eng):
         Y = torch.empty(args.rows, B, device="cuda")
         if B == 1:
             X, Y = X[:, 0].contiguous(), Y[:, 0].contiguous()
-        best, _ = autotune.tune(eng, 0, X, Y, autotune.candidates(B, gmr.RESIDENT), reps=2)
+        best, tuned = autotune.tune(eng, 0, X, Y, autotune.candidates(B, gmr.RESIDENT), reps=2)
         ms = _time_steps(eng, 0, X, Y, best, args.steps, args.warmup)
         if best["variant"] == gmr.VARIANT_PERSISTENT:
             eng.check()
@@ -364,7 +395,8 @@ def run_sweep_B(args, m, g, s_res, eng):
         d = np.abs(Yg - Ys)
         err = float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1), np.where(d == 0, 0, np.inf))))
         r = {"B": B, "ms": ms, "nnz_vec_per_s": nnz * B / (ms * 1e-3), "gbs_dag": dag_bytes(info, B) / ms / 1e6,
-             "schedule": autotune.describe(best), "fp32_err": err}
+             "schedule": autotune.describe(best), "fp32_err": err,
+             "tuned": [(autotune.describe(c), round(t, 4)) for c, t in tuned]}
         log("B", B, r)
         out.append(r)
         del X, Y
@@ -405,7 +437,7 @@ def run_rmat(args):
             x[rng.choice(n, 1024, replace=False)] = 0.0
         xt = torch.from_numpy(x).cuda()
         y = torch.empty(n, dtype=torch.int32, device="cuda")
-        best, _ = autotune.tune(eng, sr, xt, y, autotune.candidates(1, gmr.RESIDENT), reps=2)
+        best, tuned = autotune.tune(eng, sr, xt, y, autotune.candidates(1, gmr.RESIDENT), reps=2)
         ms = _time_steps(eng, sr, xt, y, best, args.steps, args.warmup)
         seq, V = m.arrays()
         y_star, _ = O.brute_force(sr, n, n, seq, V, x)
@@ -420,7 +452,7 @@ def run_rmat(args):
                     "plain_bit_exact": exact_plain,
                     "grammar": {"rules": info["num_raw_nt"], "depth": info["max_depth"], "C": g.info()["clen"],
                                 "U": info["U"], "refcompat_passthrough": s_ref.info()["num_passthrough"]},
-                    "repair_s": g.build_seconds}
+                    "repair_s": g.build_seconds, "tuned": [(autotune.describe(c), round(t, 4)) for c, t in tuned]}
         log(tag, res[tag])
         del eng, eng_ref, m, g, s_res, s_ref, seq
         torch.cuda.empty_cache()

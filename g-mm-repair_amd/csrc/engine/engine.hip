@@ -67,8 +67,8 @@ Engine::~Engine() {
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
         if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
-    void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_level_off,
-                    d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds};
+    void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_fbits, d_ford,
+                    d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -196,6 +196,25 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
             for (int q = a0; q < a1; q += kGatherSeg) gseg.push_back(make_int4(r, q, std::min(a1, q + kGatherSeg), split));
         }
     }
+    // Single-vector flat gather: which symbols start a row, per 64-symbol chunk.
+    std::vector<unsigned long long> fbits;
+    std::vector<int> ford, frow;
+    if (resident) {
+        const long nsym = static_cast<long>(E) + Et;
+        fbits.assign(static_cast<std::size_t>((nsym + 63) / 64), 0ull);
+        for (int r = 0; r < rows; ++r) {
+            if (crow[r + 1] == crow[r]) continue;
+            const int p = crow[r];
+            fbits[p >> 6] |= 1ull << (p & 63);
+            frow.push_back(r);
+        }
+        ford.resize(fbits.size());
+        int before = 0;
+        for (std::size_t c = 0; c < fbits.size(); ++c) {
+            ford[c] = before;
+            before += __builtin_popcountll(fbits[c]);
+        }
+    }
 
     // Single-vector gather: lanes per row ~ a quarter of the mean row length.
     {
@@ -216,6 +235,10 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
         d_gseg = dev_upload(gseg.data(), gseg.size(), &struct_bytes);
         n_gseg = static_cast<int>(gseg.size());
         crow_host.assign(1, static_cast<int>(csym.size()));
+        d_fbits = dev_upload(fbits.data(), fbits.size(), &struct_bytes);
+        d_ford = dev_upload(ford.data(), ford.size(), &struct_bytes);
+        d_frow = dev_upload(frow.data(), frow.size(), &struct_bytes);
+        n_fchunk = static_cast<int>(fbits.size());
     }
     d_level_off = dev_upload(level_off.data(), level_off.size(), &struct_bytes);
     d_emit_off = dev_upload(emit_off.data(), emit_off.size(), &struct_bytes);
@@ -340,7 +363,9 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         }
     }
     if (a.gather) {
-        const long threads = (a.Bt == 1 && a.B == 1) ? static_cast<long>(en.n_gseg) * a.GR : static_cast<long>(en.n_gseg) * G;
+        const long threads = a.flat ? static_cast<long>(en.n_fchunk) * 64
+                             : (a.Bt == 1 && a.B == 1) ? static_cast<long>(en.n_gseg) * a.GR
+                                                       : static_cast<long>(en.n_gseg) * G;
         hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(threads, block)), dim3(block), 0, st, a);
     }
     if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * en.L + 1], st));
@@ -418,6 +443,11 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
     a.gseg = en.d_gseg;
     a.n_gseg = en.n_gseg;
     a.gsplit = en.gather_split;
+    a.fbits = en.d_fbits;
+    a.ford = en.d_ford;
+    a.frow = en.d_frow;
+    a.n_fchunk = en.n_fchunk;
+    a.n_csym = en.resident ? en.crow_host[0] : 0;
     a.level_off = en.d_level_off;
     a.emit_off = en.d_emit_off;
     a.X = X + b0;
@@ -438,6 +468,7 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
     a.G = dev::lanes_for(a.Bs, a.vec);
     a.resident = en.resident ? 1 : 0;
     a.gather = gather ? 1 : 0;
+    a.flat = (gather && a.Bt == 1 && B == 1 && !(o.flags & GMR_RUN_ROW_SEGMENTS)) ? 1 : 0;
     a.GR = en.gather_lanes;
     a.eval_kind = o.eval_kind;
     a.level_kind = o.level_kind ? en.device_kinds(o.level_kind) : nullptr;
@@ -539,41 +570,7 @@ void time_levels_t(Engine& en, const float* X, int B, void* Y, const gmr_run_opt
                    hipStream_t st) {
     using T = typename S::T;
     en.ensure_workspace(padded_width(B), static_cast<int>(sizeof(T)));
-    SweepArgs a{};
-    a.kids = en.d_kids;
-    a.coeff = en.d_coeff;
-    a.tcol = en.d_tcol;
-    a.tval = en.d_tval;
-    a.emits = en.d_emits;
-    a.term_emits = en.d_term;
-    a.crow = en.d_crow;
-    a.csym = en.d_csym;
-    a.gseg = en.d_gseg;
-    a.n_gseg = en.n_gseg;
-    a.gsplit = en.gather_split;
-    a.level_off = en.d_level_off;
-    a.emit_off = en.d_emit_off;
-    a.X = X;
-    a.Y = Y;
-    a.Tv = en.ws_T;
-    a.W0 = en.ws_W0;
-    a.W1 = en.ws_W1;
-    a.rows = en.rows;
-    a.cols = en.cols;
-    a.U = en.U;
-    a.L = en.L;
-    a.n_term = en.Et;
-    a.B = B;
-    a.Bt = B;
-    a.Bs = padded_width(B);
-    a.vec = B > 1 ? 1 : 0;
-    a.yvec = (B % 4 == 0) ? 1 : 0;
-    a.G = dev::lanes_for(a.Bs, a.vec);
-    a.resident = en.resident ? 1 : 0;
-    a.gather = (o.emit == GMR_EMIT_GATHER) ? 1 : 0;
-    a.GR = en.gather_lanes;
-    a.eval_kind = o.eval_kind;
-    a.level_kind = o.level_kind ? en.device_kinds(o.level_kind) : nullptr;
+    const SweepArgs a = make_args(en, X, B, Y, static_cast<int>(sizeof(T)), 0, B, o.emit == GMR_EMIT_GATHER, o);
     const int block = o.block > 0 ? o.block : 256;
     std::vector<hipEvent_t> ev(2 * en.L + 2);
     for (auto& e : ev) GMR_HIP(hipEventCreate(&e));
@@ -678,6 +675,7 @@ inline long emit_tile(const dev::SweepArgs& a, int block) {
     return (a.Bt == 1 && a.B == 1) ? static_cast<long>(block) * 4 : static_cast<long>(block / a.G) * dev::kEmitChunk * 2;
 }
 inline long gather_tile(const dev::SweepArgs& a, int block) {
+    if (a.flat) return static_cast<long>(block / 64) * 2;
     return (a.Bt == 1 && a.B == 1) ? std::max(1L, static_cast<long>(block / a.GR) * 2) : static_cast<long>(block / a.G) * 2;
 }
 
@@ -789,8 +787,8 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
                     for (long e = 0; e < ne; e += step) per[p].push_back({g, dev::TT_EMIT, p - 1, 0, e, std::min(ne, e + step)});
                 } else if (p == en.L + 1) {
                     const long step = gather_tile(a, block);
-                    for (long r = 0; r < en.n_gseg; r += step)
-                        per[p].push_back({g, dev::TT_GATHER, 0, 0, r, std::min<long>(en.n_gseg, r + step)});
+                    const long ng = dev::gather_items(a);
+                    for (long r = 0; r < ng; r += step) per[p].push_back({g, dev::TT_GATHER, 0, 0, r, std::min(ng, r + step)});
                 }
             }
         }

@@ -38,6 +38,10 @@ struct Engine {
     int gather_split = 0;            // some row is split over several segments
     static constexpr int kGatherSeg = 128;
     int gather_order = 0;            // 0: row order, 1: by highest rule
+    unsigned long long* d_fbits = nullptr;  // flat single-vector gather: row starts per 64 symbols
+    int* d_ford = nullptr;           // rows started before each 64-symbol chunk
+    int* d_frow = nullptr;           // non-empty rows in order
+    int n_fchunk = 0;
     std::vector<int> crow_host;      // [0] = number of top-sequence symbols
     int* d_level_off = nullptr;
     int* d_emit_off = nullptr;

@@ -70,6 +70,12 @@ struct SweepArgs {
     const int4* gseg;        // gather segments: (row, first symbol, end, row is split)
     int n_gseg;
     int gsplit;              // some row is split: y is zeroed and pieces add atomically
+    const unsigned long long* fbits;  // flat gather: bit i of word c set = csym[64c+i] starts a row
+    const int* ford;         // flat gather: rows started before csym[64c]
+    const int* frow;         // flat gather: row id of the k-th non-empty row
+    int n_fchunk;            // flat gather: 64-symbol chunks of csym
+    int n_csym;              // top-sequence symbols (csym entries)
+    int flat;                // one vector: gather over 64-symbol chunks instead of row segments
     const int* level_off;    // max_depth + 1
     const int* emit_off;     // max_depth + 1
     // values (device)
@@ -197,9 +203,15 @@ __device__ void ph_init_range(const SweepArgs& a, long i0, long i1, long tid, lo
 }
 
 // Items of the init phase: U*Bs terminal values, then (spot emission, or a
-// gather with split rows) rows*Bt y values.
+// gather that combines pieces of rows atomically) rows*Bt y values.
 __host__ __device__ __forceinline__ long init_items(const SweepArgs& a) {
-    return static_cast<long>(a.U) * a.Bs + ((a.gather && !a.gsplit) ? 0L : static_cast<long>(a.rows) * a.Bt);
+    return static_cast<long>(a.U) * a.Bs +
+           ((a.gather && !a.gsplit && !a.flat) ? 0L : static_cast<long>(a.rows) * a.Bt);
+}
+
+// Work items of the gather phase: 64-symbol chunks (flat) or row segments.
+__host__ __device__ __forceinline__ long gather_items(const SweepArgs& a) {
+    return a.flat ? static_cast<long>(a.n_fchunk) : static_cast<long>(a.n_gseg);
 }
 
 template <class S>
@@ -468,9 +480,51 @@ __device__ void ph_gather_rows_1(const SweepArgs& a, long q0, long q1, long tid,
     }
 }
 
+// One vector, flat: a wave takes 64 consecutive top-sequence symbols, one per
+// lane (so the ids are read coalesced), gathers their values and reduces them
+// per row with a segmented shuffle scan; the chunk's row-start bitmap marks the
+// segments. A row that starts and ends inside the chunk is stored; the pieces
+// of a row that crosses a chunk boundary combine atomically into y, which the
+// init phase zeroed (as it did every empty row).
+template <class S>
+__device__ void ph_gather_flat(const SweepArgs& a, long c0, long c1, long wid, long nw, int lane) {
+    using T = typename S::T;
+    const T* W = tp<S>(a.W0);
+    const T* Tt = tp<S>(a.Tv);
+    T* Y = tp<S>(a.Y);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
+    for (long c = c0 + wid; c < c1; c += nw) {
+        const long p = c * 64 + lane;
+        const bool valid = p < a.n_csym;
+        T v = S::zero();
+        if (valid) {
+            const int s = a.csym[p];
+            v = s >= 0 ? W[s] : Tt[-s - 1];
+        }
+        const unsigned long long bits = a.fbits[c];
+        // Inclusive scan inside segments: lane takes lane-d's partial sum when
+        // no segment starts in (lane-d, lane].
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const T t = __shfl_up(v, d, 64);
+            if (lane >= d && ((bits >> (lane - d + 1)) & ((1ull << d) - 1ull)) == 0ull) v = S::oplus(v, t);
+        }
+        const bool last = valid && (lane == 63 || p + 1 == a.n_csym || ((bits >> (lane + 1)) & 1ull) != 0ull);
+        if (last) {
+            const int r = a.frow[a.ford[c] - 1 + __popcll(bits & upto)];
+            const bool starts_here = (bits & upto) != 0ull;
+            const bool ends_here = lane < 63 || a.crow[r + 1] == p + 1;
+            if (starts_here && ends_here) Y[r] = v;
+            else if (!S::skip(v)) S::emit(Y + r, v);
+        }
+    }
+}
+
 template <class S>
 __device__ __forceinline__ void ph_gather_range(const SweepArgs& a, long r0, long r1, long tid, long nth) {
-    if (a.Bt == 1 && a.B == 1) {
+    if (a.flat) {
+        ph_gather_flat<S>(a, r0, r1, tid / 64, nth / 64, static_cast<int>(tid % 64));
+    } else if (a.Bt == 1 && a.B == 1) {
         ph_gather_rows_1<S>(a, r0, r1, tid, nth);
     } else if (a.vec) {
         ph_gather_rows_v4<S>(a, r0, r1, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
@@ -481,7 +535,7 @@ __device__ __forceinline__ void ph_gather_range(const SweepArgs& a, long r0, lon
 
 template <class S>
 __device__ __forceinline__ void ph_gather(const SweepArgs& a, long tid, long nth) {
-    ph_gather_range<S>(a, 0, a.n_gseg, tid, nth);
+    ph_gather_range<S>(a, 0, gather_items(a), tid, nth);
 }
 
 // EV_THREAD: flattened (rule, chunk) items, chunk = 4 vectors (vec) or 1.

@@ -11,7 +11,7 @@ import hashlib
 import json
 import os
 
-from . import EMIT_GATHER, EMIT_SPOT, RESIDENT, VARIANT_GRAPH, VARIANT_LEVELS, VARIANT_PERSISTENT
+from . import EMIT_GATHER, EMIT_SPOT, RESIDENT, RUN_ROW_SEGMENTS, VARIANT_GRAPH, VARIANT_LEVELS, VARIANT_PERSISTENT
 
 VARIANT_NAMES = {VARIANT_LEVELS: "levels", VARIANT_GRAPH: "graph", VARIANT_PERSISTENT: "persistent"}
 EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
@@ -19,7 +19,8 @@ EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
 
 def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), (1024, 0), (256, 256))):
     """The search space for one engine and batch width: variant x emission x
-    vector tile x launch shape (x gather order for row-gather emission).
+    vector tile x launch shape (x gather order for row-gather emission of
+    several vectors; x chunked or row-segment gather for one vector).
     Persistent shapes are (block, grid cap); a cap of 0 means the engine's
     default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
@@ -30,9 +31,13 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
             for t in tiles:
                 shapes = persistent_shapes if v == VARIANT_PERSISTENT else [(b, 0) for b in blocks]
                 for blk, cap in shapes:
-                    for order in ((0, 1) if e == EMIT_GATHER else (0,)):
-                        out.append(dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap,
-                                        gather_order=order))
+                    for order in ((0, 1) if e == EMIT_GATHER and B > 1 else (0,)):
+                        for fl in ((0, RUN_ROW_SEGMENTS) if e == EMIT_GATHER and B == 1 else (0,)):
+                            c = dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap,
+                                     gather_order=order)
+                            if fl:
+                                c["flags"] = fl
+                            out.append(c)
     return out
 
 
@@ -48,7 +53,8 @@ def apply(engine, cand):
 def describe(c):
     cap = c.get("grid_cap", 0)
     return (f"{VARIANT_NAMES[c['variant']]}/{EMIT_NAMES[c['emit']]}/tile={c['b_tile'] or 'B'}/blk={c['block']}"
-            + (f"/grid<={cap}" if cap else "") + ("/order=rule" if c.get("gather_order") == 1 else ""))
+            + (f"/grid<={cap}" if cap else "") + ("/order=rule" if c.get("gather_order") == 1 else "")
+            + ("/segments" if c.get("flags", 0) & RUN_ROW_SEGMENTS else ""))
 
 
 def shape_key(info, B, semiring):
@@ -86,11 +92,20 @@ def tune(engine, semiring, X, Y, cands, cache_path=None, reps=3, log=None):
         with open(cache_path) as f:
             db = json.load(f)
     results = []
-    for c in cands:
-        ms = time_candidate(engine, semiring, X, Y, c, reps=reps)
-        results.append((c, ms))
-        if log:
-            log(f"  tune {describe(c):40s} {ms * 1e3:9.1f} us")
+    if key in db:
+        cached = {json.dumps({k: v for k, v in r.items() if k != "ms"}, sort_keys=True): r["ms"] for r in db[key]}
+        results = [(c, cached[json.dumps(c, sort_keys=True)]) for c in cands
+                   if json.dumps(c, sort_keys=True) in cached]
+        if len(results) != len(cands):
+            results = []  # a different search space: time it again
+        elif log:
+            log(f"  tune: {len(results)} timings from {cache_path}")
+    if not results:
+        for c in cands:
+            ms = time_candidate(engine, semiring, X, Y, c, reps=reps)
+            results.append((c, ms))
+            if log:
+                log(f"  tune {describe(c):40s} {ms * 1e3:9.1f} us")
     best = min(results, key=lambda r: r[1])[0]
     if cache_path:
         db[key] = [dict(c, ms=ms) for c, ms in results]

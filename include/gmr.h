@@ -51,7 +51,11 @@ enum gmr_eval_kind {
     GMR_EVAL_LDS = 3      /* single vector: left-child window staged in LDS */
 };
 
-enum gmr_run_flags { GMR_RUN_TIMED = 1 /* record device time (see gmr_engine_last_kernel_ms) */ };
+enum gmr_run_flags {
+    GMR_RUN_TIMED = 1,        /* record device time (see gmr_engine_last_kernel_ms) */
+    GMR_RUN_ROW_SEGMENTS = 2  /* one vector, row gather: lanes per row segment instead of one lane
+                                 per top-sequence symbol (64-symbol chunks) */
+};
 
 typedef struct gmr_run_opts {
     int variant;            /* gmr_variant */

@@ -24,6 +24,7 @@
  *                        proposer network has no reference counterpart)
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -383,6 +384,10 @@ int orc_sweep(int semiring, int rows, int cols, int U, const int* term_orig, con
  * frontier is copied after every level. This is the CPU baseline the engine is
  * timed against. y holds float bits for (+,x) and int32 bits otherwise.
  * (reference gpu-engine/gpu_engine_test.cu:49-104; policies gpu_engine.h:46-97) */
+/* Threads of the OpenMP sweep below (the CPU baseline is timed at a stated
+ * thread count). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 void orc_cpu_layered_f32(int semiring, int rows, int U, const int* val_idx, const int* col_idx,
                          const float* V, const float* x, int L, const int* rleft,
                          const int* rright, const float* rcoeff, const int* level_offsets,

@@ -162,6 +162,11 @@ def sweep(semiring, s, rows, cols, V, x):
     return y
 
 
+def set_threads(n):
+    """OpenMP threads of cpu_layered_f32."""
+    lib().orc_set_threads(C.c_int(n))
+
+
 def cpu_layered_f32(semiring, s, rows, cols, V, x):
     """The reference's OpenMP FP32/int32 CPU sweep (the CPU baseline)."""
     to = s["term_orig"]

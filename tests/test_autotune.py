@@ -1,0 +1,60 @@
+"""The exhaustive autotuner's search space and disk cache (CPU; the device
+timing is replaced by a deterministic stand-in)."""
+import gmr
+from gmr import autotune
+
+
+class _Engine:
+    def info(self):
+        return {"rows": 10, "cols": 4, "U": 5, "max_depth": 3}
+
+    def set_gather_order(self, order):
+        pass
+
+
+class _X:
+    def __init__(self, B):
+        self.B = B
+
+    def dim(self):
+        return 1 if self.B == 1 else 2
+
+    @property
+    def shape(self):
+        return (4, self.B)
+
+
+def test_candidates_cover_variants_emits_tiles_and_orders():
+    c64 = autotune.candidates(64, gmr.RESIDENT)
+    descr = {autotune.describe(c) for c in c64}
+    assert "levels/gather/tile=B/blk=256" in descr
+    assert "graph/gather/tile=16/blk=256/order=rule" in descr
+    assert "persistent/spot/tile=32/blk=1024" in descr
+    assert len(descr) == len(c64)
+    # one vector: no tiles below B, and the gather order does not apply
+    c1 = autotune.candidates(1, gmr.RESIDENT)
+    assert all(c["b_tile"] == 0 and c["gather_order"] == 0 for c in c1)
+    # double-buffered layouts have no row gather
+    assert all(c["emit"] == gmr.EMIT_SPOT for c in autotune.candidates(16, gmr.WAVE64))
+
+
+def test_tune_picks_the_fastest_and_reuses_the_disk_cache(tmp_path, monkeypatch):
+    calls = []
+
+    def fake_time(engine, semiring, X, Y, cand, reps=3, warm=1):
+        calls.append(autotune.describe(cand))
+        return 1.0 + 0.1 * cand["b_tile"] + (0.5 if cand["emit"] == gmr.EMIT_SPOT else 0.0) + 0.01 * cand["block"] / 256
+
+    monkeypatch.setattr(autotune, "time_candidate", fake_time)
+    cache = str(tmp_path / "tune.json")
+    cands = autotune.candidates(64, gmr.RESIDENT)
+    best, res = autotune.tune(_Engine(), 0, _X(64), None, cands, cache_path=cache)
+    assert len(calls) == len(cands) and len(res) == len(cands)
+    assert best["emit"] == gmr.EMIT_GATHER and best["b_tile"] == 0 and best["block"] == 256
+    n = len(calls)
+    best2, res2 = autotune.tune(_Engine(), 0, _X(64), None, cands, cache_path=cache)
+    assert len(calls) == n  # every timing came from the cache
+    assert best2 == best and [ms for _, ms in res2] == [ms for _, ms in res]
+    # another batch width is another key: timed again
+    autotune.tune(_Engine(), 0, _X(16), None, autotune.candidates(16, gmr.RESIDENT), cache_path=cache)
+    assert len(calls) > n

@@ -398,3 +398,29 @@ def test_gather_orders_agree(hap):
         for var in ("levels", "graph", "persistent"):
             Y = run_engine(eng, 0, X, variant=gmr.VARIANTS[var], emit=gmr.EMIT_GATHER)
             check(0, Y, Y_star, den, f"gather order {order} {var}")
+
+
+@pytest.mark.parametrize("sr", [0, 1, 2])
+def test_single_vector_gathers_agree(sr):
+    """One vector, row gather: the chunked kernel (a wave per 64 top-sequence
+    symbols, segmented scan, atomics only for rows crossing a chunk) and the
+    row-segment kernel, on a matrix whose rows are empty, 1 symbol, exactly 64,
+    and long enough to cross several chunks (and split into several segments),
+    with a symbol count that is not a multiple of 64."""
+    rng = np.random.default_rng(77 + sr)
+    rows, cols = 40, 300
+    lens = [0, 1, 64, 63, 65, 0, 0, 200, 1, 129, 2, 0, 300, 5] + list(rng.integers(0, 90, rows - 14))
+    M = np.zeros((rows, cols), np.float64)
+    for r, n in enumerate(lens):
+        M[r, rng.choice(cols, int(n), replace=False)] = rng.integers(1, 4, int(n)) if sr == 2 else 1.0
+    m = gmr.Matrix.from_dense(M)
+    g = m.repair()
+    seq, V = m.arrays()
+    x = x_for(sr, cols, 5 + sr)
+    y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
+    eng = g.schedule(rows, cols, "resident").engine()
+    for flags in (0, gmr.RUN_ROW_SEGMENTS, 0):
+        for var in ("levels", "graph", "persistent"):
+            _say("one-vector gather", sr, flags, var)
+            y = run_engine(eng, sr, x, variant=gmr.VARIANTS[var], emit=gmr.EMIT_GATHER, flags=flags)
+            check(sr, y, y_star, den, f"gather flags={flags} {var}")
