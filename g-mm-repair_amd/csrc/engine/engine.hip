@@ -28,6 +28,7 @@
 #include "build_schedule.h"
 #include "engine_internal.hpp"
 #include "gmr.h"
+#include "left_kernels.hpp"
 #include "sweep_kernels.hpp"
 
 const BuiltSchedule* gmr_schedule_internal(const gmr_schedule* s);
@@ -63,12 +64,17 @@ T* dev_upload(const T* h, std::size_t n, std::size_t* acc) {
 }  // namespace
 
 Engine::~Engine() {
-    for (auto& kv : graphs) {
-        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
-        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
-    }
+    for (auto* m : {&graphs, &left_graphs})
+        for (auto& kv : *m) {
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+            if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+        }
+    for (void* p : {static_cast<void*>(d_in_off), static_cast<void*>(d_in_src), static_cast<void*>(d_in_coeff),
+                    static_cast<void*>(d_col_off), static_cast<void*>(d_col_term), ws_A})
+        if (p) (void)hipFree(p);
     void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_fbits, d_ford,
-                    d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds};
+                    d_frow, d_xsym, d_xbits, d_xord, d_xrow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1,
+                    d_barrier, d_kinds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -215,6 +221,63 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
             before += __builtin_popcountll(fbits[c]);
         }
     }
+    // XCD-partitioned single-vector gather: value keys (terminal t -> t, rule
+    // slot s -> U + s) cut into 8 ranges of equal use counts; each range keeps
+    // its symbols in row order, padded to whole 64-symbol chunks with the zero
+    // terminal (which adds the semiring zero to the row before it).
+    std::vector<int> xsym, xord, xrow;
+    std::vector<unsigned long long> xbits;
+    if (resident) {
+        const long nsym = static_cast<long>(csym.size());
+        const long nkey = static_cast<long>(U) + NL;
+        auto key = [&](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : static_cast<long>(U) + c; };
+        std::vector<long> uses(static_cast<std::size_t>(nkey) + 1, 0);
+        for (int c : csym) ++uses[key(c) + 1];
+        for (long k = 0; k < nkey; ++k) uses[k + 1] += uses[k];
+        long cut[9];
+        cut[0] = 0;
+        cut[8] = nkey;
+        for (int p = 1; p < 8; ++p)
+            cut[p] = std::lower_bound(uses.begin(), uses.end(), nsym * p / 8) - uses.begin();
+        for (int p = 1; p < 8; ++p) cut[p] = std::max(cut[p], cut[p - 1]);
+        auto part = [&](int c) -> int { return static_cast<int>(std::upper_bound(cut + 1, cut + 8, key(c)) - (cut + 1)); };
+        std::vector<std::vector<int>> lsym(8), lrow(8);
+        std::vector<std::vector<long>> lstart(8);  // positions (in the list) that start a row
+        for (int r = 0; r < rows; ++r) {
+            int last_p = -1;
+            unsigned seen = 0;
+            for (int q = crow[r]; q < crow[r + 1]; ++q) {
+                const int p = part(csym[q]);
+                if (!(seen & (1u << p))) {
+                    seen |= 1u << p;
+                    lstart[p].push_back(static_cast<long>(lsym[p].size()));
+                    lrow[p].push_back(r);
+                }
+                lsym[p].push_back(csym[q]);
+                last_p = p;
+            }
+            (void)last_p;
+        }
+        int chunks = 0, rbase = 0;
+        for (int p = 0; p < 8; ++p) {
+            xpoff[p] = chunks;
+            const long n = static_cast<long>(lsym[p].size());
+            const long nc = (n + 63) / 64;
+            std::vector<unsigned long long> bits(static_cast<std::size_t>(nc), 0ull);
+            for (long st : lstart[p]) bits[st >> 6] |= 1ull << (st & 63);
+            int before_p = rbase;
+            for (long c = 0; c < nc; ++c) {
+                xbits.push_back(bits[c]);
+                xord.push_back(before_p);
+                before_p += __builtin_popcountll(bits[c]);
+            }
+            for (long q = 0; q < nc * 64; ++q) xsym.push_back(q < n ? lsym[p][q] : -1);
+            xrow.insert(xrow.end(), lrow[p].begin(), lrow[p].end());
+            rbase += static_cast<int>(lrow[p].size());
+            chunks += static_cast<int>(nc);
+        }
+        xpoff[8] = chunks;
+    }
 
     // Single-vector gather: lanes per row ~ a quarter of the mean row length.
     {
@@ -239,6 +302,10 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
         d_ford = dev_upload(ford.data(), ford.size(), &struct_bytes);
         d_frow = dev_upload(frow.data(), frow.size(), &struct_bytes);
         n_fchunk = static_cast<int>(fbits.size());
+        d_xsym = dev_upload(xsym.data(), xsym.size(), &struct_bytes);
+        d_xbits = dev_upload(xbits.data(), xbits.size(), &struct_bytes);
+        d_xord = dev_upload(xord.data(), xord.size(), &struct_bytes);
+        d_xrow = dev_upload(xrow.data(), xrow.size(), &struct_bytes);
     }
     d_level_off = dev_upload(level_off.data(), level_off.size(), &struct_bytes);
     d_emit_off = dev_upload(emit_off.data(), emit_off.size(), &struct_bytes);
@@ -363,7 +430,8 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         }
     }
     if (a.gather) {
-        const long threads = a.flat ? static_cast<long>(en.n_fchunk) * 64
+        const long threads = a.xparts ? static_cast<long>(en.xpoff[8]) * 64
+                             : a.flat ? static_cast<long>(en.n_fchunk) * 64
                              : (a.Bt == 1 && a.B == 1) ? static_cast<long>(en.n_gseg) * a.GR
                                                        : static_cast<long>(en.n_gseg) * G;
         hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(threads, block)), dim3(block), 0, st, a);
@@ -469,6 +537,12 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
     a.resident = en.resident ? 1 : 0;
     a.gather = gather ? 1 : 0;
     a.flat = (gather && a.Bt == 1 && B == 1 && !(o.flags & GMR_RUN_ROW_SEGMENTS)) ? 1 : 0;
+    a.xsym = en.d_xsym;
+    a.xbits = en.d_xbits;
+    a.xord = en.d_xord;
+    a.xrow = en.d_xrow;
+    for (int p = 0; p < 9; ++p) a.xpoff[p] = en.xpoff[p];
+    a.xparts = (a.flat && en.resident && !(o.flags & GMR_RUN_SINGLE_LIST)) ? 1 : 0;
     a.GR = en.gather_lanes;
     a.eval_kind = o.eval_kind;
     a.level_kind = o.level_kind ? en.device_kinds(o.level_kind) : nullptr;
@@ -610,6 +684,175 @@ void Engine::time_levels(int semiring, const float* X, int B, void* Y, const gmr
 }
 
 
+// ---------------------------------------------------------------- left product
+// Incoming lists of every node, built once from the uploaded structure.
+void Engine::build_left() {
+    if (left_ready) return;
+    if (!resident) throw std::invalid_argument("the left product needs the Resident layout");
+    GMR_HIP(hipDeviceSynchronize());
+    std::vector<int2> kids(NL), em(E), te(Et);
+    std::vector<float> coeff(has_runlen ? NL : 0);
+    std::vector<int> tcol(U);
+    if (NL) GMR_HIP(hipMemcpy(kids.data(), d_kids, sizeof(int2) * NL, hipMemcpyDeviceToHost));
+    if (has_runlen && NL) GMR_HIP(hipMemcpy(coeff.data(), d_coeff, sizeof(float) * NL, hipMemcpyDeviceToHost));
+    if (E) GMR_HIP(hipMemcpy(em.data(), d_emits, sizeof(int2) * E, hipMemcpyDeviceToHost));
+    if (Et) GMR_HIP(hipMemcpy(te.data(), d_term, sizeof(int2) * Et, hipMemcpyDeviceToHost));
+    if (U) GMR_HIP(hipMemcpy(tcol.data(), d_tcol, sizeof(int) * U, hipMemcpyDeviceToHost));
+    const long nodes = static_cast<long>(NL) + U;
+    // A child c >= 0 is a slot; c < 0 the compact terminal -c-1 (node NL + t).
+    // Terminal 0 is the semiring zero: nothing flows into it that matters.
+    auto node_of = [&](int c) -> long { return c >= 0 ? c : static_cast<long>(NL) + (-c - 1); };
+    auto wanted = [&](int c) { return c >= 0 || c < -1; };
+    std::vector<long> off(static_cast<std::size_t>(nodes) + 1, 0);
+    for (int p = 0; p < NL; ++p) {
+        if (wanted(kids[p].x)) ++off[node_of(kids[p].x) + 1];
+        if (wanted(kids[p].y)) ++off[node_of(kids[p].y) + 1];
+    }
+    for (const int2& e : em) ++off[e.x + 1];
+    for (const int2& e : te)
+        if (e.x > 0) ++off[static_cast<long>(NL) + e.x + 1];
+    for (long n = 0; n < nodes; ++n) off[n + 1] += off[n];
+    if (off[nodes] > 0x7fffffffL) throw std::invalid_argument("left product: more than 2^31 incoming entries");
+    std::vector<int> src(static_cast<std::size_t>(off[nodes]));
+    std::vector<float> cf(has_runlen ? src.size() : 0);
+    std::vector<long> fill(off.begin(), off.end() - 1);
+    // Parents in slot order, then the top-sequence uses in row order.
+    for (int p = 0; p < NL; ++p) {
+        if (wanted(kids[p].x)) {
+            const long q = fill[node_of(kids[p].x)]++;
+            src[q] = p;
+            if (has_runlen) cf[q] = coeff[p];
+        }
+        if (wanted(kids[p].y)) {
+            const long q = fill[node_of(kids[p].y)]++;
+            src[q] = p;
+            if (has_runlen) cf[q] = 1.0f;
+        }
+    }
+    for (const int2& e : em) {
+        const long q = fill[e.x]++;
+        src[q] = -e.y - 1;
+        if (has_runlen) cf[q] = 1.0f;
+    }
+    for (const int2& e : te) {
+        if (e.x <= 0) continue;
+        const long q = fill[static_cast<long>(NL) + e.x]++;
+        src[q] = -e.y - 1;
+        if (has_runlen) cf[q] = 1.0f;
+    }
+    std::vector<int> off32(off.begin(), off.end());
+    std::vector<int> coff(static_cast<std::size_t>(cols) + 1, 0), cterm(U > 0 ? U - 1 : 0);
+    for (int t = 1; t < U; ++t) ++coff[tcol[t] + 1];
+    for (int c = 0; c < cols; ++c) coff[c + 1] += coff[c];
+    {
+        std::vector<int> cf2(coff.begin(), coff.end() - 1);
+        for (int t = 1; t < U; ++t) cterm[cf2[tcol[t]]++] = t;
+    }
+    d_in_off = dev_upload(off32.data(), off32.size(), &struct_bytes);
+    d_in_src = dev_upload(src.data(), src.size(), &struct_bytes);
+    d_in_coeff = has_runlen ? dev_upload(cf.data(), cf.size(), &struct_bytes) : nullptr;
+    d_col_off = dev_upload(coff.data(), coff.size(), &struct_bytes);
+    d_col_term = dev_upload(cterm.data(), cterm.size(), &struct_bytes);
+    left_ready = true;
+}
+
+namespace {
+
+template <class S>
+void launch_left(Engine& en, const dev::LeftArgs& a, int block, hipStream_t st) {
+    for (int k = en.L; k >= 1; --k) {
+        const int n0 = en.level_off[k - 1], n1 = en.level_off[k];
+        if (n1 > n0)
+            hipLaunchKernelGGL(dev::k_left_nodes<S>, dim3(grid_for(static_cast<long>(n1 - n0) * a.G, block)),
+                               dim3(block), 0, st, a, n0, n1);
+    }
+    if (en.U > 0)
+        hipLaunchKernelGGL(dev::k_left_nodes<S>, dim3(grid_for(static_cast<long>(en.U) * a.G, block)), dim3(block), 0,
+                           st, a, en.NL, en.NL + en.U);
+    if (en.cols > 0)
+        hipLaunchKernelGGL(dev::k_left_cols<S>, dim3(grid_for(static_cast<long>(en.cols) * a.G, block)), dim3(block),
+                           0, st, a, en.cols);
+    GMR_HIP(hipGetLastError());
+}
+
+template <class S>
+void run_left_t(Engine& en, int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st) {
+    using T = typename S::T;
+    const int Bt = (o.b_tile > 0 && o.b_tile < B) ? o.b_tile : B;
+    const int block = o.block > 0 ? o.block : 256;
+    const std::size_t need = static_cast<std::size_t>(std::max<long>(static_cast<long>(en.NL) + en.U, 1)) * Bt * sizeof(T);
+    if (need > en.ws_A_bytes) {
+        for (auto& kv : en.left_graphs) {
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+            if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+        }
+        en.left_graphs.clear();
+        GMR_HIP(hipDeviceSynchronize());
+        if (en.ws_A) GMR_HIP(hipFree(en.ws_A));
+        en.ws_A = nullptr;
+        GMR_HIP(hipMalloc(&en.ws_A, need));
+        en.ws_A_bytes = need;
+    }
+    for (int b0 = 0; b0 < B; b0 += Bt) {
+        dev::LeftArgs a{};
+        a.in_off = en.d_in_off;
+        a.in_src = en.d_in_src;
+        a.in_coeff = en.d_in_coeff;
+        a.col_off = en.d_col_off;
+        a.col_term = en.d_col_term;
+        a.tval = en.d_tval;
+        a.X = X + b0;
+        a.Y = static_cast<char*>(Y) + static_cast<std::size_t>(b0) * sizeof(T);
+        a.A = en.ws_A;
+        a.NL = en.NL;
+        a.B = B;
+        a.Bt = std::min(Bt, B - b0);
+        a.Bs = a.Bt;
+        a.G = dev::lanes_for(a.Bt, 0);
+        if (o.variant == GMR_VARIANT_GRAPH) {
+            auto key = std::make_tuple(semiring, reinterpret_cast<uintptr_t>(a.X), reinterpret_cast<uintptr_t>(a.Y), a.B,
+                                       a.Bt, 0, block, reinterpret_cast<uintptr_t>(st));
+            auto it = en.left_graphs.find(key);
+            if (it == en.left_graphs.end()) {
+                if (!en.capture_stream) GMR_HIP(hipStreamCreateWithFlags(&en.capture_stream, hipStreamNonBlocking));
+                Engine::Graph g;
+                GMR_HIP(hipStreamBeginCapture(en.capture_stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    launch_left<S>(en, a, block, en.capture_stream);
+                } catch (...) {
+                    hipGraph_t dummy;
+                    (void)hipStreamEndCapture(en.capture_stream, &dummy);
+                    throw;
+                }
+                GMR_HIP(hipStreamEndCapture(en.capture_stream, &g.graph));
+                GMR_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+                it = en.left_graphs.emplace(key, g).first;
+            }
+            GMR_HIP(hipGraphLaunch(it->second.exec, st));
+        } else {
+            launch_left<S>(en, a, block, st);
+        }
+    }
+}
+
+}  // namespace
+
+void Engine::run_left(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st) {
+    if (B < 1) throw std::invalid_argument("B must be >= 1");
+    if (!X || !Y) throw std::invalid_argument("X or Y is NULL");
+    if (o.block != 0 && (o.block < 64 || o.block > 1024 || o.block % 64))
+        throw std::invalid_argument("block must be a multiple of 64 in [64, 1024]");
+    if (o.variant != GMR_VARIANT_LEVELS && o.variant != GMR_VARIANT_GRAPH)
+        throw std::invalid_argument("the left product runs as per-level launches or one hipGraph");
+    build_left();
+    switch (semiring) {
+        case GMR_PLUS_TIMES: run_left_t<dev::PlusTimesOp>(*this, semiring, X, B, Y, o, st); break;
+        case GMR_BOOLEAN: run_left_t<dev::BooleanOp>(*this, semiring, X, B, Y, o, st); break;
+        case GMR_TROPICAL: run_left_t<dev::TropicalOp>(*this, semiring, X, B, Y, o, st); break;
+        default: throw std::invalid_argument("unknown semiring");
+    }
+}
+
 // ---------------------------------------------------------------- batches
 // A fixed set of engines swept together (Y_g = M_g X_g, B_g vectors each).
 struct Batch {
@@ -710,6 +953,10 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
         bt.eng[g]->ensure_workspace(padded_width(B[g]), static_cast<int>(sizeof(T)));
         args[g] = make_args(*bt.eng[g], X[g], B[g], Y[g], static_cast<int>(sizeof(T)), 0, B[g],
                             o[g].emit == GMR_EMIT_GATHER, o[g]);
+        // Tiles of the fused kernel are taken by whichever workgroup is free, so
+        // the XCD-partitioned gather (which needs workgroup b on XCD b % 8) is
+        // replaced there by the single chunked list.
+        if (variant == GMR_VARIANT_PERSISTENT) args[g].xparts = 0;
     }
     if (variant == GMR_VARIANT_GRAPH) {
         std::size_t key = mix(static_cast<std::size_t>(semiring), static_cast<std::size_t>(block));
@@ -917,6 +1164,16 @@ int gmr_engine_time_levels(gmr_engine* h, int semiring, const void* d_X, int B, 
     return dev_guarded([&] {
         h->e.time_levels(semiring, static_cast<const float*>(d_X), B, d_Y, o, reps, level_ms,
                          static_cast<hipStream_t>(stream));
+    });
+}
+
+int gmr_engine_run_left(gmr_engine* h, int semiring, const void* d_X, int B, void* d_Y, const gmr_run_opts* opts,
+                        void* stream) {
+    if (!h) return dev_fail(GMR_ERR_INVALID, "engine is NULL");
+    gmr_run_opts o{};
+    if (opts) o = *opts;
+    return dev_guarded([&] {
+        h->e.run_left(semiring, static_cast<const float*>(d_X), B, d_Y, o, static_cast<hipStream_t>(stream));
     });
 }
 

@@ -42,6 +42,11 @@ struct Engine {
     int* d_ford = nullptr;           // rows started before each 64-symbol chunk
     int* d_frow = nullptr;           // non-empty rows in order
     int n_fchunk = 0;
+    int* d_xsym = nullptr;           // XCD-partitioned single-vector gather (8 padded lists)
+    unsigned long long* d_xbits = nullptr;
+    int* d_xord = nullptr;
+    int* d_xrow = nullptr;
+    int xpoff[9] = {0};
     std::vector<int> crow_host;      // [0] = number of top-sequence symbols
     int* d_level_off = nullptr;
     int* d_emit_off = nullptr;
@@ -71,6 +76,17 @@ struct Engine {
     using GraphKey = std::tuple<int, uintptr_t, uintptr_t, int, int, int, int, uintptr_t>;
     std::map<GraphKey, Graph> graphs;
 
+    // left product (built on first use, Resident layout only)
+    bool left_ready = false;
+    int* d_in_off = nullptr;         // nodes + 1 (rule slots, then terminals)
+    int* d_in_src = nullptr;         // >= 0 parent slot, < 0 row -e-1
+    float* d_in_coeff = nullptr;     // per entry, only with run-length rules
+    int* d_col_off = nullptr;        // cols + 1
+    int* d_col_term = nullptr;
+    void* ws_A = nullptr;            // nodes x Bt values
+    std::size_t ws_A_bytes = 0;
+    std::map<GraphKey, Graph> left_graphs;
+
     Engine() = default;
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
@@ -87,6 +103,9 @@ struct Engine {
     void set_gather_order(int order);
     void time_levels(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, int reps, float* out,
                      hipStream_t st);
+    // Y^T = X^T M: X rows x B, Y cols x B (Resident layout).
+    void run_left(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st);
+    void build_left();
 };
 
 }  // namespace gmr

@@ -76,6 +76,15 @@ struct SweepArgs {
     int n_fchunk;            // flat gather: 64-symbol chunks of csym
     int n_csym;              // top-sequence symbols (csym entries)
     int flat;                // one vector: gather over 64-symbol chunks instead of row segments
+    // One vector, XCD-partitioned gather: the symbols split into 8 value ranges
+    // (terminals then rule slots, equal use counts), each a row-ordered list
+    // padded to 64-symbol chunks with the zero terminal.
+    const int* xsym;         // the 8 lists, concatenated
+    const unsigned long long* xbits;  // per chunk: which symbols start a row of their list
+    const int* xord;         // per chunk: index into xrow of the first row started before it
+    const int* xrow;         // per list: its non-empty rows in order
+    int xpoff[9];            // chunk range of list p: [xpoff[p], xpoff[p+1])
+    int xparts;              // use the partitioned gather
     const int* level_off;    // max_depth + 1
     const int* emit_off;     // max_depth + 1
     // values (device)
@@ -206,7 +215,7 @@ __device__ void ph_init_range(const SweepArgs& a, long i0, long i1, long tid, lo
 // gather that combines pieces of rows atomically) rows*Bt y values.
 __host__ __device__ __forceinline__ long init_items(const SweepArgs& a) {
     return static_cast<long>(a.U) * a.Bs +
-           ((a.gather && !a.gsplit && !a.flat) ? 0L : static_cast<long>(a.rows) * a.Bt);
+           ((a.gather && !a.gsplit && !a.flat && !a.xparts) ? 0L : static_cast<long>(a.rows) * a.Bt);
 }
 
 // Work items of the gather phase: 64-symbol chunks (flat) or row segments.
@@ -520,6 +529,50 @@ __device__ void ph_gather_flat(const SweepArgs& a, long c0, long c1, long wid, l
     }
 }
 
+// One vector, XCD-partitioned: workgroups are dispatched to the 8 XCDs round-
+// robin, so workgroup b works on list b % 8 and every XCD's L2 only holds the
+// values of one eighth of the terminals and rule slots. Each list is swept like
+// the chunked gather; every row piece is combined atomically into the zeroed y,
+// since the row's other symbols live in other lists. `blk`/`nblk` are the raw
+// workgroup index and count (not renumbered).
+template <class S>
+__device__ void ph_gather_parts(const SweepArgs& a, long blk, long nblk, int tid_in_blk, int bdim) {
+    using T = typename S::T;
+    const T* W = tp<S>(a.W0);
+    const T* Tt = tp<S>(a.Tv);
+    T* Y = tp<S>(a.Y);
+    const int wpb = bdim >> 6, w = tid_in_blk >> 6, lane = tid_in_blk & 63;
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    long p0, pstep, j, nj;
+    if (nblk >= 8) {
+        p0 = blk & 7;
+        pstep = 8;
+        j = blk >> 3;
+        nj = (nblk >> 3) + ((blk & 7) < (nblk & 7) ? 1 : 0);
+    } else {
+        p0 = blk;
+        pstep = nblk;
+        j = 0;
+        nj = 1;
+    }
+    for (long p = p0; p < 8; p += pstep) {
+        for (long c = a.xpoff[p] + j * wpb + w; c < a.xpoff[p + 1]; c += nj * wpb) {
+            const int s = a.xsym[c * 64 + lane];
+            T v = s >= 0 ? W[s] : Tt[-s - 1];
+            const unsigned long long bits = a.xbits[c];
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const T t = __shfl_up(v, d, 64);
+                if (lane >= d && ((bits >> (lane - d + 1)) & ((1ull << d) - 1ull)) == 0ull) v = S::oplus(v, t);
+            }
+            if (lane == 63 || ((bits >> (lane + 1)) & 1ull) != 0ull) {
+                const int r = a.xrow[a.xord[c] - 1 + __popcll(bits & upto)];
+                if (!S::skip(v)) S::emit(Y + r, v);
+            }
+        }
+    }
+}
+
 template <class S>
 __device__ __forceinline__ void ph_gather_range(const SweepArgs& a, long r0, long r1, long tid, long nth) {
     if (a.flat) {
@@ -714,6 +767,10 @@ __global__ void k_emit(SweepArgs a, int k) {
 
 template <class S>
 __global__ void k_gather(SweepArgs a) {
+    if (a.xparts) {
+        ph_gather_parts<S>(a, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
+        return;
+    }
     long tid = xcd_tid();
     long nth = static_cast<long>(gridDim.x) * blockDim.x;
     ph_gather<S>(a, tid, nth);
@@ -782,7 +839,9 @@ __global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long ba
                         level_dst<S>(a, k), tid, nth);
         grid_barrier(gb, base + nb * static_cast<unsigned long long>(k + 1), deadline);
     }
-    if (a.gather) {
+    if (a.gather && a.xparts) {
+        ph_gather_parts<S>(a, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
+    } else if (a.gather) {
         ph_gather<S>(a, tid, nth);
     } else if (a.L == 0) {
         ph_emit<S>(a, a.term_emits, a.n_term, tp<S>(a.Tv), tid, nth, lane);

@@ -33,7 +33,7 @@ VARIANTS = {"levels": VARIANT_LEVELS, "graph": VARIANT_GRAPH, "persistent": VARI
 EMITS = {"spot": EMIT_SPOT, "gather": EMIT_GATHER}
 # Per-level node-update kernels (gmr.h gmr_eval_kind).
 EVAL_GROUP, EVAL_THREAD, EVAL_WAVE, EVAL_LDS = 0, 1, 2, 3
-RUN_TIMED, RUN_ROW_SEGMENTS = 1, 2
+RUN_TIMED, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST = 1, 2, 4
 EVAL_KINDS = {"group": EVAL_GROUP, "thread": EVAL_THREAD, "wave": EVAL_WAVE, "lds": EVAL_LDS}
 
 TROPICAL_INF = 1000000000
@@ -338,6 +338,33 @@ class Engine:
             Y = torch.empty((i["rows"], B) if X.dim() == 2 else (i["rows"],), dtype=dt, device=X.device)
         stream = torch.cuda.current_stream(X.device).cuda_stream
         self.run(semiring, X.data_ptr(), B, Y.data_ptr(), stream=stream, **kw)
+        return Y
+
+    def run_left(self, semiring, X_ptr, B, Y_ptr, stream=0, **kw):
+        """Y^T = X^T M on device pointers: X rows x B, Y cols x B (Resident
+        layout; variant levels or graph)."""
+        if isinstance(semiring, str):
+            semiring = SEMIRINGS[semiring]
+        o, keep = self._opts(**kw)
+        _check(_lib.dev().gmr_engine_run_left(self._h, semiring, C.c_void_p(X_ptr), B, C.c_void_p(Y_ptr),
+                                              C.byref(o), C.c_void_p(stream)))
+        del keep
+
+    def run_left_torch(self, semiring, X, Y=None, **kw):
+        """Left product for torch device tensors: X float32 (rows, B) or (rows,)."""
+        import torch
+        if isinstance(semiring, str):
+            semiring = SEMIRINGS[semiring]
+        assert X.is_cuda and X.dtype == torch.float32 and X.is_contiguous()
+        B = 1 if X.dim() == 1 else X.shape[1]
+        i = self.info()
+        if X.shape[0] != i["rows"]:
+            raise ValueError(f"X has {X.shape[0]} rows, the matrix has {i['rows']} rows")
+        dt = torch.float32 if semiring == PLUS_TIMES else torch.int32
+        if Y is None:
+            Y = torch.empty((i["cols"], B) if X.dim() == 2 else (i["cols"],), dtype=dt, device=X.device)
+        stream = torch.cuda.current_stream(X.device).cuda_stream
+        self.run_left(semiring, X.data_ptr(), B, Y.data_ptr(), stream=stream, **kw)
         return Y
 
     def __del__(self):

@@ -11,7 +11,8 @@ import hashlib
 import json
 import os
 
-from . import EMIT_GATHER, EMIT_SPOT, RESIDENT, RUN_ROW_SEGMENTS, VARIANT_GRAPH, VARIANT_LEVELS, VARIANT_PERSISTENT
+from . import (EMIT_GATHER, EMIT_SPOT, RESIDENT, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, VARIANT_GRAPH, VARIANT_LEVELS,
+               VARIANT_PERSISTENT)
 
 VARIANT_NAMES = {VARIANT_LEVELS: "levels", VARIANT_GRAPH: "graph", VARIANT_PERSISTENT: "persistent"}
 EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
@@ -20,7 +21,8 @@ EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
 def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), (1024, 0), (256, 256))):
     """The search space for one engine and batch width: variant x emission x
     vector tile x launch shape (x gather order for row-gather emission of
-    several vectors; x chunked or row-segment gather for one vector).
+    several vectors; x XCD-partitioned, single-list or row-segment gather for
+    one vector).
     Persistent shapes are (block, grid cap); a cap of 0 means the engine's
     default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
@@ -32,7 +34,7 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
                 shapes = persistent_shapes if v == VARIANT_PERSISTENT else [(b, 0) for b in blocks]
                 for blk, cap in shapes:
                     for order in ((0, 1) if e == EMIT_GATHER and B > 1 else (0,)):
-                        for fl in ((0, RUN_ROW_SEGMENTS) if e == EMIT_GATHER and B == 1 else (0,)):
+                        for fl in ((0, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST) if e == EMIT_GATHER and B == 1 else (0,)):
                             c = dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap,
                                      gather_order=order)
                             if fl:
@@ -54,7 +56,8 @@ def describe(c):
     cap = c.get("grid_cap", 0)
     return (f"{VARIANT_NAMES[c['variant']]}/{EMIT_NAMES[c['emit']]}/tile={c['b_tile'] or 'B'}/blk={c['block']}"
             + (f"/grid<={cap}" if cap else "") + ("/order=rule" if c.get("gather_order") == 1 else "")
-            + ("/segments" if c.get("flags", 0) & RUN_ROW_SEGMENTS else ""))
+            + ("/segments" if c.get("flags", 0) & RUN_ROW_SEGMENTS else "")
+            + ("/single-list" if c.get("flags", 0) & RUN_SINGLE_LIST else ""))
 
 
 def shape_key(info, B, semiring):

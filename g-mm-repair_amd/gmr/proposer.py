@@ -296,8 +296,9 @@ class Proposer:
     @classmethod
     def default(cls):
         """The network shipped with the package: trained on MI355X sweep
-        timings of haplotype, genotype and R-MAT grammars (gmr.train_proposer
-        --graphs) at B in {1, 3, 4, 16, 29, 64, 100, 256}."""
+        timings of haplotype and genotype grammars, including members generated
+        with weak-compression parameters (gmr.train_proposer --vary), at B in
+        {1, 3, 4, 16, 29, 64, 100, 256}."""
         return cls.load(DEFAULT_WEIGHTS)
 
 

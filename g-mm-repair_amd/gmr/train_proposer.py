@@ -34,12 +34,21 @@ def log(*a):
     print(f"[proposer {time.time() - T0:6.1f}s]", *a, file=sys.stderr, flush=True)
 
 
-def training_set(quick, graphs=False):
+def training_set(quick, graphs=False, vary=False):
     """Matrices of the haplotype (config 1) and genotype (config 2) families,
     the genotype ones at row counts from 64 to 500,000 so the set spans the
     shapes the decisions depend on (row-parallel work, level widths); with
-    `graphs`, also R-MAT graphs (the config 4 family) at several scales."""
+    `vary`, also members of those two families with their generator parameters
+    moved towards weak compression (dense, noisy haplotypes; genotypes with
+    short founder segments and many founders); with `graphs`, also R-MAT graphs
+    (the config 4 family) at several scales."""
     import gmr
+    if vary:
+        yield "hap4096x2048_noisy", gmr.Matrix.haplotype(31, 4096, 2048, density=0.2, flip=0.15)
+        yield "hap65536x1024_noisy", gmr.Matrix.haplotype(32, 65536, 1024, density=0.1, flip=0.1, seg_mean=16.0)
+        yield "hap262144x256_sparse", gmr.Matrix.haplotype(33, 262144, 256, density=0.05, flip=0.05, seg_mean=8.0)
+        yield "geno200000x10000_short", gmr.Matrix.genotype(34, 200000, 10000, founders=2048, seg_mean=8.0)
+        yield "geno20000x10000_short", gmr.Matrix.genotype(35, 20000, 10000, founders=1024, seg_mean=16.0)
     if graphs:
         for scale, edges, seed in ((12, 60000, 31), (15, 450000, 32), (18, 3500000, 33), (21, 28000000, 34)):
             yield f"rmat{scale}", gmr.Matrix.rmat(seed, scale, edges)
@@ -83,6 +92,8 @@ def main(argv=None):
     ap.add_argument("--nnz-hi", type=float, default=1e8)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--graphs", action="store_true", help="also train on R-MAT graphs (config 4 family)")
+    ap.add_argument("--vary", action="store_true",
+                    help="also train on haplotype/genotype matrices generated with weak-compression parameters")
     args = ap.parse_args(argv)
     import torch
 
@@ -93,7 +104,7 @@ def main(argv=None):
     train_B = [1, 3, 4, 16, 29, 64, 100, 256] if not args.quick else [1, 4, 16, 37, 64]
     Xs, Ts, Ms = [], [], []
     t_collect = time.time()
-    for name, m in training_set(args.quick, args.graphs):
+    for name, m in training_set(args.quick, args.graphs, args.vary):
         g = m.repair()
         s = g.schedule(m.rows, m.cols, "resident")
         arr = s.arrays()
@@ -170,7 +181,8 @@ def main(argv=None):
         fused[vname] = e0.elapsed_time(e1) / 3
     nnz_vec = sum(r["nnz"] * r["B"] for r in rows_out)
     report = dict(
-        training=dict(samples=int(len(X)), epochs=args.epochs, graphs=bool(args.graphs), loss_first=float(losses[0]),
+        training=dict(samples=int(len(X)), epochs=args.epochs, graphs=bool(args.graphs), vary=bool(args.vary),
+                      loss_first=float(losses[0]),
                       loss_last=float(losses[-1]), collect_s=t_collect, train_s=t_train, B=train_B),
         heldout=dict(n=len(rows_out), regret_mean=float(reg.mean()), regret_median=float(np.median(reg)),
                      regret_max=float(reg.max()), regret_of_totals=tot_mine / tot_ex - 1.0,

@@ -53,8 +53,11 @@ enum gmr_eval_kind {
 
 enum gmr_run_flags {
     GMR_RUN_TIMED = 1,        /* record device time (see gmr_engine_last_kernel_ms) */
-    GMR_RUN_ROW_SEGMENTS = 2  /* one vector, row gather: lanes per row segment instead of one lane
-                                 per top-sequence symbol (64-symbol chunks) */
+    /* One vector, row gather. By default the top-sequence symbols are split into 8 value
+     * ranges, one per XCD, each swept in 64-symbol chunks, so each XCD's L2 caches an eighth
+     * of the values (in a fused multi-grammar launch the single list below is used). */
+    GMR_RUN_ROW_SEGMENTS = 2, /* lanes per row segment instead */
+    GMR_RUN_SINGLE_LIST = 4   /* the whole top sequence as one list of 64-symbol chunks instead */
 };
 
 typedef struct gmr_run_opts {
@@ -93,6 +96,14 @@ int gmr_engine_run(gmr_engine* e, int semiring, const void* d_X, int B, void* d_
  * 0 = row order (default); 1 = segments sharing their highest rule together
  * (cache locality for the value rows they read). Results are unchanged. */
 int gmr_engine_set_gather_order(gmr_engine* e, int order);
+/* Left product Y^T = X^T M on `stream`: X is rows x B (X[r*B + b]), Y is
+ * cols x B (Y[c*B + b]); same semirings and leaves as gmr_engine_run, with the
+ * matrix value on the left. Resident layout only; opts->variant LEVELS or GRAPH
+ * (b_tile and block honoured). The first call builds the incoming-edge lists
+ * (synchronising the device). mm-repair computes this product beside Ax
+ * (reference run_benchmark.py:138-141); the reference's GPU engine has none. */
+int gmr_engine_run_left(gmr_engine* e, int semiring, const void* d_X, int B, void* d_Y,
+                        const gmr_run_opts* opts, void* stream);
 /* Persistent variant health check: *barrier_error = 1 if a grid-barrier wait
  * of an earlier persistent launch on this engine timed out (its results are
  * invalid; the barrier has been reset), else 0. Synchronises the device. */

@@ -22,6 +22,8 @@
  *   orc_cpu_layered_f32  orc_sweep on the same schedule (FP32 tolerance)
  *   orc_mlp_loss_grad    central finite differences of its own loss (the
  *                        proposer network has no reference counterpart)
+ *   orc_brute_force_left dense x^T M computed by numpy in the tests, and the
+ *                        right product of the transposed matrix (x^T M = M^T x)
  */
 #include <math.h>
 #include <omp.h>
@@ -63,6 +65,41 @@ int orc_brute_force(int semiring, int rows, int cols, const int* seq, long long 
         } else {
             int l = leaf_trop((float)V[v], x[c]);
             if (l < y[row]) y[row] = l;
+        }
+    }
+    return 0;
+}
+
+/* Left product Y^T = X^T M straight from the CSRV string: X[r*B + b] (rows x B),
+ * Y[c*B + b] (cols x B), y_c = (+)_r leaf(V, x_r) with the same leaves as the
+ * right product; den[c*B + b] = sum_r |M_rc x_r| for (+,x). mm-repair computes
+ * this product beside Ax (reference run_benchmark.py:138-141); its code is not
+ * in the reference, so this restates the definition. */
+int orc_brute_force_left(int semiring, int rows, int cols, const int* seq, long long len, const double* V,
+                         const float* X, int B, double* Y, double* den) {
+    for (long long i = 0; i < (long long)cols * B; ++i) {
+        Y[i] = semiring == 2 ? (double)ORC_INF : 0.0;
+        if (den) den[i] = 0.0;
+    }
+    int row = 0;
+    for (long long i = 0; i < len; ++i) {
+        int t = seq[i];
+        if (t == 0) { ++row; continue; }
+        if (row >= rows) return 1;
+        int v = (t - 1) / cols, c = (t - 1) % cols;
+        double* yc = Y + (long long)c * B;
+        const float* xr = X + (long long)row * B;
+        for (int b = 0; b < B; ++b) {
+            if (semiring == 0) {
+                double p = V[v] * (double)xr[b];
+                yc[b] += p;
+                if (den) den[(long long)c * B + b] += fabs(p);
+            } else if (semiring == 1) {
+                if (leaf_bool((float)V[v], xr[b])) yc[b] = 1.0;
+            } else {
+                int l = leaf_trop((float)V[v], xr[b]);
+                if (l < yc[b]) yc[b] = l;
+            }
         }
     }
     return 0;

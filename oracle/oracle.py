@@ -72,6 +72,23 @@ def brute_force_batched(semiring, rows, cols, seq, V, X):
     return Y, den
 
 
+def brute_force_left(semiring, rows, cols, seq, V, X):
+    """X float32 (rows,) or (rows, B) -> (Y float64 (cols[, B]), den): Y^T = X^T M."""
+    seq = np.ascontiguousarray(seq, np.int32)
+    V = np.ascontiguousarray(V, np.float64)
+    X = np.ascontiguousarray(X, np.float32)
+    B = 1 if X.ndim == 1 else X.shape[1]
+    shape = (cols,) if X.ndim == 1 else (cols, B)
+    Y = np.empty(shape, np.float64)
+    den = np.empty(shape, np.float64)
+    rc = lib().orc_brute_force_left(C.c_int(semiring), C.c_int(rows), C.c_int(cols), _p(seq, C.c_int),
+                                    C.c_longlong(len(seq)), _p(V, C.c_double), _p(X, C.c_float), C.c_int(B),
+                                    _p(Y, C.c_double), _p(den, C.c_double))
+    if rc:
+        raise ValueError("CSRV has more rows than `rows`")
+    return Y, den
+
+
 def expand(g):
     """g: dict with alpha, kind, left, right, rep, C -> CSRV int32."""
     f = lib().orc_expand

@@ -17,6 +17,8 @@ def rel_err(y, y_star, den):
     match exactly (it contributes inf otherwise)."""
     y = np.asarray(y, np.float64)
     y_star = np.asarray(y_star, np.float64)
+    if y.shape != y_star.shape:
+        raise ValueError(f"shapes differ: {y.shape} vs {y_star.shape}")
     den = np.asarray(den, np.float64)
     d = np.abs(y - y_star)
     with np.errstate(divide="ignore", invalid="ignore"):

@@ -41,6 +41,7 @@ def run_engine(eng, sr, X, **kw):
 
 
 def check(sr, y, y_star, den, what):
+    assert np.shape(y) == np.shape(y_star), f"{what}: shape {np.shape(y)} != {np.shape(y_star)}"
     if sr == 0:
         e = rel_err(y, y_star, den)
         _say(f"  {what}: err={e:.3g}")
@@ -402,8 +403,8 @@ def test_gather_orders_agree(hap):
 
 @pytest.mark.parametrize("sr", [0, 1, 2])
 def test_single_vector_gathers_agree(sr):
-    """One vector, row gather: the chunked kernel (a wave per 64 top-sequence
-    symbols, segmented scan, atomics only for rows crossing a chunk) and the
+    """One vector, row gather: the XCD-partitioned and single-list chunked
+    kernels (a wave per 64 top-sequence symbols, segmented scan) and the
     row-segment kernel, on a matrix whose rows are empty, 1 symbol, exactly 64,
     and long enough to cross several chunks (and split into several segments),
     with a symbol count that is not a multiple of 64."""
@@ -419,8 +420,27 @@ def test_single_vector_gathers_agree(sr):
     x = x_for(sr, cols, 5 + sr)
     y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
     eng = g.schedule(rows, cols, "resident").engine()
-    for flags in (0, gmr.RUN_ROW_SEGMENTS, 0):
+    for flags in (0, gmr.RUN_ROW_SEGMENTS, gmr.RUN_SINGLE_LIST, 0):
         for var in ("levels", "graph", "persistent"):
             _say("one-vector gather", sr, flags, var)
             y = run_engine(eng, sr, x, variant=gmr.VARIANTS[var], emit=gmr.EMIT_GATHER, flags=flags)
             check(sr, y, y_star, den, f"gather flags={flags} {var}")
+    # grid shapes the partitioned gather must handle: fewer than 8 workgroups
+    # (each takes several lists) and counts that are not multiples of 8
+    for cap in (1, 3, 12):
+        _say("xcd-partitioned gather, persistent grid cap", cap)
+        y = run_engine(eng, sr, x, variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_GATHER, grid_cap=cap, block=64)
+        check(sr, y, y_star, den, f"xcd parts grid<={cap}")
+    # in a fused batch the partitioned gather falls back to the chunked one
+    xt = torch.from_numpy(x).cuda()[:, None]
+    eng2 = g.schedule(rows, cols, "resident").engine()
+    batch = gmr.Batch([eng, eng2])
+    for var in ("persistent", "graph", "levels"):
+        _say("batch with xcd-partitioned gather", sr, var)
+        Ys = batch.run_torch(sr, [xt, xt], variant=gmr.VARIANTS[var],
+                             opts=[{"emit": gmr.EMIT_GATHER}] * 2)
+        torch.cuda.synchronize()
+        if var == "persistent":
+            batch.check()
+        for Yb in Ys:
+            check(sr, Yb.cpu().numpy()[:, 0], y_star, den, f"batch {var} xcd parts")

@@ -86,6 +86,35 @@ def test_brute_force_against_dense_products(seed):
     np.testing.assert_allclose(Y, M @ X.astype(float), rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_left_brute_force_against_dense_products(seed):
+    """x^T M from the CSRV string equals numpy's dense product, and equals the
+    right product of the transposed matrix (a dropped or swapped index fails
+    one of them)."""
+    rng = np.random.default_rng(100 + seed)
+    M, vals, seq = _random_csrv(rng, 29, 41, 4, 0.25)
+    rows, cols = M.shape
+    X = x_for(0, rows, seed, B=3)
+    Y, den = O.brute_force_left(0, rows, cols, seq, vals, X)
+    np.testing.assert_allclose(Y, M.T @ X.astype(float), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(den, np.abs(M.T) @ np.abs(X.astype(float)), rtol=1e-12)
+    # the same through the right product of M^T
+    MT, valsT, seqT = M.T.copy(), vals, []
+    for r in range(cols):
+        for c in np.nonzero(MT[r])[0]:
+            seqT.append(1 + int(np.searchsorted(valsT, MT[r, c])) * rows + c)
+        seqT.append(0)
+    YT, _ = O.brute_force_batched(0, cols, rows, np.array(seqT, np.int32), valsT, X)
+    np.testing.assert_allclose(Y, YT, rtol=1e-12, atol=1e-12)
+    xb = x_for(1, rows, seed)
+    yb, _ = O.brute_force_left(1, rows, cols, seq, vals, xb)
+    np.testing.assert_array_equal(yb, ((M != 0) & (xb[:, None] != 0)).any(0).astype(float))
+    xt = x_for(2, rows, seed)
+    yt, _ = O.brute_force_left(2, rows, cols, seq, vals, xt)
+    cand = np.where(M != 0, np.where(xt[:, None] >= 1e9, 1e9, M + xt[:, None]), 1e9)
+    np.testing.assert_array_equal(yt, cand.min(0))
+
+
 def test_rel_err_flags_a_planted_mistake():
     rng = np.random.default_rng(3)
     M, vals, seq = _random_csrv(rng, 20, 10, 3, 0.5)
