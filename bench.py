@@ -79,6 +79,10 @@ def main():
                          "rmat: 2^23-vertex graph, Boolean and min/plus (own JSON line)")
     ap.add_argument("--sweep-B", default=None,
                     help="comma-separated vector counts: throughput per B on this workload (own JSON line)")
+    ap.add_argument("--left-B", default=None,
+                    help="comma-separated vector counts: the left product X^T M per B on this workload (own JSON line)")
+    ap.add_argument("--traverse", action="store_true",
+                    help="rmat workload: also run the fixpoint traversal (reachability / shortest paths)")
     ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--cols", type=int, default=None)
     ap.add_argument("--seed", type=int, default=None)
@@ -149,6 +153,8 @@ def main():
 
     if args.sweep_B:
         return run_sweep_B(args, m, g, s_res, eng)
+    if args.left_B:
+        return run_left_B(args, m, s_res, eng)
 
     gen = torch.Generator(device="cuda").manual_seed(args.seed)
     X = torch.randn(args.cols, B, device="cuda", generator=gen, dtype=torch.float32)
@@ -406,6 +412,54 @@ def run_sweep_B(args, m, g, s_res, eng):
                       "per_B": out}), flush=True)
 
 
+def run_left_B(args, m, s_res, eng):
+    """The left product Y^T = X^T M per vector count B (levels and graph
+    variants, the faster kept), with the FP32 error of two columns against the
+    FP64 oracle's left brute force."""
+    import numpy as np
+    import torch
+
+    import gmr
+    from oracle import oracle as O
+    seq, V = m.arrays()
+    nnz = m.nnz()
+    out = []
+    for B in [int(b) for b in args.left_B.split(",")]:
+        gen = torch.Generator(device="cuda").manual_seed(args.seed + 7 * B)
+        X = torch.randn(args.rows, B, device="cuda", generator=gen)
+        if B == 1:
+            X = X[:, 0].contiguous()
+        best = None
+        for vname in ("levels", "graph"):
+            kw = dict(variant=gmr.VARIANTS[vname])
+            Y = eng.run_left_torch(0, X, **kw)
+            for _ in range(args.warmup):
+                eng.run_left_torch(0, X, Y, **kw)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(args.steps):
+                eng.run_left_torch(0, X, Y, **kw)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.steps
+            log(f"left B={B} {vname}: {ms:.3f} ms")
+            if best is None or ms < best[0]:
+                best = (ms, vname, Y)
+        ms, vname, Y = best
+        cols = [0] if B == 1 else [0, B - 1]
+        Xs = X.reshape(args.rows, -1)[:, cols].contiguous().cpu().numpy()
+        Ys, den = O.brute_force_left(0, args.rows, args.cols, seq, V, Xs)
+        Yg = Y.reshape(args.cols, -1)[:, cols].cpu().numpy().astype(np.float64)
+        d = np.abs(Yg - Ys)
+        err = float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1), np.where(d == 0, 0, np.inf))))
+        r = {"B": B, "ms": ms, "nnz_vec_per_s": nnz * B / (ms * 1e-3), "variant": vname, "fp32_err": err}
+        log("left", r)
+        out.append(r)
+    print(json.dumps({"metric": "left product Y^T = X^T M, nnz*vectors/s", "workload": f"{args.workload}"
+                      f"{args.rows}x{args.cols}", "nnz": nnz, "per_B": out}), flush=True)
+
+
 def run_rmat(args):
     """R-MAT graph on 2^23 vertices from 2^27 edge draws (a, b, c, d) = (.57, .19,
     .19, .05), duplicates removed: (a) Boolean one-step reachability from a 1%
@@ -446,8 +500,35 @@ def run_rmat(args):
         plain_ms = _time_steps(eng_ref, sr, xt, y, dict(variant=gmr.VARIANT_LEVELS, emit=gmr.EMIT_SPOT),
                                args.steps, args.warmup)
         exact_plain = bool(np.array_equal(y.cpu().numpy().astype(np.int64), y_star.astype(np.int64)))
+        # fixpoint traversal from the same start (Boolean: the reachable set;
+        # min/plus: shortest distances), checked against the oracle's left
+        # product iterated on the CPU
+        trav = None
+        if args.traverse:
+            xs = torch.from_numpy(x).cuda()
+            eng.traverse_torch(sr, xs.clone(), max_iters=2)  # builds the incoming lists
+            torch.cuda.synchronize()
+            t0 = time.time()
+            iters = eng.traverse_torch(sr, xs)
+            torch.cuda.synchronize()
+            t_ms = (time.time() - t0) * 1e3
+            got = xs.cpu().numpy()
+            if sr == gmr.TROPICAL:
+                got = np.minimum(got, np.float32(1e9))  # distances >= 1e9 mean unreached
+            xc = x.copy()
+            for it in range(1, 10000):
+                yc, _ = O.brute_force_left(sr, n, n, seq, V, xc)
+                xn = (np.maximum(xc, yc) if sr == gmr.BOOLEAN else np.minimum(np.minimum(xc, 1e9), yc)).astype(np.float32)
+                if np.array_equal(xn, np.minimum(xc, 1e9) if sr == gmr.TROPICAL else xc):
+                    break
+                xc = xn
+            want = np.minimum(xc, 1e9) if sr == gmr.TROPICAL else xc
+            trav = {"products": iters, "ms": t_ms, "ms_per_product": t_ms / iters, "cpu_products": it,
+                    "bit_exact": bool(np.array_equal(got, want)), "moved": int((got != np.minimum(x, 1e9)).sum())}
+            log(tag, "traversal", trav)
         info = s_res.info()
         res[tag] = {"nnz": nnz, "ms": ms, "nnz_per_s": nnz / (ms * 1e-3), "schedule": autotune.describe(best),
+                    "traversal": trav,
                     "bit_exact": exact, "plain_ms": plain_ms, "plain_nnz_per_s": nnz / (plain_ms * 1e-3),
                     "plain_bit_exact": exact_plain,
                     "grammar": {"rules": info["num_raw_nt"], "depth": info["max_depth"], "C": g.info()["clen"],

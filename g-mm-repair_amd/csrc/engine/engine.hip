@@ -70,7 +70,10 @@ Engine::~Engine() {
             if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
         }
     for (void* p : {static_cast<void*>(d_in_off), static_cast<void*>(d_in_src), static_cast<void*>(d_in_coeff),
-                    static_cast<void*>(d_col_off), static_cast<void*>(d_col_term), ws_A})
+                    static_cast<void*>(d_col_off), static_cast<void*>(d_col_term), ws_A, ws_Xs,
+                    static_cast<void*>(d_lseg), static_cast<void*>(d_lsplit), static_cast<void*>(d_lpsrc),
+                    static_cast<void*>(d_lpoff), static_cast<void*>(d_lbits), static_cast<void*>(d_lord),
+                    static_cast<void*>(d_lrow), ws_trav})
         if (p) (void)hipFree(p);
     void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_fbits, d_ford,
                     d_frow, d_xsym, d_xbits, d_xord, d_xrow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1,
@@ -748,6 +751,68 @@ void Engine::build_left() {
         std::vector<int> cf2(coff.begin(), coff.end() - 1);
         for (int t = 1; t < U; ++t) cterm[cf2[tcol[t]]++] = t;
     }
+    if (!has_runlen) {
+        // Row-gather form. Groups: levels 1..L, then the terminals; node order
+        // inside a group (nodes of a level are contiguous slots).
+        std::vector<long> gnode(L + 2);
+        for (int k = 0; k <= L; ++k) gnode[k] = level_off[k];
+        gnode[L + 1] = nodes;
+        // B > 1: segments of at most kGatherSeg entries per node.
+        std::vector<int4> seg;
+        std::vector<int> split;
+        lseg_off.assign(L + 2, 0);
+        for (int gi = 0; gi <= L; ++gi) {
+            lseg_off[gi] = static_cast<int>(seg.size());
+            for (long n = gnode[gi]; n < gnode[gi + 1]; ++n) {
+                const int a0 = static_cast<int>(off[n]), a1 = static_cast<int>(off[n + 1]);
+                const int sp = (a1 - a0) > kGatherSeg ? 1 : 0;
+                if (sp) split.push_back(static_cast<int>(n));
+                if (a1 == a0) seg.push_back(make_int4(static_cast<int>(n), a0, a1, 0));
+                for (int q = a0; q < a1; q += kGatherSeg)
+                    seg.push_back(make_int4(static_cast<int>(n), q, std::min(a1, q + kGatherSeg), sp));
+            }
+        }
+        lseg_off[L + 1] = static_cast<int>(seg.size());
+        // One vector: every group's entries padded to whole 64-entry chunks
+        // with the zero row of Xs (row `rows`), row-start bitmap per chunk.
+        const int pad = -rows - 1;
+        std::vector<int> psrc, poff(static_cast<std::size_t>(nodes) + 1, 0), prow, pord;
+        std::vector<unsigned long long> pbits;
+        lchunk_off.assign(L + 2, 0);
+        for (int gi = 0; gi <= L; ++gi) {
+            lchunk_off[gi] = static_cast<int>(psrc.size() / 64);
+            for (long n = gnode[gi]; n < gnode[gi + 1]; ++n) {
+                poff[n] = static_cast<int>(psrc.size());
+                if (off[n + 1] > off[n]) {
+                    const std::size_t p0 = psrc.size();
+                    if ((p0 >> 6) >= pbits.size()) pbits.resize((p0 >> 6) + 1, 0ull);
+                    pbits[p0 >> 6] |= 1ull << (p0 & 63);
+                    prow.push_back(static_cast<int>(n));
+                }
+                psrc.insert(psrc.end(), src.begin() + off[n], src.begin() + off[n + 1]);
+            }
+            while (psrc.size() % 64) psrc.push_back(pad);
+        }
+        poff[nodes] = static_cast<int>(psrc.size());
+        lchunk_off[L + 1] = static_cast<int>(psrc.size() / 64);
+        pbits.resize(psrc.size() / 64, 0ull);
+        pord.resize(pbits.size());
+        int before = 0;
+        for (std::size_t c = 0; c < pbits.size(); ++c) {
+            pord[c] = before;
+            before += __builtin_popcountll(pbits[c]);
+        }
+        if (psrc.size() > 0x7fffffffUL) throw std::invalid_argument("left product: too many entries");
+        d_lseg = dev_upload(seg.data(), seg.size(), &struct_bytes);
+        d_lsplit = dev_upload(split.data(), split.size(), &struct_bytes);
+        n_lsplit = static_cast<int>(split.size());
+        d_lpsrc = dev_upload(psrc.data(), psrc.size(), &struct_bytes);
+        d_lpoff = dev_upload(poff.data(), poff.size(), &struct_bytes);
+        d_lbits = dev_upload(pbits.data(), pbits.size(), &struct_bytes);
+        d_lord = dev_upload(pord.data(), pord.size(), &struct_bytes);
+        d_lrow = dev_upload(prow.data(), prow.size(), &struct_bytes);
+        n_lpsrc = static_cast<int>(psrc.size());
+    }
     d_in_off = dev_upload(off32.data(), off32.size(), &struct_bytes);
     d_in_src = dev_upload(src.data(), src.size(), &struct_bytes);
     d_in_coeff = has_runlen ? dev_upload(cf.data(), cf.size(), &struct_bytes) : nullptr;
@@ -758,17 +823,79 @@ void Engine::build_left() {
 
 namespace {
 
+// The row-gather arguments for one group of nodes: nodes play rows, incoming
+// entries play top-sequence symbols, A plays both the rule values and Y (row
+// stride Bs), Xs the terminal rows.
+SweepArgs left_gather_args(Engine& en, int Bt, int Bs) {
+    SweepArgs g{};
+    g.W0 = en.ws_A;
+    g.Tv = en.ws_Xs;
+    g.Y = en.ws_A;
+    g.B = Bs;
+    g.Bt = Bt;
+    g.Bs = Bs;
+    g.vec = Bt > 1 ? 1 : 0;
+    g.yvec = (Bs % 4 == 0) ? 1 : 0;
+    g.G = dev::lanes_for(Bs, g.vec);
+    g.gather = 1;
+    g.resident = 1;
+    g.rows = static_cast<int>(std::min<long>(static_cast<long>(en.NL) + en.U, 0x7fffffffL));
+    if (Bt == 1) {
+        g.flat = 1;
+        g.csym = en.d_lpsrc;
+        g.crow = en.d_lpoff;
+        g.fbits = en.d_lbits;
+        g.ford = en.d_lord;
+        g.frow = en.d_lrow;
+        g.n_csym = en.n_lpsrc;
+        g.n_fchunk = en.n_lpsrc / 64;
+    } else {
+        g.csym = en.d_in_src;
+        g.gseg = en.d_lseg;
+        g.n_gseg = en.lseg_off.back();
+        g.gsplit = en.n_lsplit > 0 ? 1 : 0;
+    }
+    return g;
+}
+
 template <class S>
 void launch_left(Engine& en, const dev::LeftArgs& a, int block, hipStream_t st) {
-    for (int k = en.L; k >= 1; --k) {
-        const int n0 = en.level_off[k - 1], n1 = en.level_off[k];
-        if (n1 > n0)
-            hipLaunchKernelGGL(dev::k_left_nodes<S>, dim3(grid_for(static_cast<long>(n1 - n0) * a.G, block)),
-                               dim3(block), 0, st, a, n0, n1);
+    const bool fast = !en.has_runlen;
+    if (fast) {
+        const long zero_all = a.Bt == 1 ? (static_cast<long>(en.NL) + en.U) * a.Bs : 0;
+        const long work = std::max<long>(static_cast<long>(en.rows + 1) * a.Bs,
+                                         zero_all ? zero_all : static_cast<long>(en.n_lsplit) * a.Bs);
+        hipLaunchKernelGGL(dev::k_left_init<S>, dim3(grid_for(work, block, 4096)), dim3(block), 0, st, a.X, a.B, a.Bt,
+                           a.Bs, en.rows, en.ws_Xs, en.ws_A, en.d_lsplit, static_cast<long>(en.n_lsplit), zero_all);
+        const SweepArgs g = left_gather_args(en, a.Bt, a.Bs);
+        // group gi = level gi + 1 for gi < L, the terminals for gi = L
+        auto group = [&](int gi) {
+            long r0, r1, threads;
+            if (g.flat) {
+                r0 = en.lchunk_off[gi];
+                r1 = en.lchunk_off[gi + 1];
+                threads = (r1 - r0) * 64;
+            } else {
+                r0 = en.lseg_off[gi];
+                r1 = en.lseg_off[gi + 1];
+                threads = (r1 - r0) * g.G;
+            }
+            if (r1 > r0)
+                hipLaunchKernelGGL(dev::k_gather_range<S>, dim3(grid_for(threads, block)), dim3(block), 0, st, g, r0, r1);
+        };
+        for (int k = en.L; k >= 1; --k) group(k - 1);
+        group(en.L);
+    } else {
+        for (int k = en.L; k >= 1; --k) {
+            const int n0 = en.level_off[k - 1], n1 = en.level_off[k];
+            if (n1 > n0)
+                hipLaunchKernelGGL(dev::k_left_nodes<S>, dim3(grid_for(static_cast<long>(n1 - n0) * a.G, block)),
+                                   dim3(block), 0, st, a, n0, n1);
+        }
+        if (en.U > 0)
+            hipLaunchKernelGGL(dev::k_left_nodes<S>, dim3(grid_for(static_cast<long>(en.U) * a.G, block)), dim3(block),
+                               0, st, a, en.NL, en.NL + en.U);
     }
-    if (en.U > 0)
-        hipLaunchKernelGGL(dev::k_left_nodes<S>, dim3(grid_for(static_cast<long>(en.U) * a.G, block)), dim3(block), 0,
-                           st, a, en.NL, en.NL + en.U);
     if (en.cols > 0)
         hipLaunchKernelGGL(dev::k_left_cols<S>, dim3(grid_for(static_cast<long>(en.cols) * a.G, block)), dim3(block),
                            0, st, a, en.cols);
@@ -779,19 +906,29 @@ template <class S>
 void run_left_t(Engine& en, int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st) {
     using T = typename S::T;
     const int Bt = (o.b_tile > 0 && o.b_tile < B) ? o.b_tile : B;
+    const int Bs = en.has_runlen ? Bt : padded_width(Bt);
     const int block = o.block > 0 ? o.block : 256;
-    const std::size_t need = static_cast<std::size_t>(std::max<long>(static_cast<long>(en.NL) + en.U, 1)) * Bt * sizeof(T);
-    if (need > en.ws_A_bytes) {
+    const std::size_t need = static_cast<std::size_t>(std::max<long>(static_cast<long>(en.NL) + en.U, 1)) * Bs * sizeof(T);
+    const std::size_t need_x = static_cast<std::size_t>(en.rows + 1) * Bs * sizeof(T);
+    if (need > en.ws_A_bytes || need_x > en.ws_Xs_bytes) {
         for (auto& kv : en.left_graphs) {
             if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
             if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
         }
         en.left_graphs.clear();
         GMR_HIP(hipDeviceSynchronize());
+    }
+    if (need > en.ws_A_bytes) {
         if (en.ws_A) GMR_HIP(hipFree(en.ws_A));
         en.ws_A = nullptr;
         GMR_HIP(hipMalloc(&en.ws_A, need));
         en.ws_A_bytes = need;
+    }
+    if (need_x > en.ws_Xs_bytes) {
+        if (en.ws_Xs) GMR_HIP(hipFree(en.ws_Xs));
+        en.ws_Xs = nullptr;
+        GMR_HIP(hipMalloc(&en.ws_Xs, need_x));
+        en.ws_Xs_bytes = need_x;
     }
     for (int b0 = 0; b0 < B; b0 += Bt) {
         dev::LeftArgs a{};
@@ -807,7 +944,7 @@ void run_left_t(Engine& en, int semiring, const float* X, int B, void* Y, const 
         a.NL = en.NL;
         a.B = B;
         a.Bt = std::min(Bt, B - b0);
-        a.Bs = a.Bt;
+        a.Bs = en.has_runlen ? a.Bt : padded_width(a.Bt);
         a.G = dev::lanes_for(a.Bt, 0);
         if (o.variant == GMR_VARIANT_GRAPH) {
             auto key = std::make_tuple(semiring, reinterpret_cast<uintptr_t>(a.X), reinterpret_cast<uintptr_t>(a.Y), a.B,
@@ -851,6 +988,46 @@ void Engine::run_left(int semiring, const float* X, int B, void* Y, const gmr_ru
         case GMR_TROPICAL: run_left_t<dev::TropicalOp>(*this, semiring, X, B, Y, o, st); break;
         default: throw std::invalid_argument("unknown semiring");
     }
+}
+
+// Fixpoint traversal over a square matrix: x <- x (+) x^T M until nothing
+// moves (Boolean: the set reachable from x along row -> column edges;
+// Tropical: shortest distances from the x distances, Bellman-Ford style).
+// The change flag is read back after every product.
+int Engine::traverse(int semiring, float* x, int max_iters, const gmr_run_opts& o, hipStream_t st) {
+    if (rows != cols) throw std::invalid_argument("traversal needs a square matrix");
+    if (semiring != GMR_BOOLEAN && semiring != GMR_TROPICAL)
+        throw std::invalid_argument("traversal runs in the Boolean or Tropical semiring");
+    if (!x) throw std::invalid_argument("x is NULL");
+    if (max_iters < 1) throw std::invalid_argument("max_iters must be >= 1");
+    const std::size_t need = static_cast<std::size_t>(std::max(rows, 1)) * sizeof(int) + 256;
+    if (need > ws_trav_bytes) {
+        GMR_HIP(hipStreamSynchronize(st));
+        if (ws_trav) GMR_HIP(hipFree(ws_trav));
+        ws_trav = nullptr;
+        GMR_HIP(hipMalloc(&ws_trav, need));
+        ws_trav_bytes = need;
+    }
+    void* y = ws_trav;
+    int* flag = reinterpret_cast<int*>(static_cast<char*>(ws_trav) + (need - 256));
+    const int block = 256;
+    const unsigned grid = grid_for(std::max(rows, 1), block);
+    int it = 0;
+    for (; it < max_iters;) {
+        run_left(semiring, x, 1, y, o, st);
+        hipLaunchKernelGGL(dev::k_clear_flag, dim3(1), dim3(64), 0, st, flag);
+        if (semiring == GMR_BOOLEAN)
+            hipLaunchKernelGGL(dev::k_relax<dev::BooleanOp>, dim3(grid), dim3(block), 0, st, x, y, rows, flag);
+        else
+            hipLaunchKernelGGL(dev::k_relax<dev::TropicalOp>, dim3(grid), dim3(block), 0, st, x, y, rows, flag);
+        GMR_HIP(hipGetLastError());
+        int h = 0;
+        GMR_HIP(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        GMR_HIP(hipStreamSynchronize(st));
+        ++it;
+        if (!h) break;
+    }
+    return it;
 }
 
 // ---------------------------------------------------------------- batches
@@ -1174,6 +1351,17 @@ int gmr_engine_run_left(gmr_engine* h, int semiring, const void* d_X, int B, voi
     if (opts) o = *opts;
     return dev_guarded([&] {
         h->e.run_left(semiring, static_cast<const float*>(d_X), B, d_Y, o, static_cast<hipStream_t>(stream));
+    });
+}
+
+int gmr_engine_traverse(gmr_engine* h, int semiring, void* d_x, int max_iters, int* iters_out,
+                        const gmr_run_opts* opts, void* stream) {
+    if (!h) return dev_fail(GMR_ERR_INVALID, "engine is NULL");
+    gmr_run_opts o{};
+    if (opts) o = *opts;
+    return dev_guarded([&] {
+        const int n = h->e.traverse(semiring, static_cast<float*>(d_x), max_iters, o, static_cast<hipStream_t>(stream));
+        if (iters_out) *iters_out = n;
     });
 }
 

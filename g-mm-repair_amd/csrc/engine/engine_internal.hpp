@@ -83,8 +83,22 @@ struct Engine {
     float* d_in_coeff = nullptr;     // per entry, only with run-length rules
     int* d_col_off = nullptr;        // cols + 1
     int* d_col_term = nullptr;
-    void* ws_A = nullptr;            // nodes x Bt values
+    void* ws_A = nullptr;            // nodes x Bs values
     std::size_t ws_A_bytes = 0;
+    void* ws_Xs = nullptr;           // (rows + 1) x Bs: x as semiring values, then a zero row
+    std::size_t ws_Xs_bytes = 0;
+    // the same incoming lists in row-gather form (grammars without run-length rules)
+    int4* d_lseg = nullptr;          // (node, first, end, split) segments, node order
+    int* d_lsplit = nullptr;         // nodes cut into several segments
+    int n_lsplit = 0;
+    int* d_lpsrc = nullptr;          // entries, each level's run padded to 64 (single vector)
+    int* d_lpoff = nullptr;          // nodes + 1 offsets into d_lpsrc
+    unsigned long long* d_lbits = nullptr;
+    int* d_lord = nullptr;
+    int* d_lrow = nullptr;
+    int n_lpsrc = 0;
+    std::vector<int> lseg_off;       // groups (levels 1..L, then terminals) -> segment ranges
+    std::vector<int> lchunk_off;     // groups -> 64-entry chunk ranges
     std::map<GraphKey, Graph> left_graphs;
 
     Engine() = default;
@@ -105,6 +119,10 @@ struct Engine {
                      hipStream_t st);
     // Y^T = X^T M: X rows x B, Y cols x B (Resident layout).
     void run_left(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st);
+    // Fixpoint x <- x (+) x^T M (square matrix); returns the products computed.
+    int traverse(int semiring, float* x, int max_iters, const gmr_run_opts& o, hipStream_t st);
+    void* ws_trav = nullptr;         // rows values (the product) + a change flag
+    std::size_t ws_trav_bytes = 0;
     void build_left();
 };
 

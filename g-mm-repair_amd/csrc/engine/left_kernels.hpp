@@ -7,11 +7,15 @@
 //   A[s] = (+)_{occurrences of s in row r of the top sequence} x_r
 //        (+) (+)_{rules p with s as a child} coeff(p -> s) * A[p]
 // -- and every parent of a rule sits on a higher level, so levels L..1 run in
-// order, then the terminals, with no atomics. A column then collects its
-// terminals: y_c = (+)_{terminals t in column c} leaf(V_t, A[t]).
+// order, then the terminals. A column then collects its terminals:
+// y_c = (+)_{terminals t in column c} leaf(V_t, A[t]).
 // Incoming lists are CSR over nodes (rule slots, then terminals); an entry
-// >= 0 is a parent slot, < 0 is a row -e-1. Vectors are innermost as in the
-// right product: X[r*B + b], A[node*Bs + b], Y[c*B + b].
+// >= 0 is a parent slot, < 0 is a row -e-1. That is the shape of the right
+// product's row gather (nodes for rows, entries for top-sequence symbols, x as
+// a semiring value for the terminal rows), so without run-length rules the
+// levels run on the row-gather kernels (k_gather_range); with them, on
+// k_left_nodes below, which applies the coefficients. Vectors are innermost
+// as in the right product: X[r*B + b], A[node*Bs + b], Y[c*B + b].
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -56,8 +60,8 @@ struct LeftArgs {
     int B, Bt, Bs, G;
 };
 
-// Nodes [n0, n1): a group of G lanes per node, lanes over the vectors (four
-// per lane when Bs is a multiple of 4), incoming entries two at a time.
+// Nodes [n0, n1) with coefficients: a group of G lanes per node, lanes over
+// the vectors, entries one after another.
 template <class S>
 __device__ void ph_left_nodes(const LeftArgs& a, int n0, int n1, long gid, long ng, int lane) {
     using T = typename S::T;
@@ -101,6 +105,32 @@ __device__ void ph_left_cols(const LeftArgs& a, int cols, long gid, long ng, int
     }
 }
 
+// Before a sweep without run-length rules (which runs on the row-gather
+// kernels): Xs[r*Bs + b] = x as a semiring value for b < Bt, the zero elsewhere
+// and on the extra row `rows` (the target of padding entries); then the node
+// rows that are combined atomically are zeroed -- every node (one vector) or
+// the listed split nodes.
+template <class S>
+__global__ void k_left_init(const float* X, int B, int Bt, int Bs, int rows, void* Xs_, void* A_, const int* zero_nodes,
+                            long n_zero, long zero_all) {
+    using T = typename S::T;
+    T* Xs = tp<S>(Xs_);
+    T* A = tp<S>(A_);
+    const long tid = xcd_tid();
+    const long nth = static_cast<long>(gridDim.x) * blockDim.x;
+    const long nx = static_cast<long>(rows + 1) * Bs;
+    for (long i = tid; i < nx; i += nth) {
+        const long r = i / Bs;
+        const int b = static_cast<int>(i - r * Bs);
+        Xs[i] = (r < rows && b < Bt) ? LeftOps<S>::from_x(X[r * B + b]) : S::zero();
+    }
+    if (zero_all) {
+        for (long i = tid; i < zero_all; i += nth) A[i] = S::zero();
+    } else {
+        for (long i = tid; i < n_zero * Bs; i += nth) A[static_cast<long>(zero_nodes[i / Bs]) * Bs + i % Bs] = S::zero();
+    }
+}
+
 template <class S>
 __global__ void k_left_nodes(LeftArgs a, int n0, int n1) {
     const long tid = xcd_tid();
@@ -113,6 +143,30 @@ __global__ void k_left_cols(LeftArgs a, int cols) {
     const long tid = xcd_tid();
     const long nth = static_cast<long>(gridDim.x) * blockDim.x;
     ph_left_cols<S>(a, cols, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+}
+
+// One relaxation step of a fixpoint traversal: x <- x (+) y for every vertex;
+// *changed is set (plain stores of 1) when some x moved.
+template <class S>
+__global__ void k_relax(float* x, const void* y_, int n, int* changed) {
+    using T = typename S::T;
+    const T* y = static_cast<const T*>(y_);
+    const long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long nth = static_cast<long>(gridDim.x) * blockDim.x;
+    bool moved = false;
+    for (long i = tid; i < n; i += nth) {
+        const T old = LeftOps<S>::from_x(x[i]);
+        const T nv = S::oplus(old, y[i]);
+        if (nv != old) {
+            x[i] = static_cast<float>(nv);
+            moved = true;
+        }
+    }
+    if (moved) *changed = 1;
+}
+
+static __global__ void k_clear_flag(int* flag) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;
 }
 
 }  // namespace dev

@@ -765,6 +765,14 @@ __global__ void k_emit(SweepArgs a, int k) {
     }
 }
 
+// Items [r0, r1) of the gather phase (the left product runs it level by level).
+template <class S>
+__global__ void k_gather_range(SweepArgs a, long r0, long r1) {
+    const long tid = xcd_tid();
+    const long nth = static_cast<long>(gridDim.x) * blockDim.x;
+    ph_gather_range<S>(a, r0, r1, tid, nth);
+}
+
 template <class S>
 __global__ void k_gather(SweepArgs a) {
     if (a.xparts) {

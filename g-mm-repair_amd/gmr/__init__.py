@@ -350,6 +350,21 @@ class Engine:
                                               C.byref(o), C.c_void_p(stream)))
         del keep
 
+    def traverse_torch(self, semiring, x, max_iters=1000, **kw):
+        """Fixpoint x <- x (+) x^T M in place on a float32 CUDA vector of
+        length rows (square matrix); returns the number of products computed."""
+        import torch
+        if isinstance(semiring, str):
+            semiring = SEMIRINGS[semiring]
+        assert x.is_cuda and x.dtype == torch.float32 and x.is_contiguous() and x.dim() == 1
+        o, keep = self._opts(**kw)
+        n = C.c_int()
+        stream = torch.cuda.current_stream(x.device).cuda_stream
+        _check(_lib.dev().gmr_engine_traverse(self._h, semiring, C.c_void_p(x.data_ptr()), max_iters, C.byref(n),
+                                              C.byref(o), C.c_void_p(stream)))
+        del keep
+        return n.value
+
     def run_left_torch(self, semiring, X, Y=None, **kw):
         """Left product for torch device tensors: X float32 (rows, B) or (rows,)."""
         import torch

@@ -107,6 +107,7 @@ def dev():
     _sig(L, "gmr_engine_check", C.c_int, [vp, c_int_p])
     _sig(L, "gmr_engine_debug", C.c_int, [vp, c_ll_p])
     _sig(L, "gmr_engine_set_gather_order", C.c_int, [vp, C.c_int])
+    _sig(L, "gmr_engine_traverse", C.c_int, [vp, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(RunOpts), C.c_void_p])
     _sig(L, "gmr_engine_run_left", C.c_int, [vp, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(RunOpts), C.c_void_p])
     _sig(L, "gmr_engine_time_levels", C.c_int, [vp, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(RunOpts), C.c_int, c_float_p, C.c_void_p])
     _sig(L, "gmr_engine_destroy", None, [vp])

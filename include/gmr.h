@@ -104,6 +104,15 @@ int gmr_engine_set_gather_order(gmr_engine* e, int order);
  * (reference run_benchmark.py:138-141); the reference's GPU engine has none. */
 int gmr_engine_run_left(gmr_engine* e, int semiring, const void* d_X, int B, void* d_Y,
                         const gmr_run_opts* opts, void* stream);
+/* Fixpoint traversal over a square matrix: x <- x (+) x^T M until no entry
+ * changes or max_iters products were computed. d_x: rows floats, in: the
+ * start (Boolean 0/1 sources; Tropical distances, 1e9 = unreached), out: the
+ * reachable set / shortest distances along row -> column edges. Synchronises
+ * `stream` after every product; *iters_out = products computed. The
+ * reference's fixpoint comparison is cuGraph BFS / SSSP
+ * (reference cugraph_bench.py:53-61). */
+int gmr_engine_traverse(gmr_engine* e, int semiring, void* d_x, int max_iters, int* iters_out,
+                        const gmr_run_opts* opts, void* stream);
 /* Persistent variant health check: *barrier_error = 1 if a grid-barrier wait
  * of an earlier persistent launch on this engine timed out (its results are
  * invalid; the barrier has been reset), else 0. Synchronises the device. */
