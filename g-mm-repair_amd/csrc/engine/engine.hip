@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -75,9 +76,13 @@ Engine::~Engine() {
                     static_cast<void*>(d_lpoff), static_cast<void*>(d_lbits), static_cast<void*>(d_lord),
                     static_cast<void*>(d_lrow), ws_trav})
         if (p) (void)hipFree(p);
+    for (auto& kv : xparts)
+        for (void* p : {static_cast<void*>(kv.second.d_sym), static_cast<void*>(kv.second.d_bits),
+                        static_cast<void*>(kv.second.d_ord), static_cast<void*>(kv.second.d_row),
+                        static_cast<void*>(kv.second.d_poff)})
+            if (p) (void)hipFree(p);
     void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_fbits, d_ford,
-                    d_frow, d_xsym, d_xbits, d_xord, d_xrow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1,
-                    d_barrier, d_kinds};
+                    d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -224,64 +229,6 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
             before += __builtin_popcountll(fbits[c]);
         }
     }
-    // XCD-partitioned single-vector gather: value keys (terminal t -> t, rule
-    // slot s -> U + s) cut into 8 ranges of equal use counts; each range keeps
-    // its symbols in row order, padded to whole 64-symbol chunks with the zero
-    // terminal (which adds the semiring zero to the row before it).
-    std::vector<int> xsym, xord, xrow;
-    std::vector<unsigned long long> xbits;
-    if (resident) {
-        const long nsym = static_cast<long>(csym.size());
-        const long nkey = static_cast<long>(U) + NL;
-        auto key = [&](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : static_cast<long>(U) + c; };
-        std::vector<long> uses(static_cast<std::size_t>(nkey) + 1, 0);
-        for (int c : csym) ++uses[key(c) + 1];
-        for (long k = 0; k < nkey; ++k) uses[k + 1] += uses[k];
-        long cut[9];
-        cut[0] = 0;
-        cut[8] = nkey;
-        for (int p = 1; p < 8; ++p)
-            cut[p] = std::lower_bound(uses.begin(), uses.end(), nsym * p / 8) - uses.begin();
-        for (int p = 1; p < 8; ++p) cut[p] = std::max(cut[p], cut[p - 1]);
-        auto part = [&](int c) -> int { return static_cast<int>(std::upper_bound(cut + 1, cut + 8, key(c)) - (cut + 1)); };
-        std::vector<std::vector<int>> lsym(8), lrow(8);
-        std::vector<std::vector<long>> lstart(8);  // positions (in the list) that start a row
-        for (int r = 0; r < rows; ++r) {
-            int last_p = -1;
-            unsigned seen = 0;
-            for (int q = crow[r]; q < crow[r + 1]; ++q) {
-                const int p = part(csym[q]);
-                if (!(seen & (1u << p))) {
-                    seen |= 1u << p;
-                    lstart[p].push_back(static_cast<long>(lsym[p].size()));
-                    lrow[p].push_back(r);
-                }
-                lsym[p].push_back(csym[q]);
-                last_p = p;
-            }
-            (void)last_p;
-        }
-        int chunks = 0, rbase = 0;
-        for (int p = 0; p < 8; ++p) {
-            xpoff[p] = chunks;
-            const long n = static_cast<long>(lsym[p].size());
-            const long nc = (n + 63) / 64;
-            std::vector<unsigned long long> bits(static_cast<std::size_t>(nc), 0ull);
-            for (long st : lstart[p]) bits[st >> 6] |= 1ull << (st & 63);
-            int before_p = rbase;
-            for (long c = 0; c < nc; ++c) {
-                xbits.push_back(bits[c]);
-                xord.push_back(before_p);
-                before_p += __builtin_popcountll(bits[c]);
-            }
-            for (long q = 0; q < nc * 64; ++q) xsym.push_back(q < n ? lsym[p][q] : -1);
-            xrow.insert(xrow.end(), lrow[p].begin(), lrow[p].end());
-            rbase += static_cast<int>(lrow[p].size());
-            chunks += static_cast<int>(nc);
-        }
-        xpoff[8] = chunks;
-    }
-
     // Single-vector gather: lanes per row ~ a quarter of the mean row length.
     {
         const double mean = !gseg.empty() ? static_cast<double>(E + Et) / gseg.size() : 0.0;
@@ -305,10 +252,6 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
         d_ford = dev_upload(ford.data(), ford.size(), &struct_bytes);
         d_frow = dev_upload(frow.data(), frow.size(), &struct_bytes);
         n_fchunk = static_cast<int>(fbits.size());
-        d_xsym = dev_upload(xsym.data(), xsym.size(), &struct_bytes);
-        d_xbits = dev_upload(xbits.data(), xbits.size(), &struct_bytes);
-        d_xord = dev_upload(xord.data(), xord.size(), &struct_bytes);
-        d_xrow = dev_upload(xrow.data(), xrow.size(), &struct_bytes);
     }
     d_level_off = dev_upload(level_off.data(), level_off.size(), &struct_bytes);
     d_emit_off = dev_upload(emit_off.data(), emit_off.size(), &struct_bytes);
@@ -323,6 +266,91 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
     GMR_HIP(hipMemset(d_barrier, 0, sizeof(dev::GridBarrier)));
     GMR_HIP(hipDeviceSynchronize());
     barrier_base = 0;
+}
+
+// Passes that keep each XCD's share of one value range near 3 MB (a 4 MB L2
+// per XCD), at most 8.
+int Engine::auto_xcd_passes() const {
+    const double bytes = (static_cast<double>(U) + NL) * 4.0;
+    const double per = 8.0 * 3.0 * 1024 * 1024;
+    return std::max(1, std::min(8, static_cast<int>(std::ceil(bytes / per))));
+}
+
+// The XCD-partitioned single-vector gather's lists for `passes` passes: value
+// keys (terminal t -> t, rule slot s -> U + s) cut into 8 x passes ranges of
+// equal use counts; each range keeps its symbols in row order, padded to whole
+// 64-symbol chunks with the zero terminal (which adds the semiring zero to the
+// row before it). Built from the uploaded top sequence on first use.
+const Engine::XParts& Engine::xparts_for(int passes) {
+    auto it = xparts.find(passes);
+    if (it != xparts.end()) return it->second;
+    if (!resident) throw std::invalid_argument("the partitioned gather needs the Resident layout");
+    const int P = 8 * passes;
+    const long nsym = crow_host[0];
+    std::vector<int> crow(static_cast<std::size_t>(rows) + 1), csym(static_cast<std::size_t>(nsym));
+    GMR_HIP(hipDeviceSynchronize());
+    GMR_HIP(hipMemcpy(crow.data(), d_crow, sizeof(int) * crow.size(), hipMemcpyDeviceToHost));
+    if (nsym) GMR_HIP(hipMemcpy(csym.data(), d_csym, sizeof(int) * nsym, hipMemcpyDeviceToHost));
+    const long nkey = static_cast<long>(U) + NL;
+    auto key = [&](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : static_cast<long>(U) + c; };
+    std::vector<long> uses(static_cast<std::size_t>(nkey) + 1, 0);
+    for (int c : csym) ++uses[key(c) + 1];
+    for (long k = 0; k < nkey; ++k) uses[k + 1] += uses[k];
+    std::vector<long> cut(P + 1);
+    cut[0] = 0;
+    cut[P] = nkey;
+    for (int p = 1; p < P; ++p)
+        cut[p] = std::max(cut[p - 1], static_cast<long>(std::lower_bound(uses.begin(), uses.end(), nsym * p / P) -
+                                                        uses.begin()));
+    auto part = [&](int c) -> int {
+        return static_cast<int>(std::upper_bound(cut.begin() + 1, cut.begin() + P, key(c)) - (cut.begin() + 1));
+    };
+    std::vector<std::vector<int>> lsym(P), lrow(P);
+    std::vector<std::vector<long>> lstart(P);  // positions (in the list) that start a row
+    std::vector<int> seen(P, -1);
+    for (int r = 0; r < rows; ++r) {
+        for (int q = crow[r]; q < crow[r + 1]; ++q) {
+            const int p = part(csym[q]);
+            if (seen[p] != r) {
+                seen[p] = r;
+                lstart[p].push_back(static_cast<long>(lsym[p].size()));
+                lrow[p].push_back(r);
+            }
+            lsym[p].push_back(csym[q]);
+        }
+    }
+    XParts x;
+    std::vector<int> xsym, xord, xrow;
+    std::vector<unsigned long long> xbits;
+    x.poff.assign(P + 1, 0);
+    long chunks = 0, rbase = 0;
+    for (int p = 0; p < P; ++p) {
+        x.poff[p] = static_cast<int>(chunks);
+        const long n = static_cast<long>(lsym[p].size());
+        const long nc = (n + 63) / 64;
+        std::vector<unsigned long long> bits(static_cast<std::size_t>(nc), 0ull);
+        for (long st : lstart[p]) bits[st >> 6] |= 1ull << (st & 63);
+        long before_p = rbase;
+        for (long c = 0; c < nc; ++c) {
+            xbits.push_back(bits[c]);
+            xord.push_back(static_cast<int>(before_p));
+            before_p += __builtin_popcountll(bits[c]);
+        }
+        for (long q = 0; q < nc * 64; ++q) xsym.push_back(q < n ? lsym[p][q] : -1);
+        xrow.insert(xrow.end(), lrow[p].begin(), lrow[p].end());
+        rbase += static_cast<long>(lrow[p].size());
+        chunks += nc;
+        std::vector<int>().swap(lsym[p]);
+    }
+    x.poff[P] = static_cast<int>(chunks);
+    if (static_cast<long>(xsym.size()) > 0x7fffffffL || rbase > 0x7fffffffL)
+        throw std::invalid_argument("partitioned gather: list too long for int32 offsets");
+    x.d_sym = dev_upload(xsym.data(), xsym.size(), &struct_bytes);
+    x.d_bits = dev_upload(xbits.data(), xbits.size(), &struct_bytes);
+    x.d_ord = dev_upload(xord.data(), xord.size(), &struct_bytes);
+    x.d_row = dev_upload(xrow.data(), xrow.size(), &struct_bytes);
+    x.d_poff = dev_upload(x.poff.data(), x.poff.size(), &struct_bytes);
+    return xparts.emplace(passes, std::move(x)).first->second;
 }
 
 int Engine::check_barrier() {
@@ -432,9 +460,22 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
             hipLaunchKernelGGL(dev::k_emit<S>, dim3(grid_for(work, block)), dim3(block), 0, st, a, k);
         }
     }
-    if (a.gather) {
-        const long threads = a.xparts ? static_cast<long>(en.xpoff[8]) * 64
-                             : a.flat ? static_cast<long>(en.n_fchunk) * 64
+    if (a.gather && a.xparts) {
+        // One launch per pass: the lists of a pass share the XCDs' L2s only
+        // with each other.
+        const int passes = a.xpass1;  // make_args sets [0, passes)
+        const std::vector<int>& poff = en.xparts.at(passes).poff;
+        for (int pass = 0; pass < passes; ++pass) {
+            SweepArgs ap = a;
+            ap.xpass0 = pass;
+            ap.xpass1 = pass + 1;
+            long most = 0;
+            for (int q = 0; q < 8; ++q) most = std::max<long>(most, poff[8 * pass + q + 1] - poff[8 * pass + q]);
+            if (most > 0)
+                hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(most * 8 * 64, block)), dim3(block), 0, st, ap);
+        }
+    } else if (a.gather) {
+        const long threads = a.flat ? static_cast<long>(en.n_fchunk) * 64
                              : (a.Bt == 1 && a.B == 1) ? static_cast<long>(en.n_gseg) * a.GR
                                                        : static_cast<long>(en.n_gseg) * G;
         hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(threads, block)), dim3(block), 0, st, a);
@@ -447,7 +488,8 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
 // (occupancy x CUs, capped) so every workgroup is on the chip while the
 // barrier waits; a wait that outlives its deadline ends with the error word set.
 template <class S>
-void launch_persistent(Engine& en, const SweepArgs& a, int block, int grid_cap, hipStream_t st) {
+void launch_persistent(Engine& en, const SweepArgs& a, int block, int grid_cap, hipStream_t st, int tiles = 1,
+                       int tile_w = 0, int vbytes = 4) {
     // The barrier needs every workgroup resident at once. The occupancy query
     // can promise one workgroup per CU more than the dispatcher actually keeps
     // resident (measured: 7/CU reported, fewer held, at 71 VGPRs x 256 lanes),
@@ -462,9 +504,10 @@ void launch_persistent(Engine& en, const SweepArgs& a, int block, int grid_cap, 
     en.last_grid = grid;
     en.last_per_cu = per_cu;
     hipLaunchKernelGGL(dev::k_persistent<S>, dim3(grid), dim3(block), 0, st, a, en.d_barrier, en.barrier_base,
-                       deadline);
+                       deadline, tiles, tile_w > 0 ? tile_w : a.Bt, vbytes);
     GMR_HIP(hipGetLastError());
-    en.barrier_base += static_cast<unsigned long long>(grid) * static_cast<unsigned long long>(a.L + 1);
+    en.barrier_base += static_cast<unsigned long long>(grid) *
+                       (static_cast<unsigned long long>(tiles) * static_cast<unsigned long long>(a.L + 2) - 1ULL);
 }
 
 template <class S>
@@ -540,12 +583,19 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
     a.resident = en.resident ? 1 : 0;
     a.gather = gather ? 1 : 0;
     a.flat = (gather && a.Bt == 1 && B == 1 && !(o.flags & GMR_RUN_ROW_SEGMENTS)) ? 1 : 0;
-    a.xsym = en.d_xsym;
-    a.xbits = en.d_xbits;
-    a.xord = en.d_xord;
-    a.xrow = en.d_xrow;
-    for (int p = 0; p < 9; ++p) a.xpoff[p] = en.xpoff[p];
     a.xparts = (a.flat && en.resident && !(o.flags & GMR_RUN_SINGLE_LIST)) ? 1 : 0;
+    if (a.xparts) {
+        if (o.xcd_passes < 0 || o.xcd_passes > 8) throw std::invalid_argument("xcd_passes must be in [0, 8]");
+        const int passes = o.xcd_passes > 0 ? o.xcd_passes : en.auto_xcd_passes();
+        const Engine::XParts& x = en.xparts_for(passes);
+        a.xsym = x.d_sym;
+        a.xbits = x.d_bits;
+        a.xord = x.d_ord;
+        a.xrow = x.d_row;
+        a.xpoff = x.d_poff;
+        a.xpass0 = 0;
+        a.xpass1 = passes;
+    }
     a.GR = en.gather_lanes;
     a.eval_kind = o.eval_kind;
     a.level_kind = o.level_kind ? en.device_kinds(o.level_kind) : nullptr;
@@ -559,6 +609,13 @@ void run_all(Engine& en, int semiring, const float* X, int B, void* Y, const gmr
     const bool gather = o.emit == GMR_EMIT_GATHER;
     if (gather && !en.resident) throw std::invalid_argument("gather emission needs the Resident layout");
     en.ensure_workspace(padded_width(Bt), static_cast<int>(sizeof(T)));
+    if (o.variant == GMR_VARIANT_PERSISTENT && Bt < B) {
+        // every vector tile in one launch
+        const SweepArgs a = make_args(en, X, B, Y, static_cast<int>(sizeof(T)), 0, Bt, gather, o);
+        launch_persistent<S>(en, a, o.block > 0 ? o.block : 256, o.grid_cap, st, (B + Bt - 1) / Bt, Bt,
+                             static_cast<int>(sizeof(T)));
+        return;
+    }
     for (int b0 = 0; b0 < B; b0 += Bt) {
         SweepArgs a = make_args(en, X, B, Y, static_cast<int>(sizeof(T)), b0, Bt, gather, o);
         run_tile<S>(en, semiring, a, o, st);

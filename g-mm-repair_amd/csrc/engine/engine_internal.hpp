@@ -42,11 +42,18 @@ struct Engine {
     int* d_ford = nullptr;           // rows started before each 64-symbol chunk
     int* d_frow = nullptr;           // non-empty rows in order
     int n_fchunk = 0;
-    int* d_xsym = nullptr;           // XCD-partitioned single-vector gather (8 padded lists)
-    unsigned long long* d_xbits = nullptr;
-    int* d_xord = nullptr;
-    int* d_xrow = nullptr;
-    int xpoff[9] = {0};
+    // XCD-partitioned single-vector gather, per pass count (built on first use)
+    struct XParts {
+        int* d_sym = nullptr;
+        unsigned long long* d_bits = nullptr;
+        int* d_ord = nullptr;
+        int* d_row = nullptr;
+        int* d_poff = nullptr;
+        std::vector<int> poff;       // 8 x passes + 1 chunk offsets
+    };
+    std::map<int, XParts> xparts;
+    const XParts& xparts_for(int passes);
+    int auto_xcd_passes() const;
     std::vector<int> crow_host;      // [0] = number of top-sequence symbols
     int* d_level_off = nullptr;
     int* d_emit_off = nullptr;

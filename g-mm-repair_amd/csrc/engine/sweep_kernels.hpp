@@ -2,8 +2,9 @@
 //
 // Work is written as "phase" functions over a range of items, parameterised by
 // a group index and a group count, so the same code runs as one launch per
-// level (the plain sweep), inside captured graphs, and inside the persistent
-// cooperative kernel where phases are separated by grid barriers.
+// level (the plain sweep), inside captured graphs, inside the persistent
+// kernels (one grammar, or many as tiles) where phases are separated by the
+// bounded grid barrier below, and for the left product's node gathers.
 //
 // Lane mapping: a group of G lanes (G = 1, 2, 4, ..., 64, a divisor of the
 // wave) owns one item (a rule, an emission chunk, a row); its lanes stride over
@@ -76,14 +77,15 @@ struct SweepArgs {
     int n_fchunk;            // flat gather: 64-symbol chunks of csym
     int n_csym;              // top-sequence symbols (csym entries)
     int flat;                // one vector: gather over 64-symbol chunks instead of row segments
-    // One vector, XCD-partitioned gather: the symbols split into 8 value ranges
-    // (terminals then rule slots, equal use counts), each a row-ordered list
-    // padded to 64-symbol chunks with the zero terminal.
-    const int* xsym;         // the 8 lists, concatenated
+    // One vector, XCD-partitioned gather: the symbols split into 8 x passes
+    // value ranges (terminals then rule slots, equal use counts), each a
+    // row-ordered list padded to 64-symbol chunks with the zero terminal.
+    const int* xsym;         // the lists, concatenated
     const unsigned long long* xbits;  // per chunk: which symbols start a row of their list
     const int* xord;         // per chunk: index into xrow of the first row started before it
     const int* xrow;         // per list: its non-empty rows in order
-    int xpoff[9];            // chunk range of list p: [xpoff[p], xpoff[p+1])
+    const int* xpoff;        // chunk range of list p: [xpoff[p], xpoff[p+1])
+    int xpass0, xpass1;      // passes [xpass0, xpass1) of this launch; pass k owns lists 8k .. 8k+7
     int xparts;              // use the partitioned gather
     const int* level_off;    // max_depth + 1
     const int* emit_off;     // max_depth + 1
@@ -530,8 +532,8 @@ __device__ void ph_gather_flat(const SweepArgs& a, long c0, long c1, long wid, l
 }
 
 // One vector, XCD-partitioned: workgroups are dispatched to the 8 XCDs round-
-// robin, so workgroup b works on list b % 8 and every XCD's L2 only holds the
-// values of one eighth of the terminals and rule slots. Each list is swept like
+// robin, so in pass k workgroup b works on list 8k + b % 8 and every XCD's L2
+// only holds the values of one of the 8 x passes value ranges. Each list is swept like
 // the chunked gather; every row piece is combined atomically into the zeroed y,
 // since the row's other symbols live in other lists. `blk`/`nblk` are the raw
 // workgroup index and count (not renumbered).
@@ -555,7 +557,9 @@ __device__ void ph_gather_parts(const SweepArgs& a, long blk, long nblk, int tid
         j = 0;
         nj = 1;
     }
-    for (long p = p0; p < 8; p += pstep) {
+    for (long pass = a.xpass0; pass < a.xpass1; ++pass)
+    for (long q = p0; q < 8; q += pstep) {
+        const long p = pass * 8 + q;
         for (long c = a.xpoff[p] + j * wpb + w; c < a.xpoff[p + 1]; c += nj * wpb) {
             const int s = a.xsym[c * 64 + lane];
             T v = s >= 0 ? W[s] : Tt[-s - 1];
@@ -824,15 +828,16 @@ __device__ __forceinline__ void grid_barrier(GridBarrier* gb, unsigned long long
 //                  emits for k = 1) -- both only read frontier k-1
 //   after level L  spot: level L's emits; gather: the row gather
 // `base` is the arrival count before this launch.
+// One sweep of the tile described by `a`; `nbar` counts the barriers passed.
 template <class S>
-__global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long base,
-                             unsigned long long deadline) {
+__device__ void persistent_sweep(const SweepArgs& a, GridBarrier* gb, unsigned long long base, unsigned long long& nbar,
+                                 unsigned long long deadline) {
     const long tid = xcd_tid();
     const long nth = static_cast<long>(gridDim.x) * blockDim.x;
     const int lane = static_cast<int>(tid % a.G);
     const unsigned long long nb = gridDim.x;
     ph_init<S>(a, tid, nth);
-    grid_barrier(gb, base + nb, deadline);
+    grid_barrier(gb, base + nb * (++nbar), deadline);
     for (int k = 1; k <= a.L; ++k) {
         if (!a.gather) {
             if (k == 1) {
@@ -845,7 +850,7 @@ __global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long ba
         }
         ph_eval_kind<S>(a, level_kind<S>(a, k), a.level_off[k - 1], a.level_off[k], level_src<S>(a, k),
                         level_dst<S>(a, k), tid, nth);
-        grid_barrier(gb, base + nb * static_cast<unsigned long long>(k + 1), deadline);
+        grid_barrier(gb, base + nb * (++nbar), deadline);
     }
     if (a.gather && a.xparts) {
         ph_gather_parts<S>(a, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
@@ -857,6 +862,35 @@ __global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long ba
         const int e0 = a.emit_off[a.L - 1], e1 = a.emit_off[a.L];
         const typename S::T* src = a.resident ? tp<S>(a.W0) : level_dst<S>(a, a.L);
         ph_emit<S>(a, a.emits + e0, e1 - e0, src, tid, nth, lane);
+    }
+}
+
+// The whole product in one launch: `tiles` vector tiles of width `tile_w`
+// (the last may be narrower), one after another through the same workspace,
+// with a barrier between a tile's emission and the next tile's init. `a`
+// describes the first tile; the others differ only in their column base and
+// width. Barriers per launch: tiles * (L + 2) - 1.
+template <class S>
+__global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long base, unsigned long long deadline,
+                             int tiles, int tile_w, int vbytes) {
+    unsigned long long nbar = 0;
+    for (int t = 0; t < tiles; ++t) {
+        if (t > 0) {
+            const unsigned long long nb = gridDim.x;
+            grid_barrier(gb, base + nb * (++nbar), deadline);
+            const int b0 = t * tile_w;
+            SweepArgs at = a;
+            at.X = a.X + b0;
+            at.Y = static_cast<char*>(a.Y) + static_cast<long>(b0) * vbytes;
+            at.Bt = min(tile_w, a.B - b0);
+            at.Bs = at.Bt <= 1 ? 1 : (at.Bt + 3) / 4 * 4;
+            at.vec = at.Bt > 1 ? 1 : 0;
+            at.yvec = (a.B % 4 == 0 && b0 % 4 == 0) ? 1 : 0;
+            at.G = lanes_for(at.Bs, at.vec);
+            persistent_sweep<S>(at, gb, base, nbar, deadline);
+        } else {
+            persistent_sweep<S>(a, gb, base, nbar, deadline);
+        }
     }
 }
 

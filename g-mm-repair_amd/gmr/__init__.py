@@ -268,8 +268,8 @@ class Engine:
 
     @staticmethod
     def _opts(variant=VARIANT_LEVELS, emit=EMIT_SPOT, b_tile=0, block=0, grid_cap=0, eval_kind=0,
-              level_kind=None, flags=0):
-        o = RunOpts(variant, emit, b_tile, block, grid_cap, eval_kind, flags, 0, None)
+              level_kind=None, flags=0, xcd_passes=0):
+        o = RunOpts(variant, emit, b_tile, block, grid_cap, eval_kind, flags, xcd_passes, None)
         keep = None
         if level_kind is not None:
             keep = np.ascontiguousarray(level_kind, dtype=np.int32)

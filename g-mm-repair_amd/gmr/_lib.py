@@ -38,7 +38,7 @@ class RunOpts(C.Structure):
     """gmr.h gmr_run_opts."""
     _fields_ = [
         ("variant", C.c_int), ("emit", C.c_int), ("b_tile", C.c_int), ("block", C.c_int),
-        ("grid_cap", C.c_int), ("eval_kind", C.c_int), ("flags", C.c_int), ("reserved", C.c_int),
+        ("grid_cap", C.c_int), ("eval_kind", C.c_int), ("flags", C.c_int), ("xcd_passes", C.c_int),
         ("level_kind", c_int_p),
     ]
 

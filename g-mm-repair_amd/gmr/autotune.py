@@ -21,8 +21,8 @@ EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
 def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), (1024, 0), (256, 256))):
     """The search space for one engine and batch width: variant x emission x
     vector tile x launch shape (x gather order for row-gather emission of
-    several vectors; x XCD-partitioned, single-list or row-segment gather for
-    one vector).
+    several vectors; x XCD-partitioned (1, 2, 4 or automatic passes),
+    single-list or row-segment gather for one vector).
     Persistent shapes are (block, grid cap); a cap of 0 means the engine's
     default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
@@ -35,11 +35,14 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
                 for blk, cap in shapes:
                     for order in ((0, 1) if e == EMIT_GATHER and B > 1 else (0,)):
                         for fl in ((0, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST) if e == EMIT_GATHER and B == 1 else (0,)):
-                            c = dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap,
-                                     gather_order=order)
-                            if fl:
-                                c["flags"] = fl
-                            out.append(c)
+                            for passes in ((0, 1, 2, 4) if e == EMIT_GATHER and B == 1 and fl == 0 else (None,)):
+                                c = dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap,
+                                         gather_order=order)
+                                if fl:
+                                    c["flags"] = fl
+                                if passes is not None:
+                                    c["xcd_passes"] = passes
+                                out.append(c)
     return out
 
 
@@ -57,7 +60,8 @@ def describe(c):
     return (f"{VARIANT_NAMES[c['variant']]}/{EMIT_NAMES[c['emit']]}/tile={c['b_tile'] or 'B'}/blk={c['block']}"
             + (f"/grid<={cap}" if cap else "") + ("/order=rule" if c.get("gather_order") == 1 else "")
             + ("/segments" if c.get("flags", 0) & RUN_ROW_SEGMENTS else "")
-            + ("/single-list" if c.get("flags", 0) & RUN_SINGLE_LIST else ""))
+            + ("/single-list" if c.get("flags", 0) & RUN_SINGLE_LIST else "")
+            + (f"/xcd-passes={c['xcd_passes'] or 'auto'}" if "xcd_passes" in c else ""))
 
 
 def shape_key(info, B, semiring):

@@ -68,7 +68,9 @@ typedef struct gmr_run_opts {
     int grid_cap;           /* persistent variant: max workgroups (0: 16 waves per CU) */
     int eval_kind;          /* gmr_eval_kind of every level not listed in level_kind */
     int flags;              /* gmr_run_flags */
-    int reserved;
+    int xcd_passes;         /* one vector, XCD-partitioned gather: the values are cut into 8 x this
+                               many ranges, swept in this many launches so each XCD's L2 holds the
+                               values of one range at a time (0: sized from the value count) */
     const int* level_kind;  /* optional host array, max_depth + 1 entries ([0] unused) */
 } gmr_run_opts;
 

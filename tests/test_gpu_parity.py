@@ -425,6 +425,13 @@ def test_single_vector_gathers_agree(sr):
             _say("one-vector gather", sr, flags, var)
             y = run_engine(eng, sr, x, variant=gmr.VARIANTS[var], emit=gmr.EMIT_GATHER, flags=flags)
             check(sr, y, y_star, den, f"gather flags={flags} {var}")
+    for passes in (1, 2, 3, 8, 0):
+        for var in ("levels", "graph", "persistent"):
+            _say("xcd-partitioned gather", sr, passes, "passes", var)
+            y = run_engine(eng, sr, x, variant=gmr.VARIANTS[var], emit=gmr.EMIT_GATHER, xcd_passes=passes)
+            check(sr, y, y_star, den, f"xcd passes={passes} {var}")
+    with pytest.raises(gmr.GmrError):
+        run_engine(eng, sr, x, emit=gmr.EMIT_GATHER, xcd_passes=9)
     # grid shapes the partitioned gather must handle: fewer than 8 workgroups
     # (each takes several lists) and counts that are not multiples of 8
     for cap in (1, 3, 12):
