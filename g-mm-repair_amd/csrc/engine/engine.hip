@@ -74,14 +74,15 @@ Engine::~Engine() {
                     static_cast<void*>(d_col_off), static_cast<void*>(d_col_term), ws_A, ws_Xs,
                     static_cast<void*>(d_lseg), static_cast<void*>(d_lsplit), static_cast<void*>(d_lpsrc),
                     static_cast<void*>(d_lpoff), static_cast<void*>(d_lbits), static_cast<void*>(d_lord),
-                    static_cast<void*>(d_lrow), ws_trav})
+                    static_cast<void*>(d_lrow), ws_trav, static_cast<void*>(d_lxsym), static_cast<void*>(d_lxbits),
+                    static_cast<void*>(d_lxord), static_cast<void*>(d_lxrow), static_cast<void*>(d_lxpoff)})
         if (p) (void)hipFree(p);
     for (auto& kv : xparts)
         for (void* p : {static_cast<void*>(kv.second.d_sym), static_cast<void*>(kv.second.d_bits),
                         static_cast<void*>(kv.second.d_ord), static_cast<void*>(kv.second.d_row),
                         static_cast<void*>(kv.second.d_poff)})
             if (p) (void)hipFree(p);
-    void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_fbits, d_ford,
+    void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_zrows, d_fbits, d_ford,
                     d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -187,6 +188,7 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
     // Resident layout: the row-ordered top sequence for gather emission.
     std::vector<int> crow, csym;
     std::vector<int4> gseg;
+    std::vector<int> zrows;
     gather_split = 0;
     if (resident) {
         crow.assign(static_cast<std::size_t>(rows) + 1, 0);
@@ -206,6 +208,7 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
             const int a0 = crow[r], a1 = crow[r + 1];
             const int split = (a1 - a0) > kGatherSeg ? 1 : 0;
             gather_split |= split;
+            if (split) zrows.push_back(r);
             if (a1 == a0) gseg.push_back(make_int4(r, a0, a1, 0));
             for (int q = a0; q < a1; q += kGatherSeg) gseg.push_back(make_int4(r, q, std::min(a1, q + kGatherSeg), split));
         }
@@ -246,6 +249,8 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
         d_crow = dev_upload(crow.data(), crow.size(), &struct_bytes);
         d_csym = dev_upload(csym.data(), csym.size(), &struct_bytes);
         d_gseg = dev_upload(gseg.data(), gseg.size(), &struct_bytes);
+        d_zrows = dev_upload(zrows.data(), zrows.size(), &struct_bytes);
+        n_zrows = static_cast<int>(zrows.size());
         n_gseg = static_cast<int>(gseg.size());
         crow_host.assign(1, static_cast<int>(csym.size()));
         d_fbits = dev_upload(fbits.data(), fbits.size(), &struct_bytes);
@@ -276,79 +281,93 @@ int Engine::auto_xcd_passes() const {
     return std::max(1, std::min(8, static_cast<int>(std::ceil(bytes / per))));
 }
 
-// The XCD-partitioned single-vector gather's lists for `passes` passes: value
-// keys (terminal t -> t, rule slot s -> U + s) cut into 8 x passes ranges of
-// equal use counts; each range keeps its symbols in row order, padded to whole
-// 64-symbol chunks with the zero terminal (which adds the semiring zero to the
-// row before it). Built from the uploaded top sequence on first use.
-const Engine::XParts& Engine::xparts_for(int passes) {
-    auto it = xparts.find(passes);
-    if (it != xparts.end()) return it->second;
-    if (!resident) throw std::invalid_argument("the partitioned gather needs the Resident layout");
-    const int P = 8 * passes;
-    const long nsym = crow_host[0];
-    std::vector<int> crow(static_cast<std::size_t>(rows) + 1), csym(static_cast<std::size_t>(nsym));
-    GMR_HIP(hipDeviceSynchronize());
-    GMR_HIP(hipMemcpy(crow.data(), d_crow, sizeof(int) * crow.size(), hipMemcpyDeviceToHost));
-    if (nsym) GMR_HIP(hipMemcpy(csym.data(), d_csym, sizeof(int) * nsym, hipMemcpyDeviceToHost));
-    const long nkey = static_cast<long>(U) + NL;
-    auto key = [&](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : static_cast<long>(U) + c; };
+namespace {
+
+// Host lists of the XCD-partitioned single-vector gather, appended group by
+// group: the entries of rows [r0, r1) of a CSR (off, sym) are cut by value key
+// into P ranges of equal use counts; each range keeps its entries in row order,
+// padded to whole 64-entry chunks with `pad` (an entry that reads the semiring
+// zero). poff gets the P + 1 chunk offsets of this group.
+struct PartLists {
+    std::vector<int> sym, ord, row, poff;
+    std::vector<unsigned long long> bits;
+};
+
+template <class Off, class Key>
+void build_part_lists(long r0, long r1, const Off& off, const std::vector<int>& sym, Key key, long nkey, int P, int pad,
+                      PartLists& out) {
+    const long e0 = off[r0], e1 = off[r1], nent = e1 - e0;
     std::vector<long> uses(static_cast<std::size_t>(nkey) + 1, 0);
-    for (int c : csym) ++uses[key(c) + 1];
+    for (long q = e0; q < e1; ++q) ++uses[key(sym[q]) + 1];
     for (long k = 0; k < nkey; ++k) uses[k + 1] += uses[k];
     std::vector<long> cut(P + 1);
     cut[0] = 0;
     cut[P] = nkey;
     for (int p = 1; p < P; ++p)
-        cut[p] = std::max(cut[p - 1], static_cast<long>(std::lower_bound(uses.begin(), uses.end(), nsym * p / P) -
-                                                        uses.begin()));
+        cut[p] = std::max(cut[p - 1],
+                          static_cast<long>(std::lower_bound(uses.begin(), uses.end(), nent * p / P) - uses.begin()));
     auto part = [&](int c) -> int {
         return static_cast<int>(std::upper_bound(cut.begin() + 1, cut.begin() + P, key(c)) - (cut.begin() + 1));
     };
     std::vector<std::vector<int>> lsym(P), lrow(P);
     std::vector<std::vector<long>> lstart(P);  // positions (in the list) that start a row
-    std::vector<int> seen(P, -1);
-    for (int r = 0; r < rows; ++r) {
-        for (int q = crow[r]; q < crow[r + 1]; ++q) {
-            const int p = part(csym[q]);
+    std::vector<long> seen(P, -1);
+    for (long r = r0; r < r1; ++r) {
+        for (long q = off[r]; q < off[r + 1]; ++q) {
+            const int p = part(sym[q]);
             if (seen[p] != r) {
                 seen[p] = r;
                 lstart[p].push_back(static_cast<long>(lsym[p].size()));
-                lrow[p].push_back(r);
+                lrow[p].push_back(static_cast<int>(r));
             }
-            lsym[p].push_back(csym[q]);
+            lsym[p].push_back(sym[q]);
         }
     }
-    XParts x;
-    std::vector<int> xsym, xord, xrow;
-    std::vector<unsigned long long> xbits;
-    x.poff.assign(P + 1, 0);
-    long chunks = 0, rbase = 0;
     for (int p = 0; p < P; ++p) {
-        x.poff[p] = static_cast<int>(chunks);
+        out.poff.push_back(static_cast<int>(out.bits.size()));
         const long n = static_cast<long>(lsym[p].size());
         const long nc = (n + 63) / 64;
         std::vector<unsigned long long> bits(static_cast<std::size_t>(nc), 0ull);
         for (long st : lstart[p]) bits[st >> 6] |= 1ull << (st & 63);
-        long before_p = rbase;
+        long before = static_cast<long>(out.row.size());
         for (long c = 0; c < nc; ++c) {
-            xbits.push_back(bits[c]);
-            xord.push_back(static_cast<int>(before_p));
-            before_p += __builtin_popcountll(bits[c]);
+            out.bits.push_back(bits[c]);
+            out.ord.push_back(static_cast<int>(before));
+            before += __builtin_popcountll(bits[c]);
         }
-        for (long q = 0; q < nc * 64; ++q) xsym.push_back(q < n ? lsym[p][q] : -1);
-        xrow.insert(xrow.end(), lrow[p].begin(), lrow[p].end());
-        rbase += static_cast<long>(lrow[p].size());
-        chunks += nc;
+        for (long q = 0; q < nc * 64; ++q) out.sym.push_back(q < n ? lsym[p][q] : pad);
+        out.row.insert(out.row.end(), lrow[p].begin(), lrow[p].end());
         std::vector<int>().swap(lsym[p]);
     }
-    x.poff[P] = static_cast<int>(chunks);
-    if (static_cast<long>(xsym.size()) > 0x7fffffffL || rbase > 0x7fffffffL)
+    out.poff.push_back(static_cast<int>(out.bits.size()));
+    if (static_cast<long>(out.sym.size()) > 0x7fffffffL || static_cast<long>(out.row.size()) > 0x7fffffffL)
         throw std::invalid_argument("partitioned gather: list too long for int32 offsets");
-    x.d_sym = dev_upload(xsym.data(), xsym.size(), &struct_bytes);
-    x.d_bits = dev_upload(xbits.data(), xbits.size(), &struct_bytes);
-    x.d_ord = dev_upload(xord.data(), xord.size(), &struct_bytes);
-    x.d_row = dev_upload(xrow.data(), xrow.size(), &struct_bytes);
+}
+
+}  // namespace
+
+// The XCD-partitioned single-vector gather's lists for `passes` passes: value
+// keys (terminal t -> t, rule slot s -> U + s) cut into 8 x passes ranges;
+// padding reads the zero terminal. Built from the uploaded top sequence on
+// first use.
+const Engine::XParts& Engine::xparts_for(int passes) {
+    auto it = xparts.find(passes);
+    if (it != xparts.end()) return it->second;
+    if (!resident) throw std::invalid_argument("the partitioned gather needs the Resident layout");
+    const long nsym = crow_host[0];
+    std::vector<int> crow(static_cast<std::size_t>(rows) + 1), csym(static_cast<std::size_t>(nsym));
+    GMR_HIP(hipDeviceSynchronize());
+    GMR_HIP(hipMemcpy(crow.data(), d_crow, sizeof(int) * crow.size(), hipMemcpyDeviceToHost));
+    if (nsym) GMR_HIP(hipMemcpy(csym.data(), d_csym, sizeof(int) * nsym, hipMemcpyDeviceToHost));
+    auto key = [&](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : static_cast<long>(U) + c; };
+    PartLists L;
+    build_part_lists(0, rows, crow, csym, key, static_cast<long>(U) + NL, 8 * passes, -1, L);
+    XParts x;
+    x.poff = L.poff;
+    x.d_sym = dev_upload(L.sym.data(), L.sym.size(), &struct_bytes);
+    x.d_bits = dev_upload(L.bits.data(), L.bits.size(), &struct_bytes);
+    x.d_ord = dev_upload(L.ord.data(), L.ord.size(), &struct_bytes);
+    x.d_row = dev_upload(L.row.data(), L.row.size(), &struct_bytes);
     x.d_poff = dev_upload(x.poff.data(), x.poff.size(), &struct_bytes);
     return xparts.emplace(passes, std::move(x)).first->second;
 }
@@ -557,6 +576,8 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
     a.gseg = en.d_gseg;
     a.n_gseg = en.n_gseg;
     a.gsplit = en.gather_split;
+    a.zrows = en.d_zrows;
+    a.n_zrows = en.n_zrows;
     a.fbits = en.d_fbits;
     a.ford = en.d_ford;
     a.frow = en.d_frow;
@@ -860,6 +881,23 @@ void Engine::build_left() {
             before += __builtin_popcountll(pbits[c]);
         }
         if (psrc.size() > 0x7fffffffUL) throw std::invalid_argument("left product: too many entries");
+        // One vector, XCD-partitioned: value keys (row r of x -> r, the zero
+        // row -> rows, rule slot s -> rows + 1 + s), 8 x passes lists per group.
+        {
+            const double bytes = (static_cast<double>(rows) + 1 + NL) * 4.0;
+            lx_passes = std::max(1, std::min(8, static_cast<int>(std::ceil(bytes / (8.0 * 3.0 * 1024 * 1024)))));
+            auto key = [&](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : static_cast<long>(rows) + 1 + c; };
+            PartLists PL;
+            for (int gi = 0; gi <= L; ++gi)
+                build_part_lists(gnode[gi], gnode[gi + 1], off, src, key, static_cast<long>(rows) + 1 + NL, 8 * lx_passes,
+                                 pad, PL);
+            lxpoff = PL.poff;
+            d_lxsym = dev_upload(PL.sym.data(), PL.sym.size(), &struct_bytes);
+            d_lxbits = dev_upload(PL.bits.data(), PL.bits.size(), &struct_bytes);
+            d_lxord = dev_upload(PL.ord.data(), PL.ord.size(), &struct_bytes);
+            d_lxrow = dev_upload(PL.row.data(), PL.row.size(), &struct_bytes);
+            d_lxpoff = dev_upload(lxpoff.data(), lxpoff.size(), &struct_bytes);
+        }
         d_lseg = dev_upload(seg.data(), seg.size(), &struct_bytes);
         d_lsplit = dev_upload(split.data(), split.size(), &struct_bytes);
         n_lsplit = static_cast<int>(split.size());
@@ -925,8 +963,29 @@ void launch_left(Engine& en, const dev::LeftArgs& a, int block, hipStream_t st) 
         hipLaunchKernelGGL(dev::k_left_init<S>, dim3(grid_for(work, block, 4096)), dim3(block), 0, st, a.X, a.B, a.Bt,
                            a.Bs, en.rows, en.ws_Xs, en.ws_A, en.d_lsplit, static_cast<long>(en.n_lsplit), zero_all);
         const SweepArgs g = left_gather_args(en, a.Bt, a.Bs);
+        const int P = 8 * en.lx_passes;
         // group gi = level gi + 1 for gi < L, the terminals for gi = L
         auto group = [&](int gi) {
+            if (g.flat && a.xparts) {
+                // one launch per pass, as in the right product's gather
+                for (int pass = 0; pass < en.lx_passes; ++pass) {
+                    SweepArgs gp = g;
+                    gp.xparts = 1;
+                    gp.xsym = en.d_lxsym;
+                    gp.xbits = en.d_lxbits;
+                    gp.xord = en.d_lxord;
+                    gp.xrow = en.d_lxrow;
+                    gp.xpoff = en.d_lxpoff + static_cast<long>(gi) * (P + 1);
+                    gp.xpass0 = pass;
+                    gp.xpass1 = pass + 1;
+                    const int* po = en.lxpoff.data() + static_cast<long>(gi) * (P + 1);
+                    long most = 0;
+                    for (int q = 0; q < 8; ++q) most = std::max<long>(most, po[8 * pass + q + 1] - po[8 * pass + q]);
+                    if (most > 0)
+                        hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(most * 8 * 64, block)), dim3(block), 0, st, gp);
+                }
+                return;
+            }
             long r0, r1, threads;
             if (g.flat) {
                 r0 = en.lchunk_off[gi];
@@ -1003,9 +1062,10 @@ void run_left_t(Engine& en, int semiring, const float* X, int B, void* Y, const 
         a.Bt = std::min(Bt, B - b0);
         a.Bs = en.has_runlen ? a.Bt : padded_width(a.Bt);
         a.G = dev::lanes_for(a.Bt, 0);
+        a.xparts = (a.Bt == 1 && !(o.flags & GMR_RUN_SINGLE_LIST)) ? 1 : 0;
         if (o.variant == GMR_VARIANT_GRAPH) {
             auto key = std::make_tuple(semiring, reinterpret_cast<uintptr_t>(a.X), reinterpret_cast<uintptr_t>(a.Y), a.B,
-                                       a.Bt, 0, block, reinterpret_cast<uintptr_t>(st));
+                                       a.Bt, a.xparts, block, reinterpret_cast<uintptr_t>(st));
             auto it = en.left_graphs.find(key);
             if (it == en.left_graphs.end()) {
                 if (!en.capture_stream) GMR_HIP(hipStreamCreateWithFlags(&en.capture_stream, hipStreamNonBlocking));

@@ -36,6 +36,8 @@ struct Engine {
     int4* d_gseg = nullptr;          // gather segments (row, first, end, split)
     int n_gseg = 0;
     int gather_split = 0;            // some row is split over several segments
+    int* d_zrows = nullptr;          // the split rows
+    int n_zrows = 0;
     static constexpr int kGatherSeg = 128;
     int gather_order = 0;            // 0: row order, 1: by highest rule
     unsigned long long* d_fbits = nullptr;  // flat single-vector gather: row starts per 64 symbols
@@ -106,6 +108,14 @@ struct Engine {
     int n_lpsrc = 0;
     std::vector<int> lseg_off;       // groups (levels 1..L, then terminals) -> segment ranges
     std::vector<int> lchunk_off;     // groups -> 64-entry chunk ranges
+    // one vector, XCD-partitioned: per group 8 x lx_passes lists
+    int* d_lxsym = nullptr;
+    unsigned long long* d_lxbits = nullptr;
+    int* d_lxord = nullptr;
+    int* d_lxrow = nullptr;
+    int* d_lxpoff = nullptr;         // per group 8 x lx_passes + 1 chunk offsets
+    std::vector<int> lxpoff;
+    int lx_passes = 1;
     std::map<GraphKey, Graph> left_graphs;
 
     Engine() = default;

@@ -58,6 +58,7 @@ struct LeftArgs {
     void* A;               // nodes x Bs node values
     int NL;                // rule slots; terminal t is node NL + t
     int B, Bt, Bs, G;
+    int xparts;            // one vector: the XCD-partitioned lists (else one chunked list)
 };
 
 // Nodes [n0, n1) with coefficients: a group of G lanes per node, lanes over

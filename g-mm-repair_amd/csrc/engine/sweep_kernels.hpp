@@ -70,7 +70,9 @@ struct SweepArgs {
     const int* csym;         // >= 0 global slot of a rule, < 0 terminal -c-1
     const int4* gseg;        // gather segments: (row, first symbol, end, row is split)
     int n_gseg;
-    int gsplit;              // some row is split: y is zeroed and pieces add atomically
+    int gsplit;              // some row is split: its y row is zeroed and the pieces add atomically
+    const int* zrows;        // the split rows (zeroed by the init phase of a segment gather)
+    int n_zrows;
     const unsigned long long* fbits;  // flat gather: bit i of word c set = csym[64c+i] starts a row
     const int* ford;         // flat gather: rows started before csym[64c]
     const int* frow;         // flat gather: row id of the k-th non-empty row
@@ -191,6 +193,12 @@ __host__ __device__ __forceinline__ int lanes_for(int Bs, int vec) {
     return g;
 }
 
+// Whether the init phase zeroes only the listed split rows: a row-segment
+// gather stores every other row whole.
+__host__ __device__ __forceinline__ bool zero_list(const SweepArgs& a) {
+    return a.gather && !a.flat && !a.xparts;
+}
+
 // T[p*Bs + b] = leaf(V, X[col*B + b]) for b < Bt, zero for p = 0 and for the
 // padding lanes; Y = zero for spot emits (and gathers with split rows).
 template <class S>
@@ -206,18 +214,20 @@ __device__ void ph_init_range(const SweepArgs& a, long i0, long i1, long tid, lo
             Tt[i] = (p == 0 || b >= a.Bt) ? S::zero()
                                           : S::leaf(a.tval[p], a.X[static_cast<long>(a.tcol[p]) * a.B + b]);
         } else {
-            long j = i - nT;
-            long r = j / a.Bt;
-            Y[r * a.B + (j - r * a.Bt)] = S::zero();
+            const long j = i - nT;
+            const long q = j / a.Bt;
+            const long r = zero_list(a) ? a.zrows[q] : q;
+            Y[r * a.B + (j - q * a.Bt)] = S::zero();
         }
     }
 }
 
-// Items of the init phase: U*Bs terminal values, then (spot emission, or a
-// gather that combines pieces of rows atomically) rows*Bt y values.
+// Items of the init phase: U*Bs terminal values, then the y values that are
+// combined atomically: every row for spot emission and the chunked gathers,
+// the split rows for the row-segment gather.
 __host__ __device__ __forceinline__ long init_items(const SweepArgs& a) {
-    return static_cast<long>(a.U) * a.Bs +
-           ((a.gather && !a.gsplit && !a.flat && !a.xparts) ? 0L : static_cast<long>(a.rows) * a.Bt);
+    const long ny = zero_list(a) ? (a.gsplit ? static_cast<long>(a.n_zrows) : 0L) : static_cast<long>(a.rows);
+    return static_cast<long>(a.U) * a.Bs + ny * a.Bt;
 }
 
 // Work items of the gather phase: 64-symbol chunks (flat) or row segments.

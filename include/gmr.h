@@ -101,8 +101,9 @@ int gmr_engine_set_gather_order(gmr_engine* e, int order);
 /* Left product Y^T = X^T M on `stream`: X is rows x B (X[r*B + b]), Y is
  * cols x B (Y[c*B + b]); same semirings and leaves as gmr_engine_run, with the
  * matrix value on the left. Resident layout only; opts->variant LEVELS or GRAPH
- * (b_tile and block honoured). The first call builds the incoming-edge lists
- * (synchronising the device). mm-repair computes this product beside Ax
+ * (b_tile and block honoured; for one vector the gather is XCD-partitioned
+ * unless flags has GMR_RUN_SINGLE_LIST). The first call builds the
+ * incoming-edge lists (synchronising the device). mm-repair computes this product beside Ax
  * (reference run_benchmark.py:138-141); the reference's GPU engine has none. */
 int gmr_engine_run_left(gmr_engine* e, int semiring, const void* d_X, int B, void* d_Y,
                         const gmr_run_opts* opts, void* stream);

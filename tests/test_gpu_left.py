@@ -78,9 +78,10 @@ def test_haplotype_left_product(hap, sr, B):
     Y_star, den = O.brute_force_left(sr, 4096, 2048, seq, V, X)
     for var in ("levels", "graph", "graph"):
         for tile in (0, 16):
-            _say("left hap", sr, B, var, tile)
-            Y = run_left(eng, sr, X, variant=gmr.VARIANTS[var], b_tile=tile)
-            check(sr, Y, Y_star, den, f"left sr={sr} B={B} {var} tile={tile}")
+            for flags in ((0, gmr.RUN_SINGLE_LIST) if B == 1 else (0,)):
+                _say("left hap", sr, B, var, tile, flags)
+                Y = run_left(eng, sr, X, variant=gmr.VARIANTS[var], b_tile=tile, flags=flags)
+                check(sr, Y, Y_star, den, f"left sr={sr} B={B} {var} tile={tile} flags={flags}")
 
 
 def test_rmat_left_semirings():
@@ -98,9 +99,10 @@ def test_rmat_left_semirings():
         y_star, _ = O.brute_force_left(sr, n, n, seq, V, x)
         eng = g.schedule(n, n, "resident").engine()
         for var in ("levels", "graph"):
-            _say("left rmat", sr, var)
-            y = run_left(eng, sr, x, variant=gmr.VARIANTS[var])
-            check(sr, y, y_star, None, f"rmat left sr={sr} {var}")
+            for flags in (0, gmr.RUN_SINGLE_LIST):
+                _say("left rmat", sr, var, flags)
+                y = run_left(eng, sr, x, variant=gmr.VARIANTS[var], flags=flags)
+                check(sr, y, y_star, None, f"rmat left sr={sr} {var} flags={flags}")
 
 
 def test_left_equals_right_product_of_the_transpose():

@@ -124,15 +124,27 @@ def test_batched_integer_semirings(hap, sr):
 
 
 def test_b_tiles_match_untiled(hap):
+    """Vector tiles: one launch sequence per tile (levels, graph) or every tile
+    in one persistent launch, including a narrower last tile and widths that
+    are not multiples of 4."""
     m, g, seq, V = hap
-    X = x_for(0, 2048, 77, B=64)
-    Y_star, den = O.brute_force_batched(0, 4096, 2048, seq, V, X)
     eng = g.schedule(4096, 2048, "resident").engine()
-    for bt in (8, 16, 24):
-        for var in ("levels", "persistent"):
-            _say("b_tile", bt, var)
-            Y = run_engine(eng, 0, X, variant=gmr.VARIANTS[var], emit=gmr.EMIT_GATHER, b_tile=bt)
-            check(0, Y, Y_star, den, f"b_tile={bt} {var}")
+    for B, tiles in ((64, (8, 16, 24)), (70, (16, 33))):
+        X = x_for(0, 2048, 77 + B, B=B)
+        Y_star, den = O.brute_force_batched(0, 4096, 2048, seq, V, X)
+        for bt in tiles:
+            for var in ("levels", "graph", "persistent"):
+                for em in ("gather", "spot"):
+                    _say("b_tile", B, bt, var, em)
+                    Y = run_engine(eng, 0, X, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em], b_tile=bt)
+                    check(0, Y, Y_star, den, f"B={B} b_tile={bt} {var}/{em}")
+    for sr in (1, 2):
+        X = x_for(sr, 2048, 5 + sr, B=20)
+        Y_star, _ = O.brute_force_batched(sr, 4096, 2048, seq, V, X)
+        for em in ("gather", "spot"):
+            _say("b_tile int", sr, em)
+            Y = run_engine(eng, sr, X, variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMITS[em], b_tile=8)
+            check(sr, Y, Y_star, None, f"sr={sr} b_tile=8 persistent/{em}")
 
 
 def test_graphs_replay_consistently(hap):
