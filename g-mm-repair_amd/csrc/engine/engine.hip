@@ -535,8 +535,12 @@ void run_tile(Engine& en, int semiring, SweepArgs a, const gmr_run_opts& o, hipS
     if (o.variant == GMR_VARIANT_PERSISTENT) {
         launch_persistent<S>(en, a, block, o.grid_cap, st);
     } else if (o.variant == GMR_VARIANT_GRAPH) {
-        auto key = std::make_tuple(semiring * 1000003 + kinds_hash(en, o), reinterpret_cast<uintptr_t>(a.X), reinterpret_cast<uintptr_t>(a.Y), a.B, a.Bt,
-                                   a.gather, block, reinterpret_cast<uintptr_t>(st));
+        // Everything the captured launches bake in: arguments, kernels per
+        // level, gather form and pass count.
+        auto key = std::make_tuple(semiring * 1000003 + kinds_hash(en, o), reinterpret_cast<uintptr_t>(a.X),
+                                   reinterpret_cast<uintptr_t>(a.Y), a.B, a.Bt,
+                                   a.gather | (o.flags << 1) | (o.xcd_passes << 8), block,
+                                   reinterpret_cast<uintptr_t>(st));
         auto it = en.graphs.find(key);
         if (it == en.graphs.end()) {
             // Capture on the engine's own stream (the caller's may be the
@@ -1259,6 +1263,10 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
             key = mix(key, reinterpret_cast<uintptr_t>(Y[g]));
             key = mix(key, static_cast<std::size_t>(B[g]) * 4 + static_cast<std::size_t>(o[g].emit));
             key = mix(key, static_cast<std::size_t>(kinds_hash(*bt.eng[g], o[g])));
+            key = mix(key, static_cast<std::size_t>(o[g].flags) * 16 + static_cast<std::size_t>(o[g].xcd_passes));
+            // an engine's workspace may have moved since the graph was captured
+            key = mix(key, reinterpret_cast<uintptr_t>(args[g].Tv));
+            key = mix(key, reinterpret_cast<uintptr_t>(args[g].W0));
         }
         auto it = bt.graphs.find(key);
         if (it == bt.graphs.end()) {
@@ -1305,6 +1313,7 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
         key = mix(key, reinterpret_cast<uintptr_t>(Y[g]));
         key = mix(key, static_cast<std::size_t>(B[g]) * 4 + static_cast<std::size_t>(o[g].emit));
         key = mix(key, static_cast<std::size_t>(kinds_hash(*bt.eng[g], o[g])));
+        key = mix(key, static_cast<std::size_t>(o[g].flags) * 16 + static_cast<std::size_t>(o[g].xcd_passes));
         key = mix(key, reinterpret_cast<uintptr_t>(args[g].Tv));
         key = mix(key, reinterpret_cast<uintptr_t>(args[g].W0));
     }
@@ -1500,7 +1509,8 @@ int gmr_engine_debug(gmr_engine* h, long long out[8]) {
         GMR_HIP(hipDeviceSynchronize());
         GMR_HIP(hipMemcpy(&b, h->e.d_barrier, sizeof(b), hipMemcpyDeviceToHost));
         long long v[8] = {static_cast<long long>(b.arrived), b.error, static_cast<long long>(h->e.barrier_base),
-                          h->e.wall_khz, h->e.num_cus, h->e.last_grid, h->e.last_per_cu, 0};
+                          h->e.wall_khz, h->e.num_cus, h->e.last_grid, h->e.last_per_cu,
+                          static_cast<long long>(h->e.graphs.size())};
         std::memcpy(out, v, sizeof(v));
     });
 }

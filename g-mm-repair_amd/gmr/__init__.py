@@ -321,7 +321,7 @@ class Engine:
     def debug(self):
         a = (C.c_longlong * 8)()
         _check(_lib.dev().gmr_engine_debug(self._h, a))
-        return dict(zip(["arrived", "error", "expected", "wall_khz", "cus", "grid", "per_cu"], list(a)))
+        return dict(zip(["arrived", "error", "expected", "wall_khz", "cus", "grid", "per_cu", "graphs"], list(a)))
 
     def run_torch(self, semiring, X, Y=None, **kw):
         """Convenience for torch device tensors: X float32 (cols, B) or (cols,)."""

@@ -122,8 +122,8 @@ int gmr_engine_traverse(gmr_engine* e, int semiring, void* d_x, int max_iters, i
 int gmr_engine_check(gmr_engine* e, int* barrier_error);
 /* Diagnostics of the persistent variant: out[0..6] = barrier arrivals on the
  * device, barrier error word, arrivals the host expects, wall-clock rate (kHz),
- * CU count, workgroups of the last persistent launch, workgroups per CU.
- * Synchronises the device. */
+ * CU count, workgroups of the last persistent launch, workgroups per CU,
+ * captured graphs held. Synchronises the device. */
 int gmr_engine_debug(gmr_engine* e, long long out[8]);
 /* Device time of the last gmr_engine_run on this engine, when timing was
  * requested with GMR_RUN_TIMED in opts->flags (that run synchronises its stream). */

@@ -147,6 +147,25 @@ def test_b_tiles_match_untiled(hap):
             check(sr, Y, Y_star, None, f"sr={sr} b_tile=8 persistent/{em}")
 
 
+def test_graph_cache_keys_on_the_gather_form(hap):
+    """A captured graph is reused only for the same schedule: another gather
+    form or pass count on the same buffers captures its own graph."""
+    m, g, seq, V = hap
+    eng = g.schedule(4096, 2048, "resident").engine()
+    x = x_for(0, 2048, 3)
+    y_star, den = O.brute_force(0, 4096, 2048, seq, V, x)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty(4096, device="cuda")
+    seen = []
+    for kw in ({}, {"xcd_passes": 1}, {"xcd_passes": 2}, {"flags": gmr.RUN_SINGLE_LIST},
+               {"flags": gmr.RUN_ROW_SEGMENTS}, {"xcd_passes": 1}):
+        eng.run_torch(0, xt, yt, variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER, **kw)
+        torch.cuda.synchronize()
+        check(0, yt.cpu().numpy(), y_star, den, f"graph {kw}")
+        seen.append(eng.debug()["graphs"])
+    assert seen == [1, 2, 3, 4, 5, 5], seen
+
+
 def test_graphs_replay_consistently(hap):
     """A captured graph replayed several times, with eager work between the
     replays, gives the eager result every time."""
