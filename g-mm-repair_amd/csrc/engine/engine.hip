@@ -449,7 +449,7 @@ inline unsigned eval_grid(const SweepArgs& a, int kind, long w, int block) {
         case dev::EV_THREAD: return grid_for(w * (a.vec ? a.Bs / 4 : a.Bs), block);
         case dev::EV_WAVE: return grid_for(w, block / 64);
         case dev::EV_LDS: return grid_for(w, block);
-        default: return grid_for(w, block / a.G);
+        default: return grid_for((a.vec || a.Bs == 1) ? (w + 3) / 4 : w, block / a.G);  // four rules per group
     }
 }
 

@@ -115,7 +115,8 @@ struct SweepArgs {
 
 // Per-level node-update kernels (one is chosen for each level):
 //   EV_GROUP   a group of G lanes per rule, lanes over vectors (dwordx4 when
-//              possible), two rules per group in flight
+//              possible); a group takes four rules at once, their kids first,
+//              then all eight child rows (eight loads in flight per lane)
 //   EV_THREAD  one lane per (rule, 4-vector chunk): no idle lanes for narrow B
 //   EV_WAVE    a whole wave per rule, lanes over single vectors
 //   EV_LDS     single vector only: a workgroup stages the window of the
@@ -255,40 +256,6 @@ __device__ void ph_eval(const SweepArgs& a, int i0, int i1, const typename S::T*
         const T* rp = c.y >= 0 ? src + static_cast<long>(c.y) * Bt : Tt + static_cast<long>(-c.y - 1) * Bt;
         T* o = dst + (i - i0) * Bt;
         for (int b = lane; b < Bt; b += G) o[b] = S::oplus(S::apply(lp[b], cf), rp[b]);
-    }
-}
-
-// Same, four vectors per lane; two rules per group per step so a wave keeps
-// four dwordx4 child gathers in flight.
-template <class S>
-__device__ void ph_eval_v4(const SweepArgs& a, int i0, int i1, const typename S::T* src,
-                           typename S::T* dst, long gid, long ng, int lane) {
-    using T = typename S::T;
-    using O = Op4<S>;
-    const T* Tt = tp<S>(a.Tv);
-    const int Bt = a.Bs, G = a.G;
-    auto row = [&](int c) -> const T* {
-        return c >= 0 ? src + static_cast<long>(c) * Bt : Tt + static_cast<long>(-c - 1) * Bt;
-    };
-    long i = i0 + gid;
-    for (; i + ng < i1; i += 2 * ng) {
-        const int2 c0 = a.kids[i], c1 = a.kids[i + ng];
-        const float f0 = a.coeff ? a.coeff[i] : 1.0f, f1 = a.coeff ? a.coeff[i + ng] : 1.0f;
-        const T *l0 = row(c0.x), *r0 = row(c0.y), *l1 = row(c1.x), *r1 = row(c1.y);
-        T* o0 = dst + (i - i0) * Bt;
-        T* o1 = dst + (i + ng - i0) * Bt;
-        for (int b = 4 * lane; b < Bt; b += 4 * G) {
-            auto x0 = O::ld(l0 + b), y0 = O::ld(r0 + b), x1 = O::ld(l1 + b), y1 = O::ld(r1 + b);
-            O::st(o0 + b, O::oplus(O::apply(x0, f0), y0));
-            O::st(o1 + b, O::oplus(O::apply(x1, f1), y1));
-        }
-    }
-    if (i < i1) {
-        const int2 c0 = a.kids[i];
-        const float f0 = a.coeff ? a.coeff[i] : 1.0f;
-        const T *l0 = row(c0.x), *r0 = row(c0.y);
-        T* o0 = dst + (i - i0) * Bt;
-        for (int b = 4 * lane; b < Bt; b += 4 * G) O::st(o0 + b, O::oplus(O::apply(O::ld(l0 + b), f0), O::ld(r0 + b)));
     }
 }
 
@@ -605,6 +572,81 @@ __device__ __forceinline__ void ph_gather(const SweepArgs& a, long tid, long nth
     ph_gather_range<S>(a, 0, gather_items(a), tid, nth);
 }
 
+// EV_GROUP, four vectors per lane: rules i, i + ng, ..., i + (U-1)ng per
+// step, all kids first, then all 2U child rows, then the U stores. The node
+// update is latency-bound (two dependent loads per rule), so four rules in
+// flight beat one (-31% at B = 64 on the 500,000 x 10,000 genotype grammar)
+// and eight (register pressure).
+template <class S, int U = 4>
+__device__ void ph_eval_v4x4(const SweepArgs& a, int i0, int i1, const typename S::T* src, typename S::T* dst,
+                             long gid, long ng, int lane) {
+    using T = typename S::T;
+    using O = Op4<S>;
+    using T4 = typename O::T4;
+    const T* Tt = tp<S>(a.Tv);
+    const int Bt = a.Bs, G = a.G;
+    auto row = [&](int c) -> const T* {
+        return c >= 0 ? src + static_cast<long>(c) * Bt : Tt + static_cast<long>(-c - 1) * Bt;
+    };
+    for (long i = i0 + gid; i < i1; i += U * ng) {
+        int2 c[U];
+        float f[U];
+        bool on[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long r = i + u * ng;
+            on[u] = r < i1;
+            c[u] = on[u] ? a.kids[r] : make_int2(-1, -1);
+            f[u] = (on[u] && a.coeff) ? a.coeff[r] : 1.0f;
+        }
+        for (int b = 4 * lane; b < Bt; b += 4 * G) {
+            T4 l[U], r[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (on[u]) {
+                    l[u] = O::ld(row(c[u].x) + b);
+                    r[u] = O::ld(row(c[u].y) + b);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (on[u]) O::st(dst + (i + u * ng - i0) * Bt + b, O::oplus(O::apply(l[u], f[u]), r[u]));
+        }
+    }
+}
+
+// EV_GROUP, one vector: U rules per lane at once (rules i, i + n, ...,
+// i + (U-1)n), all kids first, then all 2U child values, then the U stores.
+template <class S, int U>
+__device__ void ph_eval_1xU(const SweepArgs& a, int i0, int i1, const typename S::T* src, typename S::T* dst,
+                            long tid, long nth) {
+    using T = typename S::T;
+    const T* Tt = tp<S>(a.Tv);
+    for (long i = i0 + tid; i < i1; i += U * nth) {
+        int2 c[U];
+        float f[U];
+        bool on[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long r = i + u * nth;
+            on[u] = r < i1;
+            c[u] = on[u] ? a.kids[r] : make_int2(-1, -1);
+            f[u] = (on[u] && a.coeff) ? a.coeff[r] : 1.0f;
+        }
+        T l[U], rv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (on[u]) {
+                l[u] = c[u].x >= 0 ? src[c[u].x] : Tt[-c[u].x - 1];
+                rv[u] = c[u].y >= 0 ? src[c[u].y] : Tt[-c[u].y - 1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (on[u]) dst[i + u * nth - i0] = S::oplus(S::apply(l[u], f[u]), rv[u]);
+    }
+}
+
 // EV_THREAD: flattened (rule, chunk) items, chunk = 4 vectors (vec) or 1.
 template <class S>
 __device__ void ph_eval_thread(const SweepArgs& a, int i0, int i1, const typename S::T* src,
@@ -714,7 +756,8 @@ __device__ __forceinline__ void ph_eval_kind(const SweepArgs& a, int kind, int i
         case EV_WAVE: ph_eval_wave<S>(a, i0, i1, src, dst, tid / 64, nth / 64, static_cast<int>(tid & 63)); break;
         case EV_LDS: ph_eval_lds<S>(a, i0, i1, src, dst, tid / blockDim.x, nth / blockDim.x); break;
         default:
-            if (a.vec) ph_eval_v4<S>(a, i0, i1, src, dst, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+            if (a.vec) ph_eval_v4x4<S, 4>(a, i0, i1, src, dst, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
+            else if (a.Bs == 1) ph_eval_1xU<S, 4>(a, i0, i1, src, dst, tid, nth);
             else ph_eval<S>(a, i0, i1, src, dst, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
     }
 }

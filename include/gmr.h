@@ -45,7 +45,7 @@ enum gmr_emit {
 
 /* Node-update kernel of a level. */
 enum gmr_eval_kind {
-    GMR_EVAL_GROUP = 0,   /* a lane group per rule, lanes over vectors (dwordx4) */
+    GMR_EVAL_GROUP = 0,   /* a lane group per rule, lanes over vectors (dwordx4), four rules in flight */
     GMR_EVAL_THREAD = 1,  /* one lane per (rule, 4-vector chunk) */
     GMR_EVAL_WAVE = 2,    /* one wave per rule */
     GMR_EVAL_LDS = 3      /* single vector: left-child window staged in LDS */
