@@ -490,8 +490,10 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
             ap.xpass1 = pass + 1;
             long most = 0;
             for (int q = 0; q < 8; ++q) most = std::max<long>(most, poff[8 * pass + q + 1] - poff[8 * pass + q]);
-            if (most > 0)
-                hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(most * 8 * 64, block)), dim3(block), 0, st, ap);
+            if (most > 0)  // kPartChunks chunks per wave
+                hipLaunchKernelGGL(dev::k_gather<S>,
+                                   dim3(grid_for((most + dev::kPartChunks - 1) / dev::kPartChunks * 8 * 64, block)),
+                                   dim3(block), 0, st, ap);
         }
     } else if (a.gather) {
         const long threads = a.flat ? static_cast<long>(en.n_fchunk) * 64
@@ -985,8 +987,10 @@ void launch_left(Engine& en, const dev::LeftArgs& a, int block, hipStream_t st) 
                     const int* po = en.lxpoff.data() + static_cast<long>(gi) * (P + 1);
                     long most = 0;
                     for (int q = 0; q < 8; ++q) most = std::max<long>(most, po[8 * pass + q + 1] - po[8 * pass + q]);
-                    if (most > 0)
-                        hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(most * 8 * 64, block)), dim3(block), 0, st, gp);
+                    if (most > 0)  // kPartChunks chunks per wave
+                        hipLaunchKernelGGL(dev::k_gather<S>,
+                                           dim3(grid_for((most + dev::kPartChunks - 1) / dev::kPartChunks * 8 * 64, block)),
+                                           dim3(block), 0, st, gp);
                 }
                 return;
             }
