@@ -496,9 +496,10 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
                                    dim3(block), 0, st, ap);
         }
     } else if (a.gather) {
+        // (the B-wide row gather takes two segments per lane group at once)
         const long threads = a.flat ? static_cast<long>(en.n_fchunk) * 64
                              : (a.Bt == 1 && a.B == 1) ? static_cast<long>(en.n_gseg) * a.GR
-                                                       : static_cast<long>(en.n_gseg) * G;
+                                                       : (static_cast<long>(en.n_gseg) + 1) / 2 * G;
         hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for(threads, block)), dim3(block), 0, st, a);
     }
     if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * en.L + 1], st));
@@ -1002,7 +1003,7 @@ void launch_left(Engine& en, const dev::LeftArgs& a, int block, hipStream_t st) 
             } else {
                 r0 = en.lseg_off[gi];
                 r1 = en.lseg_off[gi + 1];
-                threads = (r1 - r0) * g.G;
+                threads = (r1 - r0 + 1) / 2 * g.G;  // two segments per lane group
             }
             if (r1 > r0)
                 hipLaunchKernelGGL(dev::k_gather_range<S>, dim3(grid_for(threads, block)), dim3(block), 0, st, g, r0, r1);

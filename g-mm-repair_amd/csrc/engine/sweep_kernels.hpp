@@ -408,8 +408,12 @@ __device__ void ph_gather_rows_v4(const SweepArgs& a, long q0, long q1, long gid
     auto row = [&](int c) -> const T* {
         return c >= 0 ? W + static_cast<long>(c) * Bs : Tt + static_cast<long>(-c - 1) * Bs;
     };
-    for (long q = q0 + gid; q < q1; q += ng) {
-        const int4 sg = a.gseg[q];
+    auto store = [&](const int4& sg, int b, int nb, const T4& v) {
+        T* y = Y + static_cast<long>(sg.x) * a.B + b;
+        if (!sg.w) O::st_n(y, v, nb, a.yvec);
+        else O::emit_n(y, v, nb);
+    };
+    auto one = [&](const int4& sg) {
         for (int b = 4 * lane; b < Bs; b += 4 * a.G) {
             const int nb = min(4, Bt - b);
             if (nb <= 0) break;
@@ -432,10 +436,43 @@ __device__ void ph_gather_rows_v4(const SweepArgs& a, long q0, long q1, long gid
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (j + u < sg.z) acc0 = O::oplus(acc0, O::ld(row(c[u]) + b));
-            T* y = Y + static_cast<long>(sg.x) * a.B + b;
-            const T4 v = O::oplus(acc0, acc1);
-            if (!sg.w) O::st_n(y, v, nb, a.yvec);
-            else O::emit_n(y, v, nb);
+            store(sg, b, nb, O::oplus(acc0, acc1));
+        }
+    };
+    // Two segments of at most four symbols each: their eight reads together.
+    auto two_short = [&](const int4& s1, const int4& s2) {
+        constexpr int H = U / 2;
+        int c[U];
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            c[u] = s1.y + u < s1.z ? a.csym[s1.y + u] : 0;
+            c[H + u] = s2.y + u < s2.z ? a.csym[s2.y + u] : 0;
+        }
+        for (int b = 4 * lane; b < Bs; b += 4 * a.G) {
+            const int nb = min(4, Bt - b);
+            if (nb <= 0) break;
+            T4 v[U];
+#pragma unroll
+            for (int u = 0; u < H; ++u) {
+                v[u] = s1.y + u < s1.z ? O::ld(row(c[u]) + b) : O::zero();
+                v[H + u] = s2.y + u < s2.z ? O::ld(row(c[H + u]) + b) : O::zero();
+            }
+            store(s1, b, nb, O::oplus(O::oplus(v[0], v[1]), O::oplus(v[2], v[3])));
+            store(s2, b, nb, O::oplus(O::oplus(v[4], v[5]), O::oplus(v[6], v[7])));
+        }
+    };
+    for (long q = q0 + gid; q < q1; q += 2 * ng) {
+        const int4 s1 = a.gseg[q];
+        if (q + ng < q1) {
+            const int4 s2 = a.gseg[q + ng];
+            if (s1.z - s1.y <= U / 2 && s2.z - s2.y <= U / 2) {
+                two_short(s1, s2);
+            } else {
+                one(s1);
+                one(s2);
+            }
+        } else {
+            one(s1);
         }
     }
 }

@@ -296,13 +296,16 @@ class Proposer:
     @classmethod
     def default(cls):
         """The network shipped with the package: trained on MI355X sweep
-        timings of haplotype and genotype grammars, including members generated
-        with weak-compression parameters (gmr.train_proposer --vary), at B in
-        {1, 3, 4, 16, 29, 64, 100, 256}."""
+        timings of haplotype and genotype grammars (including members generated
+        with weak-compression parameters) and R-MAT graphs
+        (gmr.train_proposer --vary --graphs), at B in {1, 3, 4, 16, 29, 64,
+        100, 256}. CFG123_WEIGHTS holds the network trained on the haplotype
+        and genotype families alone (--vary)."""
         return cls.load(DEFAULT_WEIGHTS)
 
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(__file__), "proposer_mi355x.npz")
+CFG123_WEIGHTS = os.path.join(os.path.dirname(__file__), "proposer_mi355x_cfg123.npz")
 
 
 def run_options(choice):

@@ -52,15 +52,16 @@ def test_features_shape_and_tiles():
 
 
 def test_shipped_weights_match_the_network():
-    """The trained network shipped with the package has one parameter per
+    """The trained networks shipped with the package have one parameter per
     weight and bias of the 12 -> 64 -> 64 -> 10 MLP and finite standardisation."""
-    with np.load(P.DEFAULT_WEIGHTS) as f:
-        p, mu, sd = f["params"], f["mu"], f["sd"]
-        tsd = float(f["tsd"])
-    H, I, O = P.HIDDEN, P.N_FEAT, P.N_OUT
-    assert p.shape == (I * H + H + H * H + H + H * O + O,)
-    assert mu.shape == sd.shape == (I,)
-    assert np.isfinite(p).all() and (sd > 0).all() and tsd > 0
+    for path in (P.DEFAULT_WEIGHTS, P.CFG123_WEIGHTS):
+        with np.load(path) as f:
+            p, mu, sd = f["params"], f["mu"], f["sd"]
+            tsd = float(f["tsd"])
+        H, I, O = P.HIDDEN, P.N_FEAT, P.N_OUT
+        assert p.shape == (I * H + H + H * H + H + H * O + O,)
+        assert mu.shape == sd.shape == (I,)
+        assert np.isfinite(p).all() and (sd > 0).all() and tsd > 0
 
 
 def test_run_options_from_a_choice():
