@@ -482,3 +482,24 @@ def test_single_vector_gathers_agree(sr):
             batch.check()
         for Yb in Ys:
             check(sr, Yb.cpu().numpy()[:, 0], y_star, den, f"batch {var} xcd parts")
+
+
+def test_wide_batches_loop_over_vectors():
+    """B wider than one lane group covers (Bs > 256: every lane takes several
+    4-vector chunks) in the node update, both gathers and the left product."""
+    m = gmr.Matrix.haplotype(41, 600, 300, founders=12, seg_mean=24.0)
+    g = m.repair()
+    seq, V = m.arrays()
+    eng = g.schedule(600, 300, "resident").engine()
+    for B in (520, 1030):
+        X = x_for(0, 300, B, B=B)
+        Y_star, den = O.brute_force_batched(0, 600, 300, seq, V, X)
+        for var in ("levels", "graph", "persistent"):
+            for em in ("gather", "spot"):
+                _say("wide B", B, var, em)
+                Y = run_engine(eng, 0, X, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+                check(0, Y, Y_star, den, f"B={B} {var}/{em}")
+        XL = x_for(0, 600, B + 1, B=B)
+        YL_star, denL = O.brute_force_left(0, 600, 300, seq, V, XL)
+        YL = eng.run_left_torch(0, torch.from_numpy(XL).cuda()).cpu().numpy()
+        check(0, YL, YL_star, denL, f"left B={B}")
