@@ -960,6 +960,9 @@ SweepArgs left_gather_args(Engine& en, int Bt, int Bs) {
     return g;
 }
 
+// Groups of at least this many 64-entry chunks use the XCD-partitioned lists.
+constexpr long kLeftPartMinChunks = 4096;
+
 template <class S>
 void launch_left(Engine& en, const dev::LeftArgs& a, int block, hipStream_t st) {
     const bool fast = !en.has_runlen;
@@ -973,7 +976,10 @@ void launch_left(Engine& en, const dev::LeftArgs& a, int block, hipStream_t st) 
         const int P = 8 * en.lx_passes;
         // group gi = level gi + 1 for gi < L, the terminals for gi = L
         auto group = [&](int gi) {
-            if (g.flat && a.xparts) {
+            // Small groups (the top levels) are not worth a launch per pass:
+            // they take the single chunked list.
+            const long chunks = static_cast<long>(en.lchunk_off[gi + 1]) - en.lchunk_off[gi];
+            if (g.flat && a.xparts && chunks >= kLeftPartMinChunks) {
                 // one launch per pass, as in the right product's gather
                 for (int pass = 0; pass < en.lx_passes; ++pass) {
                     SweepArgs gp = g;

@@ -128,3 +128,20 @@ def test_left_product_errors():
         eng.run_left_torch(0, X, variant=gmr.VARIANT_PERSISTENT)
     with pytest.raises(ValueError):
         eng.run_left_torch(0, torch.zeros(32, device="cuda"))
+
+
+def test_left_partitioned_and_single_lists_on_a_larger_grammar():
+    """A grammar whose large node groups take the XCD-partitioned lists (and
+    the small ones the single list) for one vector, against the oracle."""
+    m = gmr.Matrix.genotype(515, 6000, 10000)
+    g = m.repair()
+    seq, V = m.arrays()
+    eng = g.schedule(6000, 10000, "resident").engine()
+    for sr in (0, 1, 2):
+        x = x_for(sr, 6000, 60 + sr)
+        y_star, den = O.brute_force_left(sr, 6000, 10000, seq, V, x)
+        for var in ("levels", "graph"):
+            for flags in (0, gmr.RUN_SINGLE_LIST):
+                _say("left geno", sr, var, flags)
+                y = run_left(eng, sr, x, variant=gmr.VARIANTS[var], flags=flags)
+                check(sr, y, y_star, den if sr == 0 else None, f"left geno sr={sr} {var} flags={flags}")
