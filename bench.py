@@ -98,6 +98,10 @@ def main():
                     help="run a 1 GiB device-to-device copy first (known bytes, to scale HBM counters)")
     ap.add_argument("--layout", default="resident", choices=["resident", "wave64", "refcompat"],
                     help="schedule layout of the timed engine")
+    ap.add_argument("--inline-top", default="auto",
+                    help="grammar form: 0 = as built; N = rules no rule uses and C uses at most N times "
+                         "written out in C (Grammar.inline_top); auto = tune over 0 and 2 (untuned runs: "
+                         "2 for B >= 32, else 0)")
     ap.add_argument("--grammar-cache", default=None,
                     help="directory to save/load the RePair grammar files between runs")
     args = ap.parse_args()
@@ -150,11 +154,12 @@ def main():
     log("schedules", f"{sched_s:.1f}s", "resident", info, "refcompat", info_ref)
     eng = s_res.engine()
     log("engine uploaded", eng.info())
-
-    if args.sweep_B:
-        return run_sweep_B(args, m, g, s_res, eng)
     if args.left_B:
         return run_left_B(args, m, s_res, eng)
+    g0 = g
+    forms = grammar_forms(args, g, s_res, eng, log)
+    if args.sweep_B:
+        return run_sweep_B(args, m, forms)
 
     gen = torch.Generator(device="cuda").manual_seed(args.seed)
     X = torch.randn(args.cols, B, device="cuda", generator=gen, dtype=torch.float32)
@@ -162,14 +167,18 @@ def main():
     if B == 1:
         X, Y = X[:, 0].contiguous(), Y[:, 0].contiguous()
 
-    # ---- schedule choice
+    # ---- schedule choice (grammar form, variant, emission, tile, launch shape)
     if args.no_tune or args.variant:
+        form = untuned_form(args, forms, B)
         best = dict(variant=gmr.VARIANTS[args.variant or "persistent"], emit=gmr.EMITS[args.emit or "gather"],
                     b_tile=args.b_tile or 0, block=256)
         tuned = []
     else:
-        cands = autotune.candidates(B, gmr.LAYOUTS[args.layout])
-        best, tuned = autotune.tune(eng, 0, X, Y, cands, reps=2, log=log)
+        form, best, tuned = tune_forms(forms, X, Y, autotune.candidates(B, gmr.LAYOUTS[args.layout]), reps=2,
+                                       log=log)
+    g, s_res, eng = form["grammar"], form["schedule"], form["engine"]
+    info = s_res.info()
+    log("grammar form:", form["label"])
     log("schedule:", autotune.describe(best))
     run_kw = autotune.apply(eng, best)
 
@@ -271,8 +280,8 @@ def main():
     plain_val = nnz / (plain_ms * 1e-3)
     log(f"plain per-level sweep B=1: {plain_ms:.3f} ms, {plain_val / 1e9:.2f} G nnz/s")
     # best single-vector schedule, for reference
-    best1, _ = autotune.tune(eng, 0, x1, y1, autotune.candidates(1, gmr.RESIDENT), reps=3)
-    one_ms = autotune.time_candidate(eng, 0, x1, y1, best1, reps=args.steps)
+    form1, best1, _ = tune_forms(forms, x1, y1, autotune.candidates(1, gmr.RESIDENT), reps=3)
+    one_ms = autotune.time_candidate(form1["engine"], 0, x1, y1, best1, reps=args.steps)
     del eng_ref
 
     # ---- roofline of the dominant kernel: k_gather (about 70% of the sweep's
@@ -338,7 +347,8 @@ def main():
                    "grammar": {"rules": info["num_raw_nt"], "C": g.info()["clen"], "depth": info["max_depth"],
                                "emissions": info["emit_total"], "term_emissions": info["num_term_emit"],
                                "U": info["U"], "refcompat_passthrough": info_ref["num_passthrough"]},
-                   "schedule": autotune.describe(best), "repair_s": round(g.build_seconds, 1),
+                   "grammar_form": form["label"],
+                   "schedule": autotune.describe(best), "repair_s": round(g0.build_seconds, 1),
                    "schedule_build_s": round(sched_s, 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -347,11 +357,53 @@ def main():
         "batched_over_plain": value / plain_val,
         "pcie_inclusive": {"ms": pcie_ms, "value": nnz * B / (pcie_ms * 1e-3), "unit": "nnz*vectors/s",
                            "note": "X copied in from and Y copied out to pinned host memory inside every step"},
-        "best_single_vector": {"value": nnz / (one_ms * 1e-3), "ms": one_ms, "schedule": autotune.describe(best1)},
+        "best_single_vector": {"value": nnz / (one_ms * 1e-3), "ms": one_ms, "schedule": autotune.describe(best1),
+                               "grammar_form": form1["label"]},
         "fp32_err": err,
-        "tuned": [(autotune.describe(c), round(ms, 4)) for c, ms in tuned],
+        "tuned": [(lab + " " + autotune.describe(c), round(ms, 4)) for lab, c, ms in tuned],
     }
     print(json.dumps(out), flush=True)
+
+
+def grammar_forms(args, g, s_res, eng, log):
+    """The grammar forms to run: as built and/or with its top rules written
+    out in C (Grammar.inline_top), each with its Resident schedule and engine."""
+    forms = []
+    it = str(args.inline_top)
+    if it in ("auto", "0"):
+        forms.append({"label": "as built", "max_emits": 0, "grammar": g, "schedule": s_res, "engine": eng})
+    if it != "0":
+        n = 2 if it == "auto" else int(it)
+        t = time.time()
+        g2 = g.inline_top(n)
+        s2 = g2.schedule(args.rows, args.cols, args.layout)
+        e2 = s2.engine()
+        lab = f"top rules used <= {n} times in C inlined ({g2.n_inlined} of {g.info()['nrules']} rules)"
+        log(f"grammar form: {lab}, {time.time() - t:.1f}s", s2.info())
+        forms.append({"label": lab, "max_emits": n, "grammar": g2, "schedule": s2, "engine": e2})
+    return forms
+
+
+def untuned_form(args, forms, B):
+    """Without tuning, 'auto' takes the inlined form from B = 32 on (where the
+    node updates it saves outweigh the longer row gather) and the grammar as
+    built below."""
+    if len(forms) == 1:
+        return forms[0]
+    return forms[1] if B >= 32 else forms[0]
+
+
+def tune_forms(forms, X, Y, cands, reps=2, log=None):
+    """Autotune every grammar form; return (form, best candidate,
+    [(form label, cand, ms), ...]) for the fastest."""
+    from gmr import autotune
+    runs = []
+    for f in forms:
+        best, res = autotune.tune(f["engine"], 0, X, Y, cands, reps=reps, log=log)
+        runs.append((min(ms for _, ms in res), f, best, res))
+    _, form, best, _ = min(runs, key=lambda r: r[0])
+    tuned = [(f["label"].split(" (")[0], c, ms) for _, f, _, res in runs for c, ms in res]
+    return form, best, tuned
 
 
 def _time_steps(eng, semiring, X, Y, cand, steps, warmup):
@@ -371,9 +423,10 @@ def _time_steps(eng, semiring, X, Y, cand, steps, warmup):
     return a.elapsed_time(b) / steps
 
 
-def run_sweep_B(args, m, g, s_res, eng):
-    """Throughput per vector count B on one matrix (each B autotuned), with the
-    FP32 error of two columns against the FP64 oracle."""
+def run_sweep_B(args, m, forms):
+    """Throughput per vector count B on one matrix (each B autotuned over the
+    grammar forms and schedules), with the FP32 error of two columns against
+    the FP64 oracle."""
     import numpy as np
     import torch
 
@@ -382,7 +435,7 @@ def run_sweep_B(args, m, g, s_res, eng):
     from oracle import oracle as O
     seq, V = m.arrays()
     nnz = m.nnz()
-    info = s_res.info()
+    info = forms[0]["schedule"].info()
     out = []
     for B in [int(b) for b in args.sweep_B.split(",")]:
         gen = torch.Generator(device="cuda").manual_seed(args.seed + B)
@@ -390,7 +443,8 @@ def run_sweep_B(args, m, g, s_res, eng):
         Y = torch.empty(args.rows, B, device="cuda")
         if B == 1:
             X, Y = X[:, 0].contiguous(), Y[:, 0].contiguous()
-        best, tuned = autotune.tune(eng, 0, X, Y, autotune.candidates(B, gmr.RESIDENT), reps=2)
+        form, best, tuned = tune_forms(forms, X, Y, autotune.candidates(B, gmr.RESIDENT), reps=2)
+        eng = form["engine"]
         ms = _time_steps(eng, 0, X, Y, best, args.steps, args.warmup)
         if best["variant"] == gmr.VARIANT_PERSISTENT:
             eng.check()
@@ -400,9 +454,10 @@ def run_sweep_B(args, m, g, s_res, eng):
         Yg = Y.reshape(args.rows, -1)[:, cols].cpu().numpy().astype(np.float64)
         d = np.abs(Yg - Ys)
         err = float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1), np.where(d == 0, 0, np.inf))))
-        r = {"B": B, "ms": ms, "nnz_vec_per_s": nnz * B / (ms * 1e-3), "gbs_dag": dag_bytes(info, B) / ms / 1e6,
-             "schedule": autotune.describe(best), "fp32_err": err,
-             "tuned": [(autotune.describe(c), round(t, 4)) for c, t in tuned]}
+        r = {"B": B, "ms": ms, "nnz_vec_per_s": nnz * B / (ms * 1e-3),
+             "gbs_dag": dag_bytes(form["schedule"].info(), B) / ms / 1e6,
+             "grammar_form": form["label"], "schedule": autotune.describe(best), "fp32_err": err,
+             "tuned": [(lab + " " + autotune.describe(c), round(t, 4)) for lab, c, t in tuned]}
         log("B", B, r)
         out.append(r)
         del X, Y

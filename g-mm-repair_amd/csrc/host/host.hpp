@@ -63,6 +63,11 @@ grammar::Grammar repair(const Csrv& m, const RepairOptions& opt, RepairStats* st
 // Expand the grammar back into its CSRV string.
 std::vector<int> expand_grammar(const grammar::Grammar& g);
 
+// The same matrix with every binary rule that no rule references and that C
+// uses 1..max_emits times written out in C as its two children; the other
+// rules keep their order and are renumbered. max_emits <= 0 copies.
+grammar::Grammar inline_top_rules(const grammar::Grammar& g, int max_emits, long long* n_inlined);
+
 // Files: <base>.val float64[nvals]; <base>.vc.R int32 alpha then (left,right)
 // int32 pairs; <base>.vc.C int32 top sequence. Binary rules only, as RePair
 // writes them. Throws std::runtime_error on missing or truncated files.

@@ -227,6 +227,22 @@ int gmr_grammar_expand(const gmr_grammar* g, int* out, long long cap, long long*
     });
 }
 
+int gmr_grammar_inline_top(const gmr_grammar* g, int max_emits, gmr_grammar** out,
+                           long long* n_inlined) {
+    if (!g || !out) return fail(GMR_ERR_INVALID, "NULL argument");
+    return guarded([&] {
+        auto* h = new gmr_grammar;
+        try {
+            h->g = gmr::inline_top_rules(g->g, max_emits, n_inlined);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        h->V = g->V;
+        *out = h;
+    });
+}
+
 void gmr_grammar_free(gmr_grammar* g) { delete g; }
 
 int gmr_schedule_build(const gmr_grammar* g, int rows, int cols, int layout, gmr_schedule** out) {

@@ -189,6 +189,18 @@ class Grammar:
         _check(_lib.host().gmr_grammar_expand(self._h, _ptr(out, C.c_int), n.value, C.byref(n)))
         return out
 
+    def inline_top(self, max_emits=2):
+        """The same matrix with every binary rule that no rule references and
+        that C uses at most max_emits times written out in C as its two
+        children (fewer node updates for the same row-gather reads: faster
+        sweeps over many vectors). .n_inlined = rules removed."""
+        h = C.c_void_p()
+        n = C.c_longlong()
+        _check(_lib.host().gmr_grammar_inline_top(self._h, int(max_emits), C.byref(h), C.byref(n)))
+        g = Grammar(h)
+        g.n_inlined = n.value
+        return g
+
     def schedule(self, rows, cols, layout=REFCOMPAT):
         if isinstance(layout, str):
             layout = LAYOUTS[layout]

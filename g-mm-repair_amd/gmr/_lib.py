@@ -79,6 +79,7 @@ def host():
     _sig(L, "gmr_grammar_info", C.c_int, [vp, c_int_p, c_int_p, c_ll_p, c_int_p])
     _sig(L, "gmr_grammar_copy", C.c_int, [vp, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_double_p])
     _sig(L, "gmr_grammar_expand", C.c_int, [vp, c_int_p, C.c_longlong, c_ll_p])
+    _sig(L, "gmr_grammar_inline_top", C.c_int, [vp, C.c_int, C.POINTER(vp), c_ll_p])
     _sig(L, "gmr_grammar_free", None, [vp])
     _sig(L, "gmr_schedule_build", C.c_int, [vp, C.c_int, C.c_int, C.c_int, vpp])
     _sig(L, "gmr_schedule_info", C.c_int, [vp, c_ll_p])

@@ -78,6 +78,15 @@ int gmr_grammar_copy(const gmr_grammar* g, int* kind, int* left, int* right, int
                      double* V);
 /* Expand into the CSRV string: out may be NULL to query *len; cap = capacity. */
 int gmr_grammar_expand(const gmr_grammar* g, int* out, long long cap, long long* len);
+/* A new grammar of the same matrix in which every binary rule that no rule
+ * references and that C uses at most max_emits times is replaced, at each of
+ * its occurrences in C, by its two children (surviving rules keep their order
+ * and are renumbered). With max_emits = 2 the stored size does not grow and a
+ * sweep over many vectors does fewer node updates for the same row-gather
+ * reads. No reference counterpart: a rewrite of this builder's output that the
+ * sweep benefits from; *n_inlined (may be NULL) = rules removed. */
+int gmr_grammar_inline_top(const gmr_grammar* g, int max_emits, gmr_grammar** out,
+                           long long* n_inlined);
 void gmr_grammar_free(gmr_grammar* g);
 
 /* ---- schedules */

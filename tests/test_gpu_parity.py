@@ -503,3 +503,25 @@ def test_wide_batches_loop_over_vectors():
         YL_star, denL = O.brute_force_left(0, 600, 300, seq, V, XL)
         YL = eng.run_left_torch(0, torch.from_numpy(XL).cuda()).cpu().numpy()
         check(0, YL, YL_star, denL, f"left B={B}")
+
+
+@pytest.mark.parametrize("sr", [0, 1, 2])
+def test_inlined_top_rules(hap, sr):
+    """The grammar with its once- and twice-used top rules written out in C
+    (Grammar.inline_top) gives the same products in every variant, single and
+    batched, and the same left product."""
+    m, g, seq, V = hap
+    g2 = g.inline_top(2)
+    assert g2.n_inlined > 0
+    eng = g2.schedule(4096, 2048, "resident").engine()
+    for B in (1, 64):
+        X = x_for(sr, 2048, 70 + B, B=None if B == 1 else B)
+        Y_star, den = (O.brute_force if B == 1 else O.brute_force_batched)(sr, 4096, 2048, seq, V, X)
+        for var, em in RES_CASES:
+            _say("inlined", sr, B, var, em)
+            Y = run_engine(eng, sr, X, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+            check(sr, Y, Y_star, den if sr == 0 else None, f"inlined sr={sr} B={B} {var}/{em}")
+    XL = x_for(sr, 4096, 90)
+    YL_star, denL = O.brute_force_left(sr, 4096, 2048, seq, V, XL)
+    YL = eng.run_left_torch(sr, torch.from_numpy(XL).cuda()).cpu().numpy()
+    check(sr, YL, YL_star, denL if sr == 0 else None, f"inlined left sr={sr}")

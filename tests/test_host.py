@@ -223,3 +223,66 @@ def test_empty_and_degenerate_grammars():
     m = gmr.Matrix.from_dense(np.zeros((4, 5)))
     g = m.repair()
     assert g.info()["nrules"] == 0 and g.schedule(4, 5).info()["num_term_emit"] == 0
+
+
+def _refs_and_emits(ga):
+    alpha, n = ga["alpha"], len(ga["left"])
+    refs, emits = np.zeros(n, np.int64), np.zeros(n, np.int64)
+    for side in ("left", "right"):
+        s = ga[side] if side == "left" else np.where(ga["kind"] == 0, ga["right"], 0)
+        np.add.at(refs, s[s >= alpha] - alpha, 1)
+    C = ga["C"]
+    np.add.at(emits, C[C >= alpha] - alpha, 1)
+    return refs, emits
+
+
+def test_inline_top_known_answers():
+    # 4 -> 1 2 is referenced by 5 -> 4 3, which C uses once: only 5 goes.
+    g = gmr.Grammar.from_arrays(4, [1, 4], [2, 3], [5, 0, 4, 3, 0], [1.0])
+    g2 = g.inline_top(2)
+    a = g2.arrays()
+    assert g2.n_inlined == 1
+    assert list(a["left"]) == [1] and list(a["right"]) == [2] and list(a["C"]) == [4, 3, 0, 4, 3, 0]
+    # max_emits = 0 copies; a rule used three times stays at max_emits = 2
+    assert list(g.inline_top(0).arrays()["C"]) == [5, 0, 4, 3, 0]
+    g = gmr.Grammar.from_arrays(3, [1], [2], [3, 3, 0, 3, 0], [1.0])
+    assert g.inline_top(2).n_inlined == 0 and g.inline_top(3).arrays()["C"].tolist() == [1, 2, 1, 2, 0, 1, 2, 0]
+    # the reference's unit grammar: the top rule 8 -> 7 5 goes, the run-length
+    # rule 7 = 6^3 (referenced by 8) stays with its coefficient
+    f = fixture("unit_grammar")
+    g = gmr.Grammar.from_arrays(f["alpha"], f["left"], f["right"], f["C"], f["V"], kind=f["kind"], rep=f["rep"])
+    g2 = g.inline_top(2)
+    a = g2.arrays()
+    assert g2.n_inlined == 1 and list(a["kind"]) == [0, 0, 1] and list(a["rep"]) == [1, 1, 3]
+    assert list(a["C"][:2]) == [7, 5]
+    assert np.array_equal(g2.expand(), g.expand())
+    y = O.sweep(0, g2.schedule(f["rows"], f["cols"], "refcompat").arrays(), f["rows"], f["cols"],
+                np.array(f["V"]), np.array(f["x"], np.float32))
+    np.testing.assert_allclose(y, f["expect_y"], atol=1e-12)
+
+
+@pytest.mark.parametrize("name,m", list(_small_matrices()), ids=[n for n, _ in _small_matrices()])
+def test_inline_top_keeps_the_matrix(name, m):
+    seq, V = m.arrays()
+    g = m.repair()
+    ga = g.arrays()
+    refs, emits = _refs_and_emits(ga)
+    for e in (1, 2, 3):
+        g2 = g.inline_top(e)
+        a = g2.arrays()
+        want = int(((refs == 0) & (emits >= 1) & (emits <= e) & (ga["kind"] == 0)).sum())
+        assert g2.n_inlined == want and len(a["left"]) == len(ga["left"]) - want
+        assert np.array_equal(g2.expand(), seq)
+        idx = np.arange(len(a["left"])) + a["alpha"]
+        assert np.all(a["left"] < idx) and np.all(a["right"] < idx)
+        size = lambda x: 2 * len(x["left"]) + len(x["C"])
+        if e <= 2:
+            assert size(a) <= size(ga)
+        for sr in (0, 1, 2):
+            x = x_for(sr, m.cols, 11 + e)
+            y_bf, den = O.brute_force(sr, m.rows, m.cols, seq, V, x)
+            y = O.sweep(sr, g2.schedule(m.rows, m.cols, "refcompat").arrays(), m.rows, m.cols, V, x)
+            if sr == 0:
+                np.testing.assert_allclose(y, y_bf, rtol=1e-9, atol=1e-9 * (1 + den.max()))
+            else:
+                np.testing.assert_array_equal(y, y_bf)
