@@ -154,10 +154,10 @@ def main():
     log("schedules", f"{sched_s:.1f}s", "resident", info, "refcompat", info_ref)
     eng = s_res.engine()
     log("engine uploaded", eng.info())
-    if args.left_B:
-        return run_left_B(args, m, s_res, eng)
     g0 = g
     forms = grammar_forms(args, g, s_res, eng, log)
+    if args.left_B:
+        return run_left_B(args, m, forms)
     if args.sweep_B:
         return run_sweep_B(args, m, forms)
 
@@ -393,13 +393,13 @@ def untuned_form(args, forms, B):
     return forms[1] if B >= 32 else forms[0]
 
 
-def tune_forms(forms, X, Y, cands, reps=2, log=None):
+def tune_forms(forms, X, Y, cands, reps=2, log=None, semiring=0):
     """Autotune every grammar form; return (form, best candidate,
     [(form label, cand, ms), ...]) for the fastest."""
     from gmr import autotune
     runs = []
     for f in forms:
-        best, res = autotune.tune(f["engine"], 0, X, Y, cands, reps=reps, log=log)
+        best, res = autotune.tune(f["engine"], semiring, X, Y, cands, reps=reps, log=log)
         runs.append((min(ms for _, ms in res), f, best, res))
     _, form, best, _ = min(runs, key=lambda r: r[0])
     tuned = [(f["label"].split(" (")[0], c, ms) for _, f, _, res in runs for c, ms in res]
@@ -467,10 +467,10 @@ def run_sweep_B(args, m, forms):
                       "per_B": out}), flush=True)
 
 
-def run_left_B(args, m, s_res, eng):
-    """The left product Y^T = X^T M per vector count B (levels and graph
-    variants, the faster kept), with the FP32 error of two columns against the
-    FP64 oracle's left brute force."""
+def run_left_B(args, m, forms):
+    """The left product Y^T = X^T M per vector count B (every grammar form in
+    the levels and graph variants, the fastest kept), with the FP32 error of
+    two columns against the FP64 oracle's left brute force."""
     import numpy as np
     import torch
 
@@ -485,30 +485,33 @@ def run_left_B(args, m, s_res, eng):
         if B == 1:
             X = X[:, 0].contiguous()
         best = None
-        for vname in ("levels", "graph"):
-            kw = dict(variant=gmr.VARIANTS[vname])
-            Y = eng.run_left_torch(0, X, **kw)
-            for _ in range(args.warmup):
-                eng.run_left_torch(0, X, Y, **kw)
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            for _ in range(args.steps):
-                eng.run_left_torch(0, X, Y, **kw)
-            e1.record()
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / args.steps
-            log(f"left B={B} {vname}: {ms:.3f} ms")
-            if best is None or ms < best[0]:
-                best = (ms, vname, Y)
-        ms, vname, Y = best
+        for f in forms:
+            eng = f["engine"]
+            for vname in ("levels", "graph"):
+                kw = dict(variant=gmr.VARIANTS[vname])
+                Y = eng.run_left_torch(0, X, **kw)
+                for _ in range(args.warmup):
+                    eng.run_left_torch(0, X, Y, **kw)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(args.steps):
+                    eng.run_left_torch(0, X, Y, **kw)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / args.steps
+                log(f"left B={B} {f['label']} {vname}: {ms:.3f} ms")
+                if best is None or ms < best[0]:
+                    best = (ms, vname, Y, f["label"])
+        ms, vname, Y, label = best
         cols = [0] if B == 1 else [0, B - 1]
         Xs = X.reshape(args.rows, -1)[:, cols].contiguous().cpu().numpy()
         Ys, den = O.brute_force_left(0, args.rows, args.cols, seq, V, Xs)
         Yg = Y.reshape(args.cols, -1)[:, cols].cpu().numpy().astype(np.float64)
         d = np.abs(Yg - Ys)
         err = float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1), np.where(d == 0, 0, np.inf))))
-        r = {"B": B, "ms": ms, "nnz_vec_per_s": nnz * B / (ms * 1e-3), "variant": vname, "fp32_err": err}
+        r = {"B": B, "ms": ms, "nnz_vec_per_s": nnz * B / (ms * 1e-3), "variant": vname, "grammar_form": label,
+             "fp32_err": err}
         log("left", r)
         out.append(r)
     print(json.dumps({"metric": "left product Y^T = X^T M, nnz*vectors/s", "workload": f"{args.workload}"
@@ -537,6 +540,8 @@ def run_rmat(args):
         s_res = g.schedule(n, n, "resident")
         s_ref = g.schedule(n, n, "refcompat")
         eng = s_res.engine()
+        args.rows = args.cols = n
+        forms = grammar_forms(args, g, s_res, eng, log)
         rng = np.random.default_rng(2026 + nw)
         if sr == gmr.BOOLEAN:
             x = np.zeros(n, np.float32)
@@ -546,8 +551,8 @@ def run_rmat(args):
             x[rng.choice(n, 1024, replace=False)] = 0.0
         xt = torch.from_numpy(x).cuda()
         y = torch.empty(n, dtype=torch.int32, device="cuda")
-        best, tuned = autotune.tune(eng, sr, xt, y, autotune.candidates(1, gmr.RESIDENT), reps=2)
-        ms = _time_steps(eng, sr, xt, y, best, args.steps, args.warmup)
+        form, best, tuned = tune_forms(forms, xt, y, autotune.candidates(1, gmr.RESIDENT), reps=2, semiring=sr)
+        ms = _time_steps(form["engine"], sr, xt, y, best, args.steps, args.warmup)
         seq, V = m.arrays()
         y_star, _ = O.brute_force(sr, n, n, seq, V, x)
         exact = bool(np.array_equal(y.cpu().numpy().astype(np.int64), y_star.astype(np.int64)))
@@ -560,14 +565,17 @@ def run_rmat(args):
         # product iterated on the CPU
         trav = None
         if args.traverse:
-            xs = torch.from_numpy(x).cuda()
-            eng.traverse_torch(sr, xs.clone(), max_iters=2)  # builds the incoming lists
-            torch.cuda.synchronize()
-            t0 = time.time()
-            iters = eng.traverse_torch(sr, xs)
-            torch.cuda.synchronize()
-            t_ms = (time.time() - t0) * 1e3
-            got = xs.cpu().numpy()
+            runs = []
+            for f in forms:
+                xs = torch.from_numpy(x).cuda()
+                f["engine"].traverse_torch(sr, xs.clone(), max_iters=2)  # builds the incoming lists
+                torch.cuda.synchronize()
+                t0 = time.time()
+                iters = f["engine"].traverse_torch(sr, xs)
+                torch.cuda.synchronize()
+                runs.append(((time.time() - t0) * 1e3, iters, xs.cpu().numpy(), f["label"]))
+                log(tag, "traversal", f["label"], f"{runs[-1][0]:.2f} ms, {iters} products")
+            t_ms, iters, got, t_form = min(runs, key=lambda r: r[0])
             if sr == gmr.TROPICAL:
                 got = np.minimum(got, np.float32(1e9))  # distances >= 1e9 mean unreached
             xc = x.copy()
@@ -578,19 +586,24 @@ def run_rmat(args):
                     break
                 xc = xn
             want = np.minimum(xc, 1e9) if sr == gmr.TROPICAL else xc
+            clip = (lambda v: np.minimum(v, np.float32(1e9))) if sr == gmr.TROPICAL else (lambda v: v)
             trav = {"products": iters, "ms": t_ms, "ms_per_product": t_ms / iters, "cpu_products": it,
-                    "bit_exact": bool(np.array_equal(got, want)), "moved": int((got != np.minimum(x, 1e9)).sum())}
+                    "grammar_form": t_form, "bit_exact": all(np.array_equal(clip(r[2]), want) for r in runs),
+                    "per_form_ms": {r[3]: round(r[0], 3) for r in runs},
+                    "moved": int((got != np.minimum(x, 1e9)).sum())}
             log(tag, "traversal", trav)
         info = s_res.info()
         res[tag] = {"nnz": nnz, "ms": ms, "nnz_per_s": nnz / (ms * 1e-3), "schedule": autotune.describe(best),
+                    "grammar_form": form["label"],
                     "traversal": trav,
                     "bit_exact": exact, "plain_ms": plain_ms, "plain_nnz_per_s": nnz / (plain_ms * 1e-3),
                     "plain_bit_exact": exact_plain,
                     "grammar": {"rules": info["num_raw_nt"], "depth": info["max_depth"], "C": g.info()["clen"],
                                 "U": info["U"], "refcompat_passthrough": s_ref.info()["num_passthrough"]},
-                    "repair_s": g.build_seconds, "tuned": [(autotune.describe(c), round(t, 4)) for c, t in tuned]}
+                    "repair_s": g.build_seconds,
+                    "tuned": [(lab + " " + autotune.describe(c), round(t, 4)) for lab, c, t in tuned]}
         log(tag, res[tag])
-        del eng, eng_ref, m, g, s_res, s_ref, seq
+        del eng, eng_ref, m, g, s_res, s_ref, seq, forms, form
         torch.cuda.empty_cache()
     print(json.dumps({"metric": METRIC, "workload": f"rmat{scale}_{edges}", "results": res}), flush=True)
 

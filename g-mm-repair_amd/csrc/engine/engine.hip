@@ -71,7 +71,8 @@ Engine::~Engine() {
             if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
         }
     for (void* p : {static_cast<void*>(d_in_off), static_cast<void*>(d_in_src), static_cast<void*>(d_in_coeff),
-                    static_cast<void*>(d_col_off), static_cast<void*>(d_col_term), ws_A, ws_Xs,
+                    static_cast<void*>(d_col_off), static_cast<void*>(d_col_val), static_cast<void*>(d_lf_bits),
+                    static_cast<void*>(d_lf_ord), static_cast<void*>(d_lf_col), ws_A, ws_Xs,
                     static_cast<void*>(d_lseg), static_cast<void*>(d_lsplit), static_cast<void*>(d_lpsrc),
                     static_cast<void*>(d_lpoff), static_cast<void*>(d_lbits), static_cast<void*>(d_lord),
                     static_cast<void*>(d_lrow), ws_trav, static_cast<void*>(d_lxsym), static_cast<void*>(d_lxbits),
@@ -781,15 +782,30 @@ void Engine::build_left() {
     std::vector<int2> kids(NL), em(E), te(Et);
     std::vector<float> coeff(has_runlen ? NL : 0);
     std::vector<int> tcol(U);
+    std::vector<float> tv(U);
     if (NL) GMR_HIP(hipMemcpy(kids.data(), d_kids, sizeof(int2) * NL, hipMemcpyDeviceToHost));
     if (has_runlen && NL) GMR_HIP(hipMemcpy(coeff.data(), d_coeff, sizeof(float) * NL, hipMemcpyDeviceToHost));
     if (E) GMR_HIP(hipMemcpy(em.data(), d_emits, sizeof(int2) * E, hipMemcpyDeviceToHost));
     if (Et) GMR_HIP(hipMemcpy(te.data(), d_term, sizeof(int2) * Et, hipMemcpyDeviceToHost));
     if (U) GMR_HIP(hipMemcpy(tcol.data(), d_tcol, sizeof(int) * U, hipMemcpyDeviceToHost));
+    if (U) GMR_HIP(hipMemcpy(tv.data(), d_tval, sizeof(float) * U, hipMemcpyDeviceToHost));
     const long nodes = static_cast<long>(NL) + U;
-    // A child c >= 0 is a slot; c < 0 the compact terminal -c-1 (node NL + t).
+    // Terminal nodes in column order: compact terminal t is node NL + tpos[t];
+    // position 0 is terminal 0, then each column's terminals together, so the
+    // column fold reads node values and terminal values in sequence.
+    std::vector<int> coff(static_cast<std::size_t>(cols) + 1, 0), tpos(U, 0);
+    std::vector<float> cval(U, 0.0f);
+    for (int t = 1; t < U; ++t) ++coff[tcol[t] + 1];
+    coff[0] = 1;
+    for (int c = 0; c < cols; ++c) coff[c + 1] += coff[c];
+    {
+        std::vector<int> next(coff.begin(), coff.end() - 1);
+        for (int t = 1; t < U; ++t) tpos[t] = next[tcol[t]]++;
+        for (int t = 0; t < U; ++t) cval[tpos[t]] = tv[t];
+    }
+    // A child c >= 0 is a slot; c < 0 the compact terminal -c-1.
     // Terminal 0 is the semiring zero: nothing flows into it that matters.
-    auto node_of = [&](int c) -> long { return c >= 0 ? c : static_cast<long>(NL) + (-c - 1); };
+    auto node_of = [&](int c) -> long { return c >= 0 ? c : static_cast<long>(NL) + tpos[-c - 1]; };
     auto wanted = [&](int c) { return c >= 0 || c < -1; };
     std::vector<long> off(static_cast<std::size_t>(nodes) + 1, 0);
     for (int p = 0; p < NL; ++p) {
@@ -798,7 +814,7 @@ void Engine::build_left() {
     }
     for (const int2& e : em) ++off[e.x + 1];
     for (const int2& e : te)
-        if (e.x > 0) ++off[static_cast<long>(NL) + e.x + 1];
+        if (e.x > 0) ++off[static_cast<long>(NL) + tpos[e.x] + 1];
     for (long n = 0; n < nodes; ++n) off[n + 1] += off[n];
     if (off[nodes] > 0x7fffffffL) throw std::invalid_argument("left product: more than 2^31 incoming entries");
     std::vector<int> src(static_cast<std::size_t>(off[nodes]));
@@ -824,17 +840,25 @@ void Engine::build_left() {
     }
     for (const int2& e : te) {
         if (e.x <= 0) continue;
-        const long q = fill[static_cast<long>(NL) + e.x]++;
+        const long q = fill[static_cast<long>(NL) + tpos[e.x]]++;
         src[q] = -e.y - 1;
         if (has_runlen) cf[q] = 1.0f;
     }
     std::vector<int> off32(off.begin(), off.end());
-    std::vector<int> coff(static_cast<std::size_t>(cols) + 1, 0), cterm(U > 0 ? U - 1 : 0);
-    for (int t = 1; t < U; ++t) ++coff[tcol[t] + 1];
-    for (int c = 0; c < cols; ++c) coff[c + 1] += coff[c];
-    {
-        std::vector<int> cf2(coff.begin(), coff.end() - 1);
-        for (int t = 1; t < U; ++t) cterm[cf2[tcol[t]]++] = t;
+    // One-vector fold: 64-position chunks of positions 1..U-1 with their
+    // column-start bitmap (as the chunked row gather).
+    n_lf_chunk = U > 1 ? (static_cast<long>(U) - 1 + 63) / 64 : 0;
+    std::vector<unsigned long long> lfbits(static_cast<std::size_t>(n_lf_chunk), 0ull);
+    std::vector<int> lford(static_cast<std::size_t>(n_lf_chunk), 0), lfcol;
+    for (int c = 0; c < cols; ++c) {
+        if (coff[c + 1] == coff[c]) continue;
+        const long q = coff[c] - 1;
+        lfbits[q >> 6] |= 1ull << (q & 63);
+        lfcol.push_back(c);
+    }
+    for (long c = 0, before = 0; c < n_lf_chunk; ++c) {
+        lford[c] = static_cast<int>(before);
+        before += __builtin_popcountll(lfbits[c]);
     }
     if (!has_runlen) {
         // Row-gather form. Groups: levels 1..L, then the terminals; node order
@@ -919,7 +943,10 @@ void Engine::build_left() {
     d_in_src = dev_upload(src.data(), src.size(), &struct_bytes);
     d_in_coeff = has_runlen ? dev_upload(cf.data(), cf.size(), &struct_bytes) : nullptr;
     d_col_off = dev_upload(coff.data(), coff.size(), &struct_bytes);
-    d_col_term = dev_upload(cterm.data(), cterm.size(), &struct_bytes);
+    d_col_val = dev_upload(cval.data(), cval.size(), &struct_bytes);
+    d_lf_bits = dev_upload(lfbits.data(), lfbits.size(), &struct_bytes);
+    d_lf_ord = dev_upload(lford.data(), lford.size(), &struct_bytes);
+    d_lf_col = dev_upload(lfcol.data(), lfcol.size(), &struct_bytes);
     left_ready = true;
 }
 
@@ -1027,9 +1054,14 @@ void launch_left(Engine& en, const dev::LeftArgs& a, int block, hipStream_t st) 
             hipLaunchKernelGGL(dev::k_left_nodes<S>, dim3(grid_for(static_cast<long>(en.U) * a.G, block)), dim3(block),
                                0, st, a, en.NL, en.NL + en.U);
     }
-    if (en.cols > 0)
+    if (en.cols > 0 && a.Bt == 1) {
+        hipLaunchKernelGGL(dev::k_left_yzero<S>, dim3(grid_for(en.cols, block, 4096)), dim3(block), 0, st, a, en.cols);
+        if (en.n_lf_chunk > 0)
+            hipLaunchKernelGGL(dev::k_left_fold<S>, dim3(grid_for(en.n_lf_chunk * 64, block)), dim3(block), 0, st, a);
+    } else if (en.cols > 0) {
         hipLaunchKernelGGL(dev::k_left_cols<S>, dim3(grid_for(static_cast<long>(en.cols) * a.G, block)), dim3(block),
                            0, st, a, en.cols);
+    }
     GMR_HIP(hipGetLastError());
 }
 
@@ -1067,8 +1099,12 @@ void run_left_t(Engine& en, int semiring, const float* X, int B, void* Y, const 
         a.in_src = en.d_in_src;
         a.in_coeff = en.d_in_coeff;
         a.col_off = en.d_col_off;
-        a.col_term = en.d_col_term;
-        a.tval = en.d_tval;
+        a.col_val = en.d_col_val;
+        a.fbits = en.d_lf_bits;
+        a.ford = en.d_lf_ord;
+        a.fcol = en.d_lf_col;
+        a.n_fchunk = en.n_lf_chunk;
+        a.npos = en.U;
         a.X = X + b0;
         a.Y = static_cast<char*>(Y) + static_cast<std::size_t>(b0) * sizeof(T);
         a.A = en.ws_A;

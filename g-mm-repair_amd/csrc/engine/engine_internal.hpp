@@ -87,11 +87,17 @@ struct Engine {
 
     // left product (built on first use, Resident layout only)
     bool left_ready = false;
-    int* d_in_off = nullptr;         // nodes + 1 (rule slots, then terminals)
+    int* d_in_off = nullptr;         // nodes + 1 (rule slots, then terminals in column order)
     int* d_in_src = nullptr;         // >= 0 parent slot, < 0 row -e-1
     float* d_in_coeff = nullptr;     // per entry, only with run-length rules
-    int* d_col_off = nullptr;        // cols + 1
-    int* d_col_term = nullptr;
+    int* d_col_off = nullptr;        // cols + 1: column c's terminals are positions [off[c], off[c+1])
+    float* d_col_val = nullptr;      // terminal value per position (position p is node NL + p)
+    // one-vector fold: positions 1..U-1 in 64-position chunks, column-start
+    // bitmap, starts before each chunk, the non-empty columns in order
+    unsigned long long* d_lf_bits = nullptr;
+    int* d_lf_ord = nullptr;
+    int* d_lf_col = nullptr;
+    long n_lf_chunk = 0;
     void* ws_A = nullptr;            // nodes x Bs values
     std::size_t ws_A_bytes = 0;
     void* ws_Xs = nullptr;           // (rows + 1) x Bs: x as semiring values, then a zero row

@@ -9,7 +9,8 @@
 // -- and every parent of a rule sits on a higher level, so levels L..1 run in
 // order, then the terminals. A column then collects its terminals:
 // y_c = (+)_{terminals t in column c} leaf(V_t, A[t]).
-// Incoming lists are CSR over nodes (rule slots, then terminals); an entry
+// Incoming lists are CSR over nodes (rule slots, then the terminals grouped
+// by column, so the fold reads them in sequence); an entry
 // >= 0 is a parent slot, < 0 is a row -e-1. That is the shape of the right
 // product's row gather (nodes for rows, entries for top-sequence symbols, x as
 // a semiring value for the terminal rows), so without run-length rules the
@@ -50,13 +51,17 @@ struct LeftArgs {
     const int* in_off;     // nodes + 1: incoming entries of node n are [in_off[n], in_off[n+1])
     const int* in_src;     // >= 0 parent slot, < 0 row -e-1
     const float* in_coeff; // per entry (nullptr when every coefficient is 1)
-    const int* col_off;    // cols + 1
-    const int* col_term;   // compact terminals grouped by column
-    const float* tval;     // value of compact terminal t
+    const int* col_off;    // cols + 1: column c's terminals are positions [col_off[c], col_off[c+1])
+    const float* col_val;  // terminal value per position
+    const unsigned long long* fbits;  // one vector: column starts per 64-position chunk (positions from 1)
+    const int* ford;       // column starts before each chunk
+    const int* fcol;       // non-empty columns in order
+    long n_fchunk;
+    int npos;              // positions 0..npos-1 (npos = U; 0 is the zero terminal)
     const float* X;        // rows x B (this tile's column base folded in)
     void* Y;               // cols x B
     void* A;               // nodes x Bs node values
-    int NL;                // rule slots; terminal t is node NL + t
+    int NL;                // rule slots; terminal position p is node NL + p
     int B, Bt, Bs, G;
     int xparts;            // one vector: the XCD-partitioned lists (else one chunked list)
 };
@@ -94,16 +99,62 @@ __device__ void ph_left_cols(const LeftArgs& a, int cols, long gid, long ng, int
     const T* A = tp<S>(a.A);
     T* Y = tp<S>(a.Y);
     for (long c = gid; c < cols; c += ng) {
-        const int t0 = a.col_off[c], t1 = a.col_off[c + 1];
+        const int p0 = a.col_off[c], p1 = a.col_off[c + 1];
         for (int b = lane; b < a.Bt; b += a.G) {
             T acc = S::zero();
-            for (int i = t0; i < t1; ++i) {
-                const int t = a.col_term[i];
-                acc = S::oplus(acc, L::leaf(a.tval[t], A[static_cast<long>(a.NL + t) * a.Bs + b]));
-            }
+            for (int p = p0; p < p1; ++p)
+                acc = S::oplus(acc, L::leaf(a.col_val[p], A[static_cast<long>(a.NL + p) * a.Bs + b]));
             Y[c * a.B + b] = acc;
         }
     }
+}
+
+// One vector: the column fold as a flat segmented reduction. A wave takes 64
+// consecutive terminal positions (one per lane, so every load is coalesced),
+// applies the leaf, and reduces per column with the segmented shuffle scan of
+// the row gather (the chunk's column-start bitmap marks the segments). A column
+// that starts and ends inside the chunk is stored; the pieces of one that
+// crosses a chunk boundary combine atomically into y, which k_left_yzero
+// cleared (as it did every column without terminals).
+template <class S>
+__global__ void k_left_fold(LeftArgs a) {
+    using T = typename S::T;
+    using L = LeftOps<S>;
+    const T* A = tp<S>(a.A);
+    T* Y = tp<S>(a.Y);
+    const long tid = xcd_tid();
+    const int lane = static_cast<int>(tid & 63);
+    const long nw = (static_cast<long>(gridDim.x) * blockDim.x) >> 6;
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
+    for (long c = tid >> 6; c < a.n_fchunk; c += nw) {
+        const long p = 1 + c * 64 + lane;
+        const bool valid = p < a.npos;
+        T v = S::zero();
+        if (valid) v = L::leaf(a.col_val[p], A[(a.NL + p) * a.Bs]);
+        const unsigned long long bits = a.fbits[c];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const T t = __shfl_up(v, d, 64);
+            if (lane >= d && ((bits >> (lane - d + 1)) & ((1ull << d) - 1ull)) == 0ull) v = S::oplus(v, t);
+        }
+        const bool last = valid && (lane == 63 || p + 1 == a.npos || ((bits >> (lane + 1)) & 1ull) != 0ull);
+        if (last) {
+            const long col = a.fcol[a.ford[c] - 1 + __popcll(bits & upto)];
+            const bool starts_here = (bits & upto) != 0ull;
+            const bool ends_here = lane < 63 || a.col_off[col + 1] == p + 1;
+            if (starts_here && ends_here) Y[col * a.B] = v;
+            else if (!S::skip(v)) S::emit(Y + col * a.B, v);
+        }
+    }
+}
+
+// One vector: y = the semiring zero on every column (before k_left_fold).
+template <class S>
+__global__ void k_left_yzero(LeftArgs a, int cols) {
+    typename S::T* Y = tp<S>(a.Y);
+    const long tid = xcd_tid();
+    const long nth = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long c = tid; c < cols; c += nth) Y[c * a.B] = S::zero();
 }
 
 // Before a sweep without run-length rules (which runs on the row-gather

@@ -56,10 +56,11 @@ def test_unit_grammar_with_a_run_length_rule():
     rows, cols = 2, 3
     eng = g.schedule(rows, cols, "resident").engine()
     X = np.array([[0.5, -1.0], [2.0, 3.0]], np.float32)
-    Y_star, den = O.brute_force_left(0, rows, cols, seq, np.array(f["V"], np.float64), X)
-    for var in ("levels", "graph"):
-        Y = run_left(eng, 0, X, variant=gmr.VARIANTS[var])
-        check(0, Y, Y_star, den, f"unit {var}")
+    for XX in (X, X[:, 0].copy()):
+        Y_star, den = O.brute_force_left(0, rows, cols, seq, np.array(f["V"], np.float64), XX)
+        for var in ("levels", "graph"):
+            Y = run_left(eng, 0, XX, variant=gmr.VARIANTS[var])
+            check(0, Y, Y_star, den, f"unit {var} B={XX.shape[1] if XX.ndim == 2 else 1}")
 
 
 @pytest.fixture(scope="module")
