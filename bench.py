@@ -394,16 +394,10 @@ def untuned_form(args, forms, B):
 
 
 def tune_forms(forms, X, Y, cands, reps=2, log=None, semiring=0):
-    """Autotune every grammar form; return (form, best candidate,
-    [(form label, cand, ms), ...]) for the fastest."""
+    """autotune.tune_forms with the form labels shortened for the JSON line."""
     from gmr import autotune
-    runs = []
-    for f in forms:
-        best, res = autotune.tune(f["engine"], semiring, X, Y, cands, reps=reps, log=log)
-        runs.append((min(ms for _, ms in res), f, best, res))
-    _, form, best, _ = min(runs, key=lambda r: r[0])
-    tuned = [(f["label"].split(" (")[0], c, ms) for _, f, _, res in runs for c, ms in res]
-    return form, best, tuned
+    form, best, tuned = autotune.tune_forms(forms, semiring, X, Y, cands, reps=reps, log=log)
+    return form, best, [(lab.split(" (")[0], c, ms) for lab, c, ms in tuned]
 
 
 def _time_steps(eng, semiring, X, Y, cand, steps, warmup):

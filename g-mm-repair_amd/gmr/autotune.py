@@ -120,3 +120,17 @@ def tune(engine, semiring, X, Y, cands, cache_path=None, reps=3, log=None):
         with open(cache_path, "w") as f:
             json.dump(db, f)
     return best, results
+
+
+def tune_forms(forms, semiring, X, Y, cands, cache_path=None, reps=2, log=None):
+    """Tune several engines of the same matrix -- e.g. the grammar as built
+    and Grammar.inline_top(2) -- and keep the fastest. forms: [{"label": str,
+    "engine": Engine, ...}]. Returns (form, best candidate, [(label, cand, ms),
+    ...] over every form)."""
+    runs = []
+    for f in forms:
+        best, res = tune(f["engine"], semiring, X, Y, cands, cache_path=cache_path, reps=reps, log=log)
+        runs.append((min(ms for _, ms in res), f, best, res))
+    _, form, best, _ = min(runs, key=lambda r: r[0])
+    return form, best, [(f["label"], c, ms) for _, f, _, res in runs for c, ms in res]
+

@@ -58,3 +58,22 @@ def test_tune_picks_the_fastest_and_reuses_the_disk_cache(tmp_path, monkeypatch)
     # another batch width is another key: timed again
     autotune.tune(_Engine(), 0, _X(16), None, autotune.candidates(16, gmr.RESIDENT), cache_path=cache)
     assert len(calls) > n
+
+
+class _Engine2(_Engine):
+    def info(self):
+        return {"rows": 10, "cols": 4, "U": 5, "max_depth": 2}
+
+
+def test_tune_forms_keeps_the_fastest_form(monkeypatch):
+    def fake_time(engine, semiring, X, Y, cand, reps=3, warm=1):
+        return (0.5 if isinstance(engine, _Engine2) else 1.0) + 0.1 * cand["b_tile"]
+
+    monkeypatch.setattr(autotune, "time_candidate", fake_time)
+    forms = [{"label": "as built", "engine": _Engine()}, {"label": "inlined", "engine": _Engine2()}]
+    cands = autotune.candidates(64, gmr.RESIDENT)
+    form, best, tuned = autotune.tune_forms(forms, 0, _X(64), None, cands)
+    assert form["label"] == "inlined" and best["b_tile"] == 0
+    assert len(tuned) == 2 * len(cands) and {lab for lab, _, _ in tuned} == {"as built", "inlined"}
+    assert min(ms for _, _, ms in tuned) == 0.5
+
