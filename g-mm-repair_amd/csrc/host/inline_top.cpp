@@ -18,16 +18,17 @@ grammar::Grammar inline_top_rules(const grammar::Grammar& g, int max_emits, long
     const int alpha = g.alpha;
     const long long n = static_cast<long long>(g.rules.size());
     std::vector<int> refs(n, 0), emits(n, 0);
-    auto is_rule = [&](int s) { return s >= alpha && s - alpha < n; };
+    auto child = [&](int s) {
+        if (s < 1 || s - alpha >= n) throw std::invalid_argument("inline_top: rule child out of range");
+        if (s >= alpha) ++refs[s - alpha];
+    };
     for (const auto& r : g.rules) {
-        if (is_rule(r.left)) ++refs[r.left - alpha];
-        if (r.kind == grammar::BINARY && is_rule(r.right)) ++refs[r.right - alpha];
+        child(r.left);
+        if (r.kind == grammar::BINARY) child(r.right);
     }
     for (int s : g.C) {
-        if (s >= alpha) {
-            if (!is_rule(s)) throw std::invalid_argument("inline_top: symbol out of range in C");
-            ++emits[s - alpha];
-        }
+        if (s < 0 || s - alpha >= n) throw std::invalid_argument("inline_top: symbol out of range in C");
+        if (s >= alpha) ++emits[s - alpha];
     }
     // Selected rules are referenced by no rule, so none of them is the child
     // of another: one pass over C replaces every occurrence.

@@ -247,6 +247,11 @@ def test_inline_top_known_answers():
     assert list(g.inline_top(0).arrays()["C"]) == [5, 0, 4, 3, 0]
     g = gmr.Grammar.from_arrays(3, [1], [2], [3, 3, 0, 3, 0], [1.0])
     assert g.inline_top(2).n_inlined == 0 and g.inline_top(3).arrays()["C"].tolist() == [1, 2, 1, 2, 0, 1, 2, 0]
+    # malformed input is rejected, not read out of bounds
+    with pytest.raises(gmr.GmrError):
+        gmr.Grammar.from_arrays(4, [1, 9], [2, 3], [5, 0], [1.0]).inline_top(2)
+    with pytest.raises(gmr.GmrError):
+        gmr.Grammar.from_arrays(4, [1], [2], [7, 0], [1.0]).inline_top(2)
     # the reference's unit grammar: the top rule 8 -> 7 5 goes, the run-length
     # rule 7 = 6^3 (referenced by 8) stays with its coefficient
     f = fixture("unit_grammar")
