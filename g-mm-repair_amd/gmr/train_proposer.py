@@ -94,6 +94,10 @@ def main(argv=None):
     ap.add_argument("--graphs", action="store_true", help="also train on R-MAT graphs (config 4 family)")
     ap.add_argument("--vary", action="store_true",
                     help="also train on haplotype/genotype matrices generated with weak-compression parameters")
+    ap.add_argument("--weights", default=None,
+                    help="score these trained weights (.npz) on the held-out set instead of training")
+    ap.add_argument("--inline-top-from", type=int, default=0,
+                    help="held-out grammars with B >= this run on Grammar.inline_top(2) (0: never)")
     args = ap.parse_args(argv)
     import torch
 
@@ -104,7 +108,7 @@ def main(argv=None):
     train_B = [1, 3, 4, 16, 29, 64, 100, 256] if not args.quick else [1, 4, 16, 37, 64]
     Xs, Ts, Ms = [], [], []
     t_collect = time.time()
-    for name, m in training_set(args.quick, args.graphs, args.vary):
+    for name, m in ([] if args.weights else training_set(args.quick, args.graphs, args.vary)):
         g = m.repair()
         s = g.schedule(m.rows, m.cols, "resident")
         arr = s.arrays()
@@ -116,14 +120,19 @@ def main(argv=None):
         log("collected", name, m.info(), "samples", len(X))
         del eng
         torch.cuda.empty_cache()
-    X, T, M = np.vstack(Xs), np.vstack(Ts), np.vstack(Ms)
     t_collect = time.time() - t_collect
-    prop = P.Proposer(seed=1)
-    t_train = time.time()
-    losses = prop.fit(X, T, M, epochs=args.epochs)
-    t_train = time.time() - t_train
-    log(f"trained on {len(X)} samples: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {t_train:.1f}s")
-    prop.save(os.path.join(args.out, "proposer.npz"))
+    if args.weights:
+        prop = P.Proposer.load(args.weights)
+        X, losses, t_train = np.zeros((0, 1)), [float("nan")], 0.0
+        log("scoring", args.weights)
+    else:
+        X, T, M = np.vstack(Xs), np.vstack(Ts), np.vstack(Ms)
+        prop = P.Proposer(seed=1)
+        t_train = time.time()
+        losses = prop.fit(X, T, M, epochs=args.epochs)
+        t_train = time.time() - t_train
+        log(f"trained on {len(X)} samples: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {t_train:.1f}s")
+        prop.save(os.path.join(args.out, "proposer.npz"))
 
     # ---- held-out evaluation
     rows_out, engines, Xg, opts = [], [], [], []
@@ -131,6 +140,9 @@ def main(argv=None):
     for name, m, B in heldout_set(args.heldout if not args.quick else 6, 20261203,
                                   args.nnz_lo if not args.quick else 2e4, args.nnz_hi if not args.quick else 3e5):
         g = m.repair()
+        inlined = bool(args.inline_top_from) and B >= args.inline_top_from
+        if inlined:
+            g = g.inline_top(2)
         s = g.schedule(m.rows, m.cols, "resident")
         arr = s.arrays()
         eng = s.engine()
@@ -146,6 +158,7 @@ def main(argv=None):
         he = P.heuristic(table, B)
         mine = P.sweep_cost(table, B, choice[0], choice[1], choice[2])
         r = dict(name=name, nnz=m.nnz(), B=B, rules=arr["info"]["num_raw_nt"], depth=arr["info"]["max_depth"],
+                 inlined=inlined,
                  exhaustive_ms=ex[0], exhaustive=dict(tile=ex[1], plan=P.PLAN_NAMES[ex[2]]),
                  proposer_ms=mine, proposer=dict(tile=choice[0], plan=P.PLAN_NAMES[choice[1]]),
                  plain_ms=he[0], regret=mine / ex[0] - 1.0, gain_over_plain=he[0] / mine,
@@ -182,6 +195,7 @@ def main(argv=None):
     nnz_vec = sum(r["nnz"] * r["B"] for r in rows_out)
     report = dict(
         training=dict(samples=int(len(X)), epochs=args.epochs, graphs=bool(args.graphs), vary=bool(args.vary),
+                      weights=args.weights, inline_top_from=args.inline_top_from,
                       loss_first=float(losses[0]),
                       loss_last=float(losses[-1]), collect_s=t_collect, train_s=t_train, B=train_B),
         heldout=dict(n=len(rows_out), regret_mean=float(reg.mean()), regret_median=float(np.median(reg)),
