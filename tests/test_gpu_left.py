@@ -118,6 +118,26 @@ def test_left_equals_right_product_of_the_transpose():
     np.testing.assert_allclose(yl, yr, rtol=1e-5, atol=1e-5)
 
 
+@pytest.mark.parametrize("sr", [0, 1, 2])
+def test_left_fold_with_columns_wider_than_a_chunk(sr):
+    """One vector, columns holding up to 150 distinct values (150 terminals):
+    the chunked column fold sees columns that start, end and run through
+    64-position chunks."""
+    rng = np.random.default_rng(17 + sr)
+    M = rng.integers(1, 151, (300, 5)) * (rng.random((300, 5)) < 0.9)
+    M[:, 3] = 0  # an empty column
+    m = gmr.Matrix.from_dense(M.astype(np.float64))
+    g = m.repair()
+    seq, V = m.arrays()
+    eng = g.schedule(300, 5, "resident").engine()
+    x = x_for(sr, 300, 23)
+    y_star, den = O.brute_force_left(sr, 300, 5, seq, V, x)
+    for var in ("levels", "graph"):
+        for flags in (0, gmr.RUN_SINGLE_LIST):
+            y = run_left(eng, sr, x, variant=gmr.VARIANTS[var], flags=flags)
+            check(sr, y, y_star, den if sr == 0 else None, f"wide columns sr={sr} {var} flags={flags}")
+
+
 def test_left_product_errors():
     m = gmr.Matrix.haplotype(5, 64, 32, founders=4, seg_mean=8.0)
     g = m.repair()
