@@ -88,6 +88,7 @@ def main():
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE CSR baseline and the energy reads")
     ap.add_argument("--no-tune", action="store_true", help="use the default schedule instead of autotuning")
     ap.add_argument("--variant", default=None, help="force a variant (levels|graph|persistent)")
     ap.add_argument("--emit", default=None, help="force an emission strategy (spot|gather)")
@@ -254,9 +255,9 @@ def main():
     from oracle import oracle as O
     seq, V = m.arrays()
     cols_chk = [0, B // 3, (2 * B) // 3, B - 1]
-    Xs = X[:, cols_chk].contiguous().cpu().numpy()
+    Xs = X.reshape(args.cols, -1)[:, cols_chk].contiguous().cpu().numpy()
     Ys, den = O.brute_force_batched(0, args.rows, args.cols, seq, V, Xs)
-    Yg = Y[:, cols_chk].cpu().numpy().astype(np.float64)
+    Yg = Y.reshape(args.rows, -1)[:, cols_chk].cpu().numpy().astype(np.float64)
     d = np.abs(Yg - Ys)
     err = float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1), np.where(d == 0, 0, np.inf))))
     log(f"fp32 err (4 columns vs FP64 oracle): {err:.3e}")
@@ -283,6 +284,12 @@ def main():
     form1, best1, _ = tune_forms(forms, x1, y1, autotune.candidates(1, gmr.RESIDENT), reps=3)
     one_ms = autotune.time_candidate(form1["engine"], 0, x1, y1, best1, reps=args.steps)
     del eng_ref
+
+    # ---- vendor baseline (rocSPARSE CSR product of the uncompressed matrix at
+    # the same B, the reference's cuSPARSE comparison) and energy per step
+    vendor_line, energy = None, None
+    if not args.no_vendor:
+        vendor_line, energy = vendor_leg(args, m, X, Y, eng, run_kw, B, nnz, value)
 
     # ---- roofline of the dominant kernel: k_gather (about 70% of the sweep's
     # device time at B = 64); the whole sweep is reported beside it
@@ -360,9 +367,79 @@ def main():
         "best_single_vector": {"value": nnz / (one_ms * 1e-3), "ms": one_ms, "schedule": autotune.describe(best1),
                                "grammar_form": form1["label"]},
         "fp32_err": err,
+        "vendor_csr": vendor_line,
+        "energy": energy,
         "tuned": [(lab + " " + autotune.describe(c), round(ms, 4)) for lab, c, ms in tuned],
     }
     print(json.dumps(out), flush=True)
+
+
+def vendor_leg(args, m, X, Y, eng, run_kw, B, nnz, value):
+    """rocSPARSE SpMM (SpMV at B = 1) of the uncompressed matrix in CSR form on
+    the same X, every CSR algorithm timed with HIP events over the same number
+    of steps (the fastest reported), its agreement with the engine's Y, and the
+    energy per step of both from the amdsmi counter over >= 1.5 s loops."""
+    import torch
+
+    import gmr
+    from gmr import vendor
+    t = time.time()
+    A = vendor.Csr.from_matrix(m)
+    build_s = time.time() - t
+    Yv = torch.empty_like(Y)
+    per_alg = {}
+    for alg in vendor.ALGS:
+        try:
+            A.run_torch(X, Yv, alg=alg)  # preprocess + warm-up
+            torch.cuda.synchronize()
+        except gmr.GmrError as e:
+            log("rocSPARSE", alg, "unsupported:", e)
+            per_alg[alg] = None
+            continue
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            A.run_torch(X, Yv, alg=alg)
+        e1.record()
+        torch.cuda.synchronize()
+        per_alg[alg] = e0.elapsed_time(e1) / args.steps
+        log(f"rocSPARSE {'SpMM' if B > 1 else 'SpMV'} ({alg}): {per_alg[alg]:.3f} ms")
+    ok = [a for a in per_alg if per_alg[a] is not None]
+    if not ok:
+        return None, None
+    best = min(ok, key=per_alg.get)
+    A.run_torch(X, Yv, alg=best)
+    torch.cuda.synchronize()
+    diff = float((Yv - Y).abs().max() / Y.abs().max().clamp_min(1e-30))
+    ms = per_alg[best]
+    line = {"library": "rocSPARSE", "op": "SpMM" if B > 1 else "SpMV", "alg": best, "ms": ms,
+            "value": nnz * B / (ms * 1e-3), "unit": "nnz*vectors/s",
+            "per_alg_ms": {a: (round(v, 4) if v is not None else None) for a, v in per_alg.items()},
+            "engine_over_vendor": value / (nnz * B / (ms * 1e-3)),
+            "max_abs_diff_vs_engine_over_max_abs_y": diff, "csr_device_bytes": A.device_bytes(),
+            "csr_build_s": round(build_s, 1)}
+    log("vendor", line)
+    energy = None
+    if vendor.energy_uj(0) is not None:
+        try:
+            mj_v, w_v = A.energy(B=B, alg=best, seconds=2.0)
+        except gmr.GmrError as e:
+            log("vendor energy:", e)
+            mj_v = w_v = None
+
+        def batch():
+            for _ in range(10):
+                eng.run_torch(0, X, Y, **run_kw)
+            torch.cuda.synchronize()
+            return 10
+        r = vendor.energy_between_updates(batch, seconds=2.0)
+        if r is not None:
+            energy = {"engine_mJ_per_step": r[0], "engine_avg_W": r[1], "vendor_mJ_per_step": mj_v,
+                      "vendor_avg_W": w_v,
+                      "note": "amdsmi energy counter, between two of its updates >= 2 s apart, steps back to back"}
+            log("energy", energy)
+    del A
+    return line, energy
 
 
 def grammar_forms(args, g, s_res, eng, log):

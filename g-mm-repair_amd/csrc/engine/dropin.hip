@@ -13,7 +13,10 @@
 // Tropical any value >= 1e9 acts as +infinity either way).
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -120,6 +123,47 @@ Resident to_resident(int total, const LayeredRule* r, const GPUSchedule& s) {
     return o;
 }
 
+// Energy per vector, as the reference's NVML loop (gpu_engine.cu:286-330):
+// sweeps run back to back between two updates of the amdsmi energy counter at
+// least GMR_ENERGY_SECONDS apart (default 1.5 s; the counter advances about
+// once a second). libgmr_vendor.so reads the counter; it is loaded on first
+// use, so this library needs neither amdsmi nor rocSPARSE. Returns false --
+// the caller prints -1, as the reference does -- when the counter cannot be
+// read or does not advance, or when GMR_ENERGY_SECONDS is 0.
+bool energy_loop(gmr_engine* eng, int sr, const float* dX, int B, void* dY, const gmr_run_opts& o, double* mj,
+                 double* mw) {
+    double secs_min = 1.5;
+    if (const char* e = std::getenv("GMR_ENERGY_SECONDS")) secs_min = std::atof(e);
+    if (!(secs_min > 0.0)) return false;
+    using EnergyFn = int (*)(int, double*, unsigned long long*);
+    static EnergyFn fn = [] {
+        void* h = dlopen("libgmr_vendor.so", RTLD_NOW | RTLD_LOCAL);
+        return h ? reinterpret_cast<EnergyFn>(dlsym(h, "gmr_energy_uj")) : nullptr;
+    }();
+    int dev = 0;
+    double e = 0.0, e0 = 0.0;
+    unsigned long long ts = 0, ts_prev = 0, ts0 = 0;
+    if (!fn || hipGetDevice(&dev) != hipSuccess || fn(dev, &e, &ts_prev) != GMR_OK) return false;
+    const auto start = std::chrono::steady_clock::now();
+    auto batch = [&] {
+        for (int i = 0; i < 10; ++i)
+            if (gmr_engine_run(eng, sr, dX, B, dY, &o, nullptr) != GMR_OK) die(gmr_engine_last_error_internal());
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count() <= 2 * secs_min + 5.0;
+    };
+    do {  // up to the next counter update
+        if (!batch() || fn(dev, &e0, &ts0) != GMR_OK) return false;
+    } while (ts0 == ts_prev);
+    long count = 0;
+    do {  // to the first update at least secs_min later
+        count += 10;
+        if (!batch() || fn(dev, &e, &ts) != GMR_OK) return false;
+    } while (ts == ts0 || static_cast<double>(ts - ts0) * 1e-9 < secs_min);
+    *mj = (e - e0) * 1e-3 / static_cast<double>(count * B);
+    *mw = (e - e0) * 1e-3 / (static_cast<double>(ts - ts0) * 1e-9);
+    return true;
+}
+
 void dropin(int rows, int cols, int alpha, int total, const LayeredRule* h_rules, const int* h_val,
             const int* h_col, const float* h_V, int nvals, const float* h_X, const GPUSchedule& s,
             float* h_Y, int B, int iters, bool batched) {
@@ -199,9 +243,10 @@ void dropin(int rows, int cols, int alpha, int total, const LayeredRule* h_rules
     if (!batched) {
         std::cout << "MEM analytic bytes: " << mem_analytic << std::endl;
         std::cout << "MEM peak bytes: " << (free0 > free1 ? free0 - free1 : 0) << std::endl;
-        // No energy counter is read here; the reference prints -1 in that case.
-        std::cout << "ENERGY mJ/vector: " << -1.0 << std::endl;
-        std::cout << "ENERGY avg power mW: " << -1.0 << std::endl;
+        double mj = -1.0, mw = -1.0;
+        if (iters <= 0 || !energy_loop(eng, sr, dX, B, dY, o, &mj, &mw)) mj = mw = -1.0;
+        std::cout << "ENERGY mJ/vector: " << mj << std::endl;
+        std::cout << "ENERGY avg power mW: " << mw << std::endl;
     }
     if (iters > 0 && rows > 0)
         hip_check(hipMemcpy(h_Y, dY, vb * static_cast<std::size_t>(rows) * B, hipMemcpyDeviceToHost), "copy Y");

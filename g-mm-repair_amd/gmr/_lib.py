@@ -1,5 +1,6 @@
-"""ctypes bindings of the engine's two C-ABI libraries (lib/libgmr_host.so and
-lib/libgmr.so, declared in include/gmr_host.h and include/gmr.h).
+"""ctypes bindings of the engine's C-ABI libraries (lib/libgmr_host.so and
+lib/libgmr.so, declared in include/gmr_host.h and include/gmr.h), and of the
+vendor baselines (lib/libgmr_vendor.so, include/gmr_vendor.h).
 
 Loading never builds anything: the libraries are built in-tree by the package
 Makefile. A missing library raises ImportError naming the file.
@@ -11,6 +12,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 HOST_LIB = os.path.join(LIB_DIR, "libgmr_host.so")
 DEV_LIB = os.path.join(LIB_DIR, "libgmr.so")
+VENDOR_LIB = os.path.join(LIB_DIR, "libgmr_vendor.so")
 
 c_int_p = C.POINTER(C.c_int)
 c_float_p = C.POINTER(C.c_float)
@@ -52,6 +54,7 @@ def _sig(lib, name, res, args):
 
 _host = None
 _dev = None
+_vendor = None
 
 
 def host():
@@ -127,3 +130,26 @@ def dev():
     _sig(L, "gmr_mlp_destroy", None, [vp])
     _dev = L
     return L
+
+
+def vendor():
+    """The vendor-baseline library (rocSPARSE CSR products, amdsmi energy).
+    Loading it does not touch the GPU."""
+    global _vendor
+    if _vendor is not None:
+        return _vendor
+    host()
+    if not os.path.exists(VENDOR_LIB):
+        raise ImportError(f"{VENDOR_LIB} is not built (run make in {PKG_DIR})")
+    L = C.CDLL(VENDOR_LIB, mode=C.RTLD_GLOBAL)
+    vp, vpp = C.c_void_p, C.POINTER(C.c_void_p)
+    _sig(L, "gmr_csr_create", C.c_int, [C.c_int, C.c_int, C.c_longlong, c_int_p, c_int_p, c_float_p, vpp])
+    _sig(L, "gmr_csr_run", C.c_int, [vp, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p])
+    _sig(L, "gmr_csr_bench", C.c_int, [vp, c_float_p, C.c_int, c_float_p, C.c_int, C.c_int, c_double_p])
+    _sig(L, "gmr_csr_energy", C.c_int, [vp, C.c_int, C.c_int, C.c_double, c_double_p, c_double_p])
+    _sig(L, "gmr_csr_device_bytes", C.c_longlong, [vp])
+    _sig(L, "gmr_csr_free", None, [vp])
+    _sig(L, "gmr_energy_uj", C.c_int, [C.c_int, c_double_p, C.POINTER(C.c_ulonglong)])
+    _vendor = L
+    return L
+

@@ -10,7 +10,8 @@ from gmr import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INC = os.path.join(ROOT, "include")
-DECL = re.compile(r"^\s*(?:extern \"C\" )?(?:const )?(?:int|void|char|double)\s*\*?\s*((?:gmr|run_gpu)_\w+)\s*\(", re.M)
+DECL = re.compile(r"^\s*(?:extern \"C\" )?(?:const )?(?:int|void|char|double|long long)\s*\*?\s*((?:gmr|run_gpu)_\w+)\s*\(",
+                  re.M)
 
 
 def declared(header):
@@ -23,11 +24,13 @@ def test_headers_declare_the_expected_entry_points():
     assert "run_gpu_right_multiply_batched" in declared("gpu_engine.h")
     assert "gmr_engine_run" in declared("gmr.h")
     assert "gmr_schedule_build" in declared("gmr_host.h")
+    assert "gmr_csr_run" in declared("gmr_vendor.h") and "gmr_csr_device_bytes" in declared("gmr_vendor.h")
 
 
-@pytest.mark.parametrize("header,lib", [("gmr_host.h", "host"), ("gmr.h", "dev"), ("gpu_engine.h", "dev")])
+@pytest.mark.parametrize("header,lib", [("gmr_host.h", "host"), ("gmr.h", "dev"), ("gpu_engine.h", "dev"),
+                                        ("gmr_vendor.h", "vendor")])
 def test_library_exports_every_declared_symbol(header, lib):
-    L = _lib.host() if lib == "host" else _lib.dev()
+    L = {"host": _lib.host, "dev": _lib.dev, "vendor": _lib.vendor}[lib]()
     names = declared(header)
     assert names
     missing = [n for n in names if not hasattr(L, n)]

@@ -227,6 +227,7 @@ def test_run_errors_are_reported(hap):
 
 def _dropin(batched, sched_arrays, rows, cols, V, X, B, semiring_env, monkeypatch):
     monkeypatch.setenv("SEMIRING", semiring_env)
+    monkeypatch.setenv("GMR_ENERGY_SECONDS", "0")  # no 1.5 s energy loop per call
     L = _lib.dev()
     s = sched_arrays
     to = s["term_orig"]
@@ -407,12 +408,17 @@ def test_gpu_test_driver_crosscheck_files(tmp_path):
         xf = tmp_path / f"x{sr}.bin"
         x.tofile(xf)
         prefix = str(tmp_path / "out")
-        env = dict(os.environ, SEMIRING=name, XVEC=str(xf), CROSSCHECK=prefix)
+        env = dict(os.environ, SEMIRING=name, XVEC=str(xf), CROSSCHECK=prefix,
+                   GMR_ENERGY_SECONDS="0.5" if sr == 0 else "0")
         _say("gpu_test", name)
         r = subprocess.run([exe, base, "3000", "1024", "2", "repair", "4"], capture_output=True, text=True, env=env,
                            timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         assert "GPU kernel:" in r.stdout and "Batched verification" in r.stdout and "SUCCESS" in r.stdout
+        # energy per vector between two counter updates >= 0.5 s apart: a
+        # positive reading, or -1 where the counter cannot be read
+        mj = float(r.stdout.split("ENERGY mJ/vector:")[1].split()[0])
+        assert (mj > 0 or mj == -1) if sr == 0 else mj == -1
         xd = np.fromfile(prefix + f".{name}.x", np.float32)
         yd = np.fromfile(prefix + f".{name}.y", np.float64)
         np.testing.assert_array_equal(xd, x)

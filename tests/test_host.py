@@ -291,3 +291,22 @@ def test_inline_top_keeps_the_matrix(name, m):
                 np.testing.assert_allclose(y, y_bf, rtol=1e-9, atol=1e-9 * (1 + den.max()))
             else:
                 np.testing.assert_array_equal(y, y_bf)
+
+
+def test_csr_arrays_from_the_csrv_string():
+    """The vendor baseline's CSR conversion reproduces the dense matrix,
+    including empty rows at the end and in the middle."""
+    from gmr import vendor
+    rng = np.random.default_rng(5)
+    M = rng.integers(1, 4, (40, 17)) * (rng.random((40, 17)) < 0.3)
+    M[7] = 0
+    M[-3:] = 0
+    m = gmr.Matrix.from_dense(M.astype(np.float64))
+    seq, V = m.arrays()
+    rp, col, vals = vendor.csr_arrays(seq, V, 40, 17)
+    assert rp.dtype == np.int32 and len(rp) == 41 and rp[0] == 0 and rp[-1] == len(col)
+    D = np.zeros((40, 17))
+    for r in range(40):
+        for i in range(rp[r], rp[r + 1]):
+            D[r, col[i]] += vals[i]
+    np.testing.assert_array_equal(D, M)
