@@ -81,6 +81,9 @@ def main():
                     help="comma-separated vector counts: throughput per B on this workload (own JSON line)")
     ap.add_argument("--left-B", default=None,
                     help="comma-separated vector counts: the left product X^T M per B on this workload (own JSON line)")
+    ap.add_argument("--blocks", type=int, default=0,
+                    help="compress the matrix as this many row-block grammars (matrepair -b) and sweep them in one "
+                         "batch launch")
     ap.add_argument("--traverse", action="store_true",
                     help="rmat workload: also run the fixpoint traversal (reachability / shortest paths)")
     ap.add_argument("--rows", type=int, default=None)
@@ -133,6 +136,8 @@ def main():
     else:
         m = gmr.Matrix.haplotype(args.seed, args.rows, args.cols)
     nnz = m.nnz()
+    if args.blocks:
+        return run_blocks(args, m)
     base = None
     if args.grammar_cache:
         os.makedirs(args.grammar_cache, exist_ok=True)
@@ -536,6 +541,56 @@ def run_sweep_B(args, m, forms):
                       "grammar": {"rules": info["num_raw_nt"], "depth": info["max_depth"],
                                   "emissions": info["emit_total"], "term_emissions": info["num_term_emit"]},
                       "per_B": out}), flush=True)
+
+
+def run_blocks(args, m):
+    """The matrix as args.blocks row-block grammars (mm-repair's -b blocks),
+    all swept in one batch launch per step (each variant timed, the fastest
+    reported), with the FP32 error of four columns against the FP64 oracle."""
+    import numpy as np
+    import torch
+
+    import gmr
+    from gmr.blocks import BlockGrammar
+    from oracle import oracle as O
+    B, nnz = args.B, m.nnz()
+    it = str(args.inline_top)
+    n_inline = (2 if B >= 32 else 0) if it == "auto" else int(it)
+    bg = BlockGrammar(m, args.blocks, inline_top=n_inline)
+    log(f"{args.blocks} row-block grammars built in {bg.build_seconds:.1f}s", bg.info())
+    gen = torch.Generator(device="cuda").manual_seed(args.seed)
+    X = torch.randn(args.cols, B, device="cuda", generator=gen)
+    if B == 1:
+        X = X[:, 0].contiguous()
+    Y = bg.run_torch(0, X)
+    per = {}
+    for vname, v in gmr.VARIANTS.items():
+        for _ in range(args.warmup):
+            bg.run_torch(0, X, Y, variant=v)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            bg.run_torch(0, X, Y, variant=v)
+        e1.record()
+        torch.cuda.synchronize()
+        if v == gmr.VARIANT_PERSISTENT:
+            bg.batch.check()
+        per[vname] = e0.elapsed_time(e1) / args.steps
+        log(f"blocks {vname}: {per[vname]:.3f} ms")
+    best = min(per, key=per.get)
+    bg.run_torch(0, X, Y, variant=gmr.VARIANTS[best])
+    seq, V = m.arrays()
+    cols_chk = [0, B // 3, (2 * B) // 3, B - 1]
+    Xs = X.reshape(args.cols, -1)[:, cols_chk].contiguous().cpu().numpy()
+    Ys, den = O.brute_force_batched(0, args.rows, args.cols, seq, V, Xs)
+    d = np.abs(Y.reshape(args.rows, -1)[:, cols_chk].cpu().numpy().astype(np.float64) - Ys)
+    err = float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1), np.where(d == 0, 0, np.inf))))
+    print(json.dumps({"metric": METRIC, "workload": f"{args.workload}{args.rows}x{args.cols}_B{B}", "nnz": nnz,
+                      "blocks": args.blocks, "inline_top": n_inline, "build_s": round(bg.build_seconds, 1),
+                      "grammar": bg.info(), "per_variant_ms": {k: round(v, 4) for k, v in per.items()},
+                      "variant": best, "ms": per[best], "value": nnz * B / (per[best] * 1e-3),
+                      "unit": "nnz*vectors/s", "fp32_err": err}), flush=True)
 
 
 def run_left_B(args, m, forms):

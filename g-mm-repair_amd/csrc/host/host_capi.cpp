@@ -113,6 +113,29 @@ int gmr_matrix_from_csrv(int rows, int cols, const int* seq, long long len, cons
     });
 }
 
+int gmr_matrix_row_block(const gmr_matrix* m, int r0, int r1, gmr_matrix** out) {
+    if (!m || !out || r0 < 0 || r1 < r0 || r1 > m->m.rows) return fail(GMR_ERR_INVALID, "bad row block");
+    return guarded([&] {
+        // Row r's terminals start after the r-th 0 of the CSRV string (every
+        // row, empty ones too, ends with its own 0).
+        const auto& seq = m->m.seq;
+        std::size_t a = 0, b = 0;
+        int zeros = 0;
+        for (std::size_t i = 0; i < seq.size() && zeros < r1; ++i) {
+            if (seq[i] != 0) continue;
+            ++zeros;
+            if (zeros == r0) a = i + 1;
+            if (zeros == r1) b = i + 1;
+        }
+        auto* h = new gmr_matrix;
+        h->m.rows = r1 - r0;
+        h->m.cols = m->m.cols;
+        h->m.seq.assign(seq.begin() + static_cast<long>(a), seq.begin() + static_cast<long>(b));
+        h->m.V = m->m.V;
+        *out = h;
+    });
+}
+
 int gmr_matrix_info(const gmr_matrix* m, int* rows, int* cols, int* nvals, int* alpha,
                     long long* len) {
     if (!m) return fail(GMR_ERR_INVALID, "matrix is NULL");

@@ -118,6 +118,18 @@ class Matrix:
         i = self.info()
         return i["len"] - i["rows"]
 
+    def row_block(self, r0, r1):
+        """Rows [r0, r1) as a Matrix of their own (same columns and values)."""
+        h = C.c_void_p()
+        _check(_lib.host().gmr_matrix_row_block(self._h, int(r0), int(r1), C.byref(h)))
+        return Matrix(h)
+
+    def row_blocks(self, n):
+        """n contiguous row blocks of near-equal height: [(r0, r1, Matrix)]."""
+        rows = self.rows
+        cuts = [rows * k // n for k in range(n + 1)]
+        return [(cuts[k], cuts[k + 1], self.row_block(cuts[k], cuts[k + 1])) for k in range(n)]
+
     def repair(self, max_rounds=0, threads=0):
         h = C.c_void_p()
         rounds, secs = C.c_int(), C.c_double()

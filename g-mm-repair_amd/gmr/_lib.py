@@ -74,6 +74,7 @@ def host():
     _sig(L, "gmr_matrix_from_csrv", C.c_int, [C.c_int, C.c_int, c_int_p, C.c_longlong, c_double_p, C.c_int, vpp])
     _sig(L, "gmr_matrix_info", C.c_int, [vp, c_int_p, c_int_p, c_int_p, c_int_p, c_ll_p])
     _sig(L, "gmr_matrix_copy", C.c_int, [vp, c_int_p, c_double_p])
+    _sig(L, "gmr_matrix_row_block", C.c_int, [vp, C.c_int, C.c_int, vpp])
     _sig(L, "gmr_matrix_free", None, [vp])
     _sig(L, "gmr_grammar_repair", C.c_int, [vp, C.c_int, C.c_int, vpp, c_int_p, c_double_p])
     _sig(L, "gmr_grammar_from_arrays", C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, C.c_longlong, c_double_p, C.c_int, vpp])

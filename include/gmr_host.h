@@ -61,6 +61,9 @@ int gmr_matrix_from_csrv(int rows, int cols, const int* seq, long long len, cons
 int gmr_matrix_info(const gmr_matrix* m, int* rows, int* cols, int* nvals, int* alpha,
                     long long* len);
 int gmr_matrix_copy(const gmr_matrix* m, int* seq, double* V);
+/* Rows [r0, r1) as a matrix of their own (same columns and values): the row
+ * blocks mm-repair's `matrepair -b` compresses separately (run_benchmark_16.py:88-91). */
+int gmr_matrix_row_block(const gmr_matrix* m, int r0, int r1, gmr_matrix** out);
 void gmr_matrix_free(gmr_matrix* m);
 
 /* ---- grammars */

@@ -310,3 +310,24 @@ def test_csr_arrays_from_the_csrv_string():
         for i in range(rp[r], rp[r + 1]):
             D[r, col[i]] += vals[i]
     np.testing.assert_array_equal(D, M)
+
+
+def test_row_blocks_partition_the_csrv_string():
+    rng = np.random.default_rng(12)
+    M = rng.integers(0, 3, (23, 9)) * (rng.random((23, 9)) < 0.4)
+    M[5] = 0
+    M[-1] = 0
+    M[0] = 0
+    m = gmr.Matrix.from_dense(M.astype(np.float64))
+    seq = m.arrays()[0]
+    for n in (1, 2, 3, 5, 23):
+        blocks = m.row_blocks(n)
+        assert [b[0] for b in blocks][0] == 0 and blocks[-1][1] == 23
+        assert all(b.rows == r1 - r0 and b.cols == 9 for r0, r1, b in blocks)
+        assert np.array_equal(np.concatenate([b.arrays()[0] for _, _, b in blocks]), seq), n
+    sub = m.row_block(4, 9)
+    np.testing.assert_array_equal(gmr.Matrix.from_dense(M[4:9].astype(np.float64)).arrays()[0], sub.arrays()[0])
+    assert m.row_block(7, 7).rows == 0 and len(m.row_block(7, 7).arrays()[0]) == 0
+    for r0, r1 in ((-1, 3), (5, 4), (0, 24)):
+        with pytest.raises(gmr.GmrError):
+            m.row_block(r0, r1)
