@@ -995,11 +995,17 @@ void launch_left(Engine& en, const dev::LeftArgs& a, int block, hipStream_t st) 
     const bool fast = !en.has_runlen;
     if (fast) {
         const long zero_all = a.Bt == 1 ? (static_cast<long>(en.NL) + en.U) * a.Bs : 0;
-        const long work = std::max<long>(static_cast<long>(en.rows + 1) * a.Bs,
+        // (+,x) over whole rows of X (the row segments of B > 1 never read the
+        // padding row): the gathers read X in place of a converted copy.
+        const bool x_direct = std::is_same<S, dev::PlusTimesOp>::value && a.Bt > 1 && a.Bt == a.B && a.Bs == a.B;
+        const long work = std::max<long>(x_direct ? 0 : static_cast<long>(en.rows + 1) * a.Bs,
                                          zero_all ? zero_all : static_cast<long>(en.n_lsplit) * a.Bs);
-        hipLaunchKernelGGL(dev::k_left_init<S>, dim3(grid_for(work, block, 4096)), dim3(block), 0, st, a.X, a.B, a.Bt,
-                           a.Bs, en.rows, en.ws_Xs, en.ws_A, en.d_lsplit, static_cast<long>(en.n_lsplit), zero_all);
-        const SweepArgs g = left_gather_args(en, a.Bt, a.Bs);
+        if (work > 0)
+            hipLaunchKernelGGL(dev::k_left_init<S>, dim3(grid_for(work, block, 4096)), dim3(block), 0, st, a.X, a.B,
+                               a.Bt, a.Bs, en.rows, x_direct ? nullptr : en.ws_Xs, en.ws_A, en.d_lsplit,
+                               static_cast<long>(en.n_lsplit), zero_all);
+        SweepArgs g = left_gather_args(en, a.Bt, a.Bs);
+        if (x_direct) g.Tv = const_cast<float*>(a.X);
         const int P = 8 * en.lx_passes;
         // group gi = level gi + 1 for gi < L, the terminals for gi = L
         auto group = [&](int gi) {

@@ -159,9 +159,10 @@ __global__ void k_left_yzero(LeftArgs a, int cols) {
 
 // Before a sweep without run-length rules (which runs on the row-gather
 // kernels): Xs[r*Bs + b] = x as a semiring value for b < Bt, the zero elsewhere
-// and on the extra row `rows` (the target of padding entries); then the node
-// rows that are combined atomically are zeroed -- every node (one vector) or
-// the listed split nodes.
+// and on the extra row `rows` (the target of padding entries) -- skipped when
+// Xs is null because the gathers can read X itself; then the node rows that
+// are combined atomically are zeroed -- every node (one vector) or the listed
+// split nodes.
 template <class S>
 __global__ void k_left_init(const float* X, int B, int Bt, int Bs, int rows, void* Xs_, void* A_, const int* zero_nodes,
                             long n_zero, long zero_all) {
@@ -170,7 +171,7 @@ __global__ void k_left_init(const float* X, int B, int Bt, int Bs, int rows, voi
     T* A = tp<S>(A_);
     const long tid = xcd_tid();
     const long nth = static_cast<long>(gridDim.x) * blockDim.x;
-    const long nx = static_cast<long>(rows + 1) * Bs;
+    const long nx = Xs ? static_cast<long>(rows + 1) * Bs : 0;  // Xs = nullptr: the gathers read X itself
     for (long i = tid; i < nx; i += nth) {
         const long r = i / Bs;
         const int b = static_cast<int>(i - r * Bs);
