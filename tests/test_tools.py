@@ -46,3 +46,25 @@ def test_gpu_test_argument_and_input_errors(tmp_path):
     np.zeros(255, np.float32).tofile(bad)   # cols is 256
     r = subprocess.run([exe, base, "600", "256"], capture_output=True, text=True, env=dict(os.environ, XVEC=str(bad)))
     assert r.returncode == 1 and "expected cols=256" in r.stderr
+
+
+def test_rocsparse_test_argument_errors(tmp_path):
+    """The vendor driver rejects what the reference's cusparse_test rejects,
+    before any GPU call."""
+    exe = os.path.join(BIN, "rocsparse_test")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    r = subprocess.run([exe, str(tmp_path / "missing"), "10", "10"], capture_output=True, text=True)
+    assert r.returncode == 1 and "Error" in r.stderr
+    m, g, base = _grammar(tmp_path)
+    r = subprocess.run([exe, base, "600", "256", "5", "recompress"], capture_output=True, text=True)
+    assert r.returncode == 1 and "no longer supported" in r.stderr
+    r = subprocess.run([exe, base, "599", "256"], capture_output=True, text=True)  # one row too few
+    assert r.returncode == 1 and "more than 599 rows" in r.stderr
+
+
+def test_bench_measures_one_gpu_only():
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
