@@ -276,10 +276,14 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
 
 // Passes that keep each XCD's share of one value range near 3 MB (a 4 MB L2
 // per XCD), at most 8.
-int Engine::auto_xcd_passes() const {
-    const double bytes = (static_cast<double>(U) + NL) * 4.0;
+int Engine::auto_xcd_passes(int Bs, int Bt) const {
+    const double bytes = (static_cast<double>(U) + NL) * 4.0 * Bs;
     const double per = 8.0 * 3.0 * 1024 * 1024;
-    return std::max(1, std::min(8, static_cast<int>(std::ceil(bytes / per))));
+    if (Bs == 1) return std::max(1, std::min(8, static_cast<int>(std::ceil(bytes / per))));
+    // Several vectors: every row piece costs Bt atomics, so fewer passes than
+    // the cache size alone asks for (measured on the 500,000 x 10,000 genotype
+    // grammar: 4 passes best at Bt = 2, 2 at Bt = 4).
+    return std::max(1, std::min(8, static_cast<int>(std::lround(bytes / (per * Bt)))));
 }
 
 namespace {
@@ -612,10 +616,11 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
     a.resident = en.resident ? 1 : 0;
     a.gather = gather ? 1 : 0;
     a.flat = (gather && a.Bt == 1 && B == 1 && !(o.flags & GMR_RUN_ROW_SEGMENTS)) ? 1 : 0;
-    a.xparts = (a.flat && en.resident && !(o.flags & GMR_RUN_SINGLE_LIST)) ? 1 : 0;
+    const bool narrow = gather && en.resident && a.vec && a.Bs == 4 && !(o.flags & GMR_RUN_ROW_SEGMENTS);
+    a.xparts = ((a.flat && en.resident && !(o.flags & GMR_RUN_SINGLE_LIST)) || narrow) ? 1 : 0;
     if (a.xparts) {
         if (o.xcd_passes < 0 || o.xcd_passes > 8) throw std::invalid_argument("xcd_passes must be in [0, 8]");
-        const int passes = o.xcd_passes > 0 ? o.xcd_passes : en.auto_xcd_passes();
+        const int passes = o.xcd_passes > 0 ? o.xcd_passes : en.auto_xcd_passes(a.Bs, a.Bt);
         const Engine::XParts& x = en.xparts_for(passes);
         a.xsym = x.d_sym;
         a.xbits = x.d_bits;

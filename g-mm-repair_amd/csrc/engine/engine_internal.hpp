@@ -55,7 +55,7 @@ struct Engine {
     };
     std::map<int, XParts> xparts;
     const XParts& xparts_for(int passes);
-    int auto_xcd_passes() const;
+    int auto_xcd_passes(int Bs = 1, int Bt = 1) const;  // pass count of the XCD-partitioned gather
     std::vector<int> crow_host;      // [0] = number of top-sequence symbols
     int* d_level_off = nullptr;
     int* d_emit_off = nullptr;

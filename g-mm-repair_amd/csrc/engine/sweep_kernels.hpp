@@ -609,6 +609,74 @@ __device__ void ph_gather_parts(const SweepArgs& a, long blk, long nblk, int tid
     }
 }
 
+// Two to four vectors (Bs = 4), XCD-partitioned: the same lists as one
+// vector, with a 16-byte value row per symbol (one dwordx4 per lane) and the
+// segmented scan and the atomics on its four components.
+template <class S>
+__device__ void ph_gather_parts4(const SweepArgs& a, long blk, long nblk, int tid_in_blk, int bdim) {
+    using T = typename S::T;
+    using O = Op4<S>;
+    using T4 = typename O::T4;
+    const T* W = tp<S>(a.W0);
+    const T* Tt = tp<S>(a.Tv);
+    T* Y = tp<S>(a.Y);
+    const int wpb = bdim >> 6, w = tid_in_blk >> 6, lane = tid_in_blk & 63;
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    long p0, pstep, j, nj;
+    if (nblk >= 8) {
+        p0 = blk & 7;
+        pstep = 8;
+        j = blk >> 3;
+        nj = (nblk >> 3) + ((blk & 7) < (nblk & 7) ? 1 : 0);
+    } else {
+        p0 = blk;
+        pstep = nblk;
+        j = 0;
+        nj = 1;
+    }
+    auto finish = [&](long c, T4 v) {
+        const unsigned long long bits = a.xbits[c];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            T4 t;
+            t.x = __shfl_up(v.x, d, 64);
+            t.y = __shfl_up(v.y, d, 64);
+            t.z = __shfl_up(v.z, d, 64);
+            t.w = __shfl_up(v.w, d, 64);
+            if (lane >= d && ((bits >> (lane - d + 1)) & ((1ull << d) - 1ull)) == 0ull) v = O::oplus(v, t);
+        }
+        if (lane == 63 || ((bits >> (lane + 1)) & 1ull) != 0ull) {
+            const long r = a.xrow[a.xord[c] - 1 + __popcll(bits & upto)];
+            O::emit_n(Y + r * a.B, v, a.Bt);
+        }
+    };
+    constexpr int K = kPartChunks;
+    for (long pass = a.xpass0; pass < a.xpass1; ++pass)
+    for (long q = p0; q < 8; q += pstep) {
+        const long p = pass * 8 + q;
+        const long end = a.xpoff[p + 1], stride = nj * wpb;
+        for (long c = a.xpoff[p] + j * wpb + w; c < end; c += K * stride) {
+            int sy[K];
+            T4 v[K];
+#pragma unroll
+            for (int u = 0; u < K; ++u) sy[u] = c + u * stride < end ? a.xsym[(c + u * stride) * 64 + lane] : -1;
+#pragma unroll
+            for (int u = 0; u < K; ++u)
+                v[u] = sy[u] >= 0 ? O::ld(W + static_cast<long>(sy[u]) * 4) : O::ld(Tt + static_cast<long>(-sy[u] - 1) * 4);
+#pragma unroll
+            for (int u = 0; u < K; ++u)
+                if (c + u * stride < end) finish(c + u * stride, v[u]);  // the same for the whole wave
+        }
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void ph_gather_parts_any(const SweepArgs& a, long blk, long nblk, int tid_in_blk,
+                                                    int bdim) {
+    if (a.vec) ph_gather_parts4<S>(a, blk, nblk, tid_in_blk, bdim);
+    else ph_gather_parts<S>(a, blk, nblk, tid_in_blk, bdim);
+}
+
 template <class S>
 __device__ __forceinline__ void ph_gather_range(const SweepArgs& a, long r0, long r1, long tid, long nth) {
     if (a.flat) {
@@ -888,7 +956,7 @@ __global__ void k_gather_range(SweepArgs a, long r0, long r1) {
 template <class S>
 __global__ void k_gather(SweepArgs a) {
     if (a.xparts) {
-        ph_gather_parts<S>(a, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
+        ph_gather_parts_any<S>(a, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
         return;
     }
     long tid = xcd_tid();
@@ -961,7 +1029,7 @@ __device__ void persistent_sweep(const SweepArgs& a, GridBarrier* gb, unsigned l
         grid_barrier(gb, base + nb * (++nbar), deadline);
     }
     if (a.gather && a.xparts) {
-        ph_gather_parts<S>(a, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
+        ph_gather_parts_any<S>(a, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
     } else if (a.gather) {
         ph_gather<S>(a, tid, nth);
     } else if (a.L == 0) {

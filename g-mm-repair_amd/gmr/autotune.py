@@ -22,7 +22,8 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
     """The search space for one engine and batch width: variant x emission x
     vector tile x launch shape (x gather order for row-gather emission of
     several vectors; x XCD-partitioned (1, 2, 4 or automatic passes),
-    single-list or row-segment gather for one vector).
+    single-list or row-segment gather for one vector; x XCD-partitioned or
+    row-segment gather for tiles of 2-4 vectors).
     Persistent shapes are (block, grid cap); a cap of 0 means the engine's
     default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
@@ -34,8 +35,18 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
                 shapes = persistent_shapes if v == VARIANT_PERSISTENT else [(b, 0) for b in blocks]
                 for blk, cap in shapes:
                     for order in ((0, 1) if e == EMIT_GATHER and B > 1 else (0,)):
-                        for fl in ((0, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST) if e == EMIT_GATHER and B == 1 else (0,)):
-                            for passes in ((0, 1, 2, 4) if e == EMIT_GATHER and B == 1 and fl == 0 else (None,)):
+                        # 2-4 vectors per tile: XCD-partitioned lists (they fix the order;
+                        # the default) or row segments (in either order)
+                        narrow = e == EMIT_GATHER and 2 <= t <= 4 and layout == RESIDENT
+                        if e == EMIT_GATHER and B == 1:
+                            flags = (0, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST)
+                        elif narrow:
+                            flags = (0, RUN_ROW_SEGMENTS) if order == 0 else (RUN_ROW_SEGMENTS,)
+                        else:
+                            flags = (0,)
+                        for fl in flags:
+                            ps = (0, 1, 2, 4) if (e == EMIT_GATHER and (B == 1 or narrow) and fl == 0) else (None,)
+                            for passes in ps:
                                 c = dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap,
                                          gather_order=order)
                                 if fl:

@@ -53,11 +53,13 @@ enum gmr_eval_kind {
 
 enum gmr_run_flags {
     GMR_RUN_TIMED = 1,        /* record device time (see gmr_engine_last_kernel_ms) */
-    /* One vector, row gather. By default the top-sequence symbols are split into 8 value
-     * ranges, one per XCD, each swept in 64-symbol chunks, so each XCD's L2 caches an eighth
-     * of the values (in a fused multi-grammar launch the single list below is used). */
+    /* Row gather of one vector, or of 2-4 vectors per tile. By default the top-sequence
+     * symbols are split into 8 x passes value ranges, 8 per pass (one per XCD), each swept in
+     * 64-symbol chunks, so each XCD's L2 caches its share of the values; row pieces combine
+     * atomically into a zeroed y (in a fused multi-grammar persistent launch the
+     * single-vector gather uses the single list below and 2-4 vectors the row segments). */
     GMR_RUN_ROW_SEGMENTS = 2, /* lanes per row segment instead */
-    GMR_RUN_SINGLE_LIST = 4   /* the whole top sequence as one list of 64-symbol chunks instead */
+    GMR_RUN_SINGLE_LIST = 4   /* one vector: the whole top sequence as one chunked list instead */
 };
 
 typedef struct gmr_run_opts {

@@ -531,3 +531,22 @@ def test_inlined_top_rules(hap, sr):
     YL_star, denL = O.brute_force_left(sr, 4096, 2048, seq, V, XL)
     YL = eng.run_left_torch(sr, torch.from_numpy(XL).cuda()).cpu().numpy()
     check(sr, YL, YL_star, denL if sr == 0 else None, f"inlined left sr={sr}")
+
+
+@pytest.mark.parametrize("sr", [0, 1, 2])
+def test_narrow_tiles_on_the_partitioned_gather(hap, sr):
+    """Tiles of 2-4 vectors through the XCD-partitioned gather (their default)
+    and the row segments: several pass counts, every variant, and a 10-vector
+    product cut into tiles of 4, 4 and 2."""
+    m, g, seq, V = hap
+    eng = g.schedule(4096, 2048, "resident").engine()
+    for B, tile in ((2, 0), (3, 0), (4, 0), (10, 4)):
+        X = x_for(sr, 2048, 80 + B, B=B)
+        Y_star, den = O.brute_force_batched(sr, 4096, 2048, seq, V, X)
+        for var in ("levels", "graph", "persistent"):
+            for flags, passes in ((0, 0), (0, 1), (0, 3), (0, 8), (gmr.RUN_ROW_SEGMENTS, 0)):
+                _say("narrow", sr, B, tile, var, flags, passes)
+                Y = run_engine(eng, sr, X, variant=gmr.VARIANTS[var], emit=gmr.EMIT_GATHER, b_tile=tile,
+                               flags=flags, xcd_passes=passes)
+                check(sr, Y, Y_star, den if sr == 0 else None,
+                      f"narrow sr={sr} B={B} tile={tile} {var} flags={flags} p={passes}")
