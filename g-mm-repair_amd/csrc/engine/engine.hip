@@ -283,7 +283,7 @@ int Engine::auto_xcd_passes(int Bs, int Bt) const {
     // Several vectors: every row piece costs Bt atomics, so fewer passes than
     // the cache size alone asks for (measured on the 500,000 x 10,000 genotype
     // grammar: 4 passes best at Bt = 2, 2 at Bt = 4).
-    return std::max(1, std::min(8, static_cast<int>(std::lround(bytes / (per * Bt)))));
+    return std::max(1, std::min(8, static_cast<int>(std::ceil(bytes / (per * Bt)))));
 }
 
 namespace {
