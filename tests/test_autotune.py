@@ -77,3 +77,16 @@ def test_tune_forms_keeps_the_fastest_form(monkeypatch):
     assert len(tuned) == 2 * len(cands) and {lab for lab, _, _ in tuned} == {"as built", "inlined"}
     assert min(ms for _, _, ms in tuned) == 0.5
 
+
+
+def test_candidates_for_narrow_tiles_cover_both_gathers():
+    """Tiles of 2-4 vectors: the XCD-partitioned gather (the default, with
+    its pass counts) and the row segments in both orders."""
+    for B in (2, 3, 4):
+        d = {autotune.describe(c) for c in autotune.candidates(B, gmr.RESIDENT)}
+        for p in ("auto", "1", "2", "4"):
+            assert f"levels/gather/tile=B/blk=256/xcd-passes={p}" in d
+        assert "levels/gather/tile=B/blk=256/segments" in d
+        assert "levels/gather/tile=B/blk=256/order=rule/segments" in d
+    d16 = {autotune.describe(c) for c in autotune.candidates(16, gmr.RESIDENT)}
+    assert not any("xcd-passes" in x or "segments" in x for x in d16)
