@@ -609,14 +609,16 @@ __device__ void ph_gather_parts(const SweepArgs& a, long blk, long nblk, int tid
     }
 }
 
-// Two to four vectors (Bs = 4), XCD-partitioned: the same lists as one
-// vector, with a 16-byte value row per symbol (one dwordx4 per lane) and the
-// segmented scan and the atomics on its four components.
-template <class S>
+// Several vectors (Bs = 4 NV), XCD-partitioned: the same lists as one vector,
+// with a 16 NV-byte value row per symbol (NV dwordx4 per lane) and the
+// segmented scan and the atomics on its 4 NV components. Used for Bs = 4; at
+// Bs = 8 (32-byte rows) it measured no faster than the row segments.
+template <class S, int NV>
 __device__ void ph_gather_parts4(const SweepArgs& a, long blk, long nblk, int tid_in_blk, int bdim) {
     using T = typename S::T;
     using O = Op4<S>;
     using T4 = typename O::T4;
+    constexpr int BS = 4 * NV;
     const T* W = tp<S>(a.W0);
     const T* Tt = tp<S>(a.Tv);
     T* Y = tp<S>(a.Y);
@@ -634,20 +636,26 @@ __device__ void ph_gather_parts4(const SweepArgs& a, long blk, long nblk, int ti
         j = 0;
         nj = 1;
     }
-    auto finish = [&](long c, T4 v) {
+    auto finish = [&](long c, T4* v) {
         const unsigned long long bits = a.xbits[c];
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
-            T4 t;
-            t.x = __shfl_up(v.x, d, 64);
-            t.y = __shfl_up(v.y, d, 64);
-            t.z = __shfl_up(v.z, d, 64);
-            t.w = __shfl_up(v.w, d, 64);
-            if (lane >= d && ((bits >> (lane - d + 1)) & ((1ull << d) - 1ull)) == 0ull) v = O::oplus(v, t);
+            const bool take = lane >= d && ((bits >> (lane - d + 1)) & ((1ull << d) - 1ull)) == 0ull;
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                T4 t;
+                t.x = __shfl_up(v[n].x, d, 64);
+                t.y = __shfl_up(v[n].y, d, 64);
+                t.z = __shfl_up(v[n].z, d, 64);
+                t.w = __shfl_up(v[n].w, d, 64);
+                if (take) v[n] = O::oplus(v[n], t);
+            }
         }
         if (lane == 63 || ((bits >> (lane + 1)) & 1ull) != 0ull) {
             const long r = a.xrow[a.xord[c] - 1 + __popcll(bits & upto)];
-            O::emit_n(Y + r * a.B, v, a.Bt);
+#pragma unroll
+            for (int n = 0; n < NV; ++n)
+                if (4 * n < a.Bt) O::emit_n(Y + r * a.B + 4 * n, v[n], a.Bt - 4 * n);
         }
     };
     constexpr int K = kPartChunks;
@@ -657,12 +665,15 @@ __device__ void ph_gather_parts4(const SweepArgs& a, long blk, long nblk, int ti
         const long end = a.xpoff[p + 1], stride = nj * wpb;
         for (long c = a.xpoff[p] + j * wpb + w; c < end; c += K * stride) {
             int sy[K];
-            T4 v[K];
+            T4 v[K][NV];
 #pragma unroll
             for (int u = 0; u < K; ++u) sy[u] = c + u * stride < end ? a.xsym[(c + u * stride) * 64 + lane] : -1;
 #pragma unroll
-            for (int u = 0; u < K; ++u)
-                v[u] = sy[u] >= 0 ? O::ld(W + static_cast<long>(sy[u]) * 4) : O::ld(Tt + static_cast<long>(-sy[u] - 1) * 4);
+            for (int u = 0; u < K; ++u) {
+                const T* row = sy[u] >= 0 ? W + static_cast<long>(sy[u]) * BS : Tt + static_cast<long>(-sy[u] - 1) * BS;
+#pragma unroll
+                for (int n = 0; n < NV; ++n) v[u][n] = O::ld(row + 4 * n);
+            }
 #pragma unroll
             for (int u = 0; u < K; ++u)
                 if (c + u * stride < end) finish(c + u * stride, v[u]);  // the same for the whole wave
@@ -673,7 +684,7 @@ __device__ void ph_gather_parts4(const SweepArgs& a, long blk, long nblk, int ti
 template <class S>
 __device__ __forceinline__ void ph_gather_parts_any(const SweepArgs& a, long blk, long nblk, int tid_in_blk,
                                                     int bdim) {
-    if (a.vec) ph_gather_parts4<S>(a, blk, nblk, tid_in_blk, bdim);
+    if (a.vec) ph_gather_parts4<S, 1>(a, blk, nblk, tid_in_blk, bdim);  // Bs = 4 only
     else ph_gather_parts<S>(a, blk, nblk, tid_in_blk, bdim);
 }
 
