@@ -78,9 +78,9 @@ class Csr:
         return int(_lib.vendor().gmr_csr_device_bytes(self._h))
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            _lib.vendor().gmr_csr_free(h)
+        lib = getattr(_lib, "_vendor", None) if _lib is not None else None
+        if getattr(self, "_h", None) and lib is not None:
+            lib.gmr_csr_free(self._h)
             self._h = None
 
 
