@@ -67,12 +67,14 @@ def test_unit_grammar_on_gpu():
     f = fixture("unit_grammar")
     g = gmr.Grammar.from_arrays(f["alpha"], f["left"], f["right"], f["C"], f["V"], kind=f["kind"], rep=f["rep"])
     x = np.array(f["x"], np.float32)
-    for lay in ("refcompat", "wave64", "resident"):
-        eng = g.schedule(2, 3, lay).engine()
-        for var, em in (RES_CASES if lay == "resident" else CASES):
-            _say("unit", lay, var, em)
-            y = run_engine(eng, 0, x, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
-            np.testing.assert_allclose(y, f["expect_y"], atol=1e-6)
+    # as given, and with its top rule inlined (the run-length rule it uses stays)
+    for gg in (g, g.inline_top(2)):
+        for lay in ("refcompat", "wave64", "resident"):
+            eng = gg.schedule(2, 3, lay).engine()
+            for var, em in (RES_CASES if lay == "resident" else CASES):
+                _say("unit", lay, var, em)
+                y = run_engine(eng, 0, x, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+                np.testing.assert_allclose(y, f["expect_y"], atol=1e-6)
 
 
 def test_readme_example_on_gpu():
