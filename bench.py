@@ -45,6 +45,16 @@ def dag_bytes(info, B):
     return 8 * R + 8 * (E + Et) + 8 * (U - 1) + 4 * B * (3 * R + 3 * (E + Et) + 2 * (U - 1) + rows)
 
 
+def layered_bytes(info_ref, B):
+    """The reference-layered count of one sweep (RefCompat schedule of the
+    grammar as built): per layered rule a 12-byte record and per vector 2 reads
+    and 1 write, emissions and terminals as in the DAG count, plus the
+    double-buffered frontier fills (2 w* per vector)."""
+    RL, E, Et, U = info_ref["num_layered"], info_ref["emit_total"], info_ref["num_term_emit"], info_ref["U"]
+    rows, w = info_ref["rows"], info_ref["max_width"]
+    return 12 * RL + 8 * E + 8 * Et + 8 * (U - 1) + 4 * B * (3 * RL + 3 * E + 3 * Et + 2 * (U - 1) + rows + 2 * w)
+
+
 def gather_bytes(info, B, n_seg=0):
     """Algorithmic bytes of one k_gather launch over B vectors (Resident layout,
     row gather): the row offsets, the segment table, one symbol id per top-
@@ -300,8 +310,11 @@ def main():
     # device time at B = 64); the whole sweep is reported beside it
     workload = f"{args.workload}{args.rows}x{args.cols}_B{B}"
     sweep_traffic = read_traffic(workload, "sweep")
-    sweep = {"ms": ms_step, "bytes_dag": dag_bytes(info, B), "traffic": sweep_traffic,
-             "note": "DAG-model bytes charge every emission a y read-modify-write that row gather does not do"}
+    sweep = {"ms": ms_step, "bytes_dag": dag_bytes(info, B), "bytes_reference_layered": layered_bytes(info_ref, B),
+             "traffic": sweep_traffic,
+             "note": "DAG-model bytes (of the grammar form run) charge every emission a y read-modify-write that "
+                     "row gather does not do; the reference-layered count is that of the reference's own schedule "
+                     "of the grammar as built"}
     if sweep_traffic:
         sweep["achieved_measured"] = round(sweep_traffic / (kern_ms * 1e-3) / 1e9, 1)
         sweep["frac_measured"] = round(sweep_traffic / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
