@@ -27,7 +27,7 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
     Persistent shapes are (block, grid cap); a cap of 0 means the engine's
     default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
-    tiles = sorted({B} | {t for t in (16, 32, 64) if t < B})
+    tiles = sorted({B} | {t for t in (16, 32, 64, 128) if t < B})
     out = []
     for v in (VARIANT_LEVELS, VARIANT_GRAPH, VARIANT_PERSISTENT):
         for e in emits:
