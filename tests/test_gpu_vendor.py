@@ -63,7 +63,7 @@ def test_energy_counter():
     from gmr import vendor
     e0 = vendor.energy_uj(0)
     if e0 is None:
-        pytest.skip("the energy counter cannot be read on this GPU box")
+        pytest.skip("the energy counter cannot be read on this machine")
     A = vendor.Csr.from_matrix(gmr.Matrix.haplotype(3, 2048, 1024))
     mj, w = A.energy(B=4, seconds=1.0)  # the counter advances about once a second
     assert mj > 0 and 10 < w < 5000, (mj, w)
