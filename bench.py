@@ -256,7 +256,8 @@ def main():
     # phase, on the stream the kernels run on, over as many sweeps as were timed
     gather_ms = None
     if run_kw.get("emit") == gmr.EMIT_GATHER:
-        lv = eng.time_levels(0, X, Y, reps=args.steps, emit=gmr.EMIT_GATHER, block=run_kw.get("block", 0))
+        lv = eng.time_levels(0, X, Y, reps=args.steps, emit=gmr.EMIT_GATHER, block=run_kw.get("block", 0),
+                             flags=run_kw.get("flags", 0), xcd_passes=run_kw.get("xcd_passes", 0) or 0)
         gather_ms = float(lv[info["max_depth"] + 1])
         log(f"phases (ms, mean of {args.steps}): init {lv[0]:.4f}, node updates {float(lv[1:-1].sum()):.4f}, "
             f"k_gather {gather_ms:.4f}")
@@ -280,7 +281,7 @@ def main():
 
     # ---- the plain per-level sweep (reference layout, B = 1, spot atomics)
     eng_ref = s_ref.engine()
-    x1 = X[:, 0].contiguous()
+    x1 = X.reshape(args.cols, -1)[:, 0].contiguous()
     y1 = torch.empty(args.rows, device="cuda", dtype=torch.float32)
     plain = dict(variant=gmr.VARIANT_LEVELS, emit=gmr.EMIT_SPOT, b_tile=0, block=256)
     for _ in range(3):
@@ -332,7 +333,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         sa = s_ref.arrays()
-        xs = X[:, 0].cpu().numpy()
+        xs = X.reshape(args.cols, -1)[:, 0].cpu().numpy()
         O.set_threads(threads)
         O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)  # warm-up
         times, t0 = [], time.time()
@@ -414,18 +415,25 @@ def vendor_leg(args, m, X, Y, eng, run_kw, B, nnz, value):
             log("rocSPARSE", alg, "unsupported:", e)
             per_alg[alg] = None
             continue
+        Yv.zero_()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(args.steps):
             A.run_torch(X, Yv, alg=alg)
         e1.record()
         torch.cuda.synchronize()
+        # a timing only counts if the timed calls produced the product
+        if float((Yv - Y).abs().max() / Y.abs().max().clamp_min(1e-30)) > 1e-4:
+            log("rocSPARSE", alg, "timed calls did not produce the product; dropped")
+            per_alg[alg] = None
+            continue
         per_alg[alg] = e0.elapsed_time(e1) / args.steps
         log(f"rocSPARSE {'SpMM' if B > 1 else 'SpMV'} ({alg}): {per_alg[alg]:.3f} ms")
     ok = [a for a in per_alg if per_alg[a] is not None]
     if not ok:
         return None, None
     best = min(ok, key=per_alg.get)
+    Yv.zero_()
     A.run_torch(X, Yv, alg=best)
     torch.cuda.synchronize()
     diff = float((Yv - Y).abs().max() / Y.abs().max().clamp_min(1e-30))

@@ -22,8 +22,8 @@ enum gmr_csr_alg {
     GMR_CSR_ALG_DEFAULT = 0,   /* rocSPARSE's default for the format */
     GMR_CSR_ALG_ROW = 1,       /* SpMV: adaptive; SpMM: csr (row split, LDS) */
     GMR_CSR_ALG_ROW_SPLIT = 2, /* SpMV: row split; SpMM: csr_row_split (shuffles) */
-    GMR_CSR_ALG_NNZ_SPLIT = 3  /* SpMV: LRB (long-row binning); SpMM: csr_merge (nnz split);
-                                  an algorithm rocSPARSE rejects returns GMR_ERR_DEVICE */
+    GMR_CSR_ALG_NNZ_SPLIT = 3  /* SpMM only: csr_merge (nnz split); GMR_ERR_INVALID for B = 1.
+                                  An algorithm rocSPARSE rejects returns GMR_ERR_DEVICE */
 };
 
 /* Upload a CSR matrix (host arrays: row_ptr[rows+1], col_idx[nnz], vals[nnz],
