@@ -698,7 +698,7 @@ def run_rmat(args):
             x[rng.choice(n, 1024, replace=False)] = 0.0
         xt = torch.from_numpy(x).cuda()
         y = torch.empty(n, dtype=torch.int32, device="cuda")
-        form, best, tuned = tune_forms(forms, xt, y, autotune.candidates(1, gmr.RESIDENT), reps=2, semiring=sr)
+        form, best, tuned = tune_forms(forms, xt, y, autotune.candidates(1, gmr.RESIDENT, semiring=sr), reps=2, semiring=sr)
         ms = _time_steps(form["engine"], sr, xt, y, best, args.steps, args.warmup)
         seq, V = m.arrays()
         y_star, _ = O.brute_force(sr, n, n, seq, V, x)

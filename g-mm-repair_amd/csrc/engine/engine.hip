@@ -24,6 +24,7 @@
 #include <string>
 #include <tuple>
 #include <type_traits>
+#include <type_traits>
 #include <vector>
 
 #include "build_schedule.h"
@@ -78,13 +79,15 @@ Engine::~Engine() {
                     static_cast<void*>(d_lrow), ws_trav, static_cast<void*>(d_lxsym), static_cast<void*>(d_lxbits),
                     static_cast<void*>(d_lxord), static_cast<void*>(d_lxrow), static_cast<void*>(d_lxpoff)})
         if (p) (void)hipFree(p);
+    xparts.emplace(-1, bparts);  // the bit-mode lists are freed with the others
     for (auto& kv : xparts)
         for (void* p : {static_cast<void*>(kv.second.d_sym), static_cast<void*>(kv.second.d_bits),
                         static_cast<void*>(kv.second.d_ord), static_cast<void*>(kv.second.d_row),
                         static_cast<void*>(kv.second.d_poff)})
             if (p) (void)hipFree(p);
     void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_zrows, d_fbits, d_ford,
-                    d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds};
+                    d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds,
+                    static_cast<void*>(ws_bits)};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -300,17 +303,22 @@ struct PartLists {
 
 template <class Off, class Key>
 void build_part_lists(long r0, long r1, const Off& off, const std::vector<int>& sym, Key key, long nkey, int P, int pad,
-                      PartLists& out) {
+                      PartLists& out, long span = 0) {
     const long e0 = off[r0], e1 = off[r1], nent = e1 - e0;
-    std::vector<long> uses(static_cast<std::size_t>(nkey) + 1, 0);
-    for (long q = e0; q < e1; ++q) ++uses[key(sym[q]) + 1];
-    for (long k = 0; k < nkey; ++k) uses[k + 1] += uses[k];
     std::vector<long> cut(P + 1);
     cut[0] = 0;
     cut[P] = nkey;
-    for (int p = 1; p < P; ++p)
-        cut[p] = std::max(cut[p - 1],
-                          static_cast<long>(std::lower_bound(uses.begin(), uses.end(), nent * p / P) - uses.begin()));
+    if (span > 0) {
+        // fixed key ranges [p * span, (p + 1) * span)
+        for (int p = 1; p < P; ++p) cut[p] = std::min(nkey, static_cast<long>(p) * span);
+    } else {
+        std::vector<long> uses(static_cast<std::size_t>(nkey) + 1, 0);
+        for (long q = e0; q < e1; ++q) ++uses[key(sym[q]) + 1];
+        for (long k = 0; k < nkey; ++k) uses[k + 1] += uses[k];
+        for (int p = 1; p < P; ++p)
+            cut[p] = std::max(cut[p - 1],
+                              static_cast<long>(std::lower_bound(uses.begin(), uses.end(), nent * p / P) - uses.begin()));
+    }
     auto part = [&](int c) -> int {
         return static_cast<int>(std::upper_bound(cut.begin() + 1, cut.begin() + P, key(c)) - (cut.begin() + 1));
     };
@@ -377,6 +385,35 @@ const Engine::XParts& Engine::xparts_for(int passes) {
     return xparts.emplace(passes, std::move(x)).first->second;
 }
 
+// Lists of the Boolean bit-mode gather: the top-sequence symbols regrouped by
+// bit index (terminal q -> q, rule slot s -> 64 * ceil(U / 64) + s) into ranges
+// of dev::kBitRangeBits bits, one range per list, so a workgroup can hold its
+// list's bits in LDS. Padding reads terminal 0 (always 0).
+const Engine::XParts& Engine::bit_lists() {
+    if (bparts.d_sym) return bparts;
+    if (!resident) throw std::invalid_argument("the bit-mode gather needs the Resident layout");
+    ensure_bits();
+    const long nsym = crow_host[0];
+    std::vector<int> crow(static_cast<std::size_t>(rows) + 1), csym(static_cast<std::size_t>(nsym));
+    GMR_HIP(hipDeviceSynchronize());
+    GMR_HIP(hipMemcpy(crow.data(), d_crow, sizeof(int) * crow.size(), hipMemcpyDeviceToHost));
+    if (nsym) GMR_HIP(hipMemcpy(csym.data(), d_csym, sizeof(int) * nsym, hipMemcpyDeviceToHost));
+    const long tbits = static_cast<long>(bits_tw) * 64;
+    auto key = [&](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : tbits + c; };
+    const long nkey = tbits + NL;
+    const long span = dev::kBitRangeBits;
+    const int P = static_cast<int>(std::max(1L, (nkey + span - 1) / span));
+    PartLists L;
+    build_part_lists(0, rows, crow, csym, key, nkey, P, -1, L, span);
+    bparts.poff = L.poff;
+    bparts.d_sym = dev_upload(L.sym.data(), L.sym.size(), &struct_bytes);
+    bparts.d_bits = dev_upload(L.bits.data(), L.bits.size(), &struct_bytes);
+    bparts.d_ord = dev_upload(L.ord.data(), L.ord.size(), &struct_bytes);
+    bparts.d_row = dev_upload(L.row.data(), L.row.size(), &struct_bytes);
+    bparts.d_poff = dev_upload(bparts.poff.data(), bparts.poff.size(), &struct_bytes);
+    return bparts;
+}
+
 int Engine::check_barrier() {
     dev::GridBarrier h{};
     GMR_HIP(hipDeviceSynchronize());
@@ -418,6 +455,17 @@ void Engine::ensure_workspace(int Bt, int vbytes) {
         if (!resident) GMR_HIP(hipMalloc(&ws_W1, wb));
         ws_W_bytes = wb;
     }
+}
+
+// The bitset of the Boolean bit mode: ceil(U / 64) terminal words, then one
+// word per 64 rule slots. Its size is fixed by the grammar, so it never moves.
+unsigned long long* Engine::ensure_bits() {
+    if (!ws_bits) {
+        bits_tw = (U + 63) / 64;
+        const std::size_t words = static_cast<std::size_t>(bits_tw) + (static_cast<std::size_t>(NL) + 63) / 64 + 1;
+        GMR_HIP(hipMalloc(&ws_bits, words * sizeof(unsigned long long)));
+    }
+    return ws_bits;
 }
 
 namespace {
@@ -465,6 +513,34 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
                    hipEvent_t* level_ev = nullptr) {
     const int G = a.G;
     if (level_ev) GMR_HIP(hipEventRecord(level_ev[0], st));
+    if (a.bits) {
+        const long init = std::max<long>(static_cast<long>(a.tw) * 64, std::max<long>(a.rows, (en.NL + 63) / 64));
+        hipLaunchKernelGGL(dev::k_bits_init, dim3(grid_for(init, block, 4096)), dim3(block), 0, st, a);
+        for (int k = 1; k <= en.L; ++k) {
+            const long lo = en.level_off[k - 1], hi = en.level_off[k];
+            if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * k - 1], st));
+            if (hi > lo)
+                hipLaunchKernelGGL(dev::k_bits_eval, dim3(grid_for(((hi + 63) / 64 - lo / 64) * 64, block)), dim3(block),
+                                   0, st, a, k);
+            if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * k], st));
+        }
+        if (!a.flat) {
+            // one LDS range per list; workgroups take equal shares of all chunks
+            const long nch = en.bparts.poff.back();
+            const long g = std::max(1L, std::min<long>(static_cast<long>(dev::kBitWgPerCu) * en.num_cus, (nch + 63) / 64));
+            if (nch > 0)
+                hipLaunchKernelGGL(dev::k_bits_gather_lds, dim3(static_cast<unsigned>(g)), dim3(dev::kBitGatherBlock), 0,
+                                   st, a);
+        } else if (en.n_fchunk > 0) {
+            hipLaunchKernelGGL(dev::k_bits_gather,
+                               dim3(grid_for((static_cast<long>(en.n_fchunk) + dev::kBitChunks - 1) / dev::kBitChunks * 64,
+                                             block)),
+                               dim3(block), 0, st, a);
+        }
+        if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * en.L + 1], st));
+        GMR_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(dev::k_init<S>, dim3(grid_for(dev::init_items(a), block, 4096)),
                        dim3(block), 0, st, a);
     if (!a.gather && en.Et > 0) {
@@ -630,6 +706,28 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
         a.xpass0 = 0;
         a.xpass1 = passes;
     }
+    if (o.flags & GMR_RUN_BITS) {
+        if (!gather || !en.resident || B != 1 || o.variant == GMR_VARIANT_PERSISTENT)
+            throw std::invalid_argument(
+                "GMR_RUN_BITS needs one vector, gather emission (Resident layout) and the levels or graph variant");
+        a.bits = 1;
+        a.vbits = en.ensure_bits();
+        a.tw = en.bits_tw;
+        a.nvw = static_cast<long>(en.bits_tw) + (static_cast<long>(en.NL) + 63) / 64;
+        a.flat = 1;
+        a.xparts = 0;
+        if (!(o.flags & GMR_RUN_SINGLE_LIST)) {
+            // the LDS-range lists (the default); GMR_RUN_SINGLE_LIST keeps the flat list
+            const Engine::XParts& x = en.bit_lists();
+            a.xsym = x.d_sym;
+            a.xbits = x.d_bits;
+            a.xord = x.d_ord;
+            a.xrow = x.d_row;
+            a.xpoff = x.d_poff;
+            a.nbl = static_cast<int>(x.poff.size()) - 1;
+            a.flat = 0;
+        }
+    }
     a.GR = en.gather_lanes;
     a.eval_kind = o.eval_kind;
     a.level_kind = o.level_kind ? en.device_kinds(o.level_kind) : nullptr;
@@ -642,6 +740,8 @@ void run_all(Engine& en, int semiring, const float* X, int B, void* Y, const gmr
     const int Bt = (o.b_tile > 0 && o.b_tile < B) ? o.b_tile : B;
     const bool gather = o.emit == GMR_EMIT_GATHER;
     if (gather && !en.resident) throw std::invalid_argument("gather emission needs the Resident layout");
+    if ((o.flags & GMR_RUN_BITS) && !std::is_same<S, dev::BooleanOp>::value)
+        throw std::invalid_argument("GMR_RUN_BITS runs the Boolean semiring");
     en.ensure_workspace(padded_width(Bt), static_cast<int>(sizeof(T)));
     if (o.variant == GMR_VARIANT_PERSISTENT && Bt < B) {
         // every vector tile in one launch
@@ -737,6 +837,8 @@ template <class S>
 void time_levels_t(Engine& en, const float* X, int B, void* Y, const gmr_run_opts& o, int reps, float* out,
                    hipStream_t st) {
     using T = typename S::T;
+    if ((o.flags & GMR_RUN_BITS) && !std::is_same<S, dev::BooleanOp>::value)
+        throw std::invalid_argument("GMR_RUN_BITS runs the Boolean semiring");
     en.ensure_workspace(padded_width(B), static_cast<int>(sizeof(T)));
     const SweepArgs a = make_args(en, X, B, Y, static_cast<int>(sizeof(T)), 0, B, o.emit == GMR_EMIT_GATHER, o);
     const int block = o.block > 0 ? o.block : 256;
@@ -1313,6 +1415,7 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
         // the XCD-partitioned gather (which needs workgroup b on XCD b % 8) is
         // replaced there by the single chunked list.
         if (variant == GMR_VARIANT_PERSISTENT) args[g].xparts = 0;
+        if (args[g].bits) throw std::invalid_argument("GMR_RUN_BITS runs only in the levels variant of a batch");
     }
     if (variant == GMR_VARIANT_GRAPH) {
         std::size_t key = mix(static_cast<std::size_t>(semiring), static_cast<std::size_t>(block));

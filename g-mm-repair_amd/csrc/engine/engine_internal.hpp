@@ -54,6 +54,8 @@ struct Engine {
         std::vector<int> poff;       // 8 x passes + 1 chunk offsets
     };
     std::map<int, XParts> xparts;
+    XParts bparts;                   // Boolean bit mode: lists by LDS-sized bit range
+    const XParts& bit_lists();
     const XParts& xparts_for(int passes);
     int auto_xcd_passes(int Bs = 1, int Bt = 1) const;  // pass count of the XCD-partitioned gather
     std::vector<int> crow_host;      // [0] = number of top-sequence symbols
@@ -65,6 +67,9 @@ struct Engine {
     void* ws_W0 = nullptr;
     void* ws_W1 = nullptr;
     std::size_t ws_T_bytes = 0, ws_W_bytes = 0;
+    unsigned long long* ws_bits = nullptr;  // Boolean bit mode: terminal words, then rule words
+    int bits_tw = 0;                 // terminal words of ws_bits: ceil(U / 64)
+    unsigned long long* ensure_bits();
 
     // device facts and timing
     int num_cus = 0, wall_khz = 0;

@@ -39,7 +39,12 @@ struct BooleanOp {
     static __device__ __forceinline__ T leaf(float v, float x) { return (v != 0.0f && x != 0.0f) ? 1 : 0; }
     static __device__ __forceinline__ T apply(T v, float) { return v; }
     static __device__ __forceinline__ bool skip(T v) { return v == 0; }  // OR with 0 is a no-op
-    static __device__ __forceinline__ void emit(T* y, T v) { atomicOr(y, v); }
+    // Values are 0 or 1 and y starts at 0, so OR-ing a 1 in is storing 1: any
+    // number of writers store the same value, with no read-modify-write (a
+    // device-scope atomic would travel to the memory side; a store stays in L2).
+    static __device__ __forceinline__ void emit(T* y, T v) {
+        if (v) *y = 1;
+    }
 };
 
 struct TropicalOp {
@@ -111,6 +116,13 @@ struct SweepArgs {
     int GR;                  // lanes per row in the single-vector gather
     int eval_kind;           // EvalKind for levels without an entry in level_kind
     const int* level_kind;   // device, max_depth + 1 entries (or nullptr)
+    // Boolean, one vector, bit mode: terminal q is bit q of vbits, rule slot s
+    // bit 64*tw + s (tw = terminal words); levels and gather read bits.
+    unsigned long long* vbits;
+    long nvw;                // words of vbits that hold values
+    int tw;
+    int bits;
+    int nbl;                 // bit mode, LDS ranges: lists (in xsym ... xpoff)
 };
 
 // Per-level node-update kernels (one is chosen for each level):
@@ -973,6 +985,199 @@ __global__ void k_gather(SweepArgs a) {
     long tid = xcd_tid();
     long nth = static_cast<long>(gridDim.x) * blockDim.x;
     ph_gather<S>(a, tid, nth);
+}
+
+// ---- Boolean, one vector: values as bits ----
+//
+// Every value of the Boolean sweep is 0 or 1, so the terminal and rule values
+// fit in a bitset of (U + rules) bits: about 1 MB for a grammar of 8 M symbols,
+// which stays in every XCD's L2 while the gather streams the top sequence from
+// HBM. A wave computes 64 consecutive values, packs them with a ballot and
+// writes one 64-bit word; loops advance whole waves together (thread counts
+// and loop bounds are multiples of 64), so every ballot sees its full wave.
+
+// Bit of a child or top-sequence symbol: c >= 0 is a rule slot, c < 0 the
+// terminal -c-1 (the convention of kids and csym).
+__device__ __forceinline__ bool value_bit(const unsigned long long* vb, int tw, int c) {
+    const long k = c >= 0 ? (static_cast<long>(tw) << 6) + c : static_cast<long>(-c - 1);
+    return ((vb[k >> 6] >> (k & 63)) & 1ull) != 0ull;
+}
+
+// Terminal bits (leaf of V and x), zeroed rule words, zeroed y rows.
+__device__ void ph_bits_init(const SweepArgs& a, long tid, long nth) {
+    const int lane = static_cast<int>(tid & 63);
+    for (long i = tid; i < static_cast<long>(a.tw) * 64; i += nth) {
+        const int q = static_cast<int>(i);
+        const bool b = q >= 1 && q < a.U && BooleanOp::leaf(a.tval[q], a.X[a.tcol[q]]) != 0;
+        const unsigned long long m = __ballot(b);
+        if (lane == 0) a.vbits[i >> 6] = m;
+    }
+    const long rw = (static_cast<long>(a.level_off[a.L]) + 63) >> 6;
+    for (long i = tid; i < rw; i += nth) a.vbits[a.tw + i] = 0ull;
+    int* Y = static_cast<int*>(a.Y);
+    for (long i = tid; i < a.rows; i += nth) Y[i] = 0;
+}
+
+// Level k: rule slots [lo, hi), swept as the 64-slot words that hold them. A
+// word wholly inside the level is stored; one shared with a neighbouring level
+// is ORed in (the init phase zeroed it, and only nonzero words are sent).
+__device__ void ph_bits_eval(const SweepArgs& a, int k, long tid, long nth) {
+    const int lane = static_cast<int>(tid & 63);
+    const long lo = a.level_off[k - 1], hi = a.level_off[k];
+    const long s0 = (lo >> 6) << 6, s1 = (hi + 63) >> 6 << 6;
+    for (long s = s0 + tid; s < s1; s += nth) {
+        bool b = false;
+        if (s >= lo && s < hi) {
+            const int2 c = a.kids[s];
+            b = value_bit(a.vbits, a.tw, c.x) || value_bit(a.vbits, a.tw, c.y);
+        }
+        const unsigned long long m = __ballot(b);
+        if (lane == 0) {
+            const long w = s >> 6;
+            unsigned long long* dst = a.vbits + a.tw + w;
+            if ((w << 6) >= lo && (w << 6) + 64 <= hi) *dst = m;
+            else if (m) atomicOr(dst, m);
+        }
+    }
+}
+
+// Gather: 64-symbol chunks of the row-ordered top sequence, kBitChunks
+// consecutive ones per wave in flight. The ballot of the symbols' bits is a
+// chunk's values; a row piece ending at lane l is 1 iff its bits from the
+// piece's first lane to l meet the ballot. Rows wholly inside a chunk are
+// stored, pieces of longer rows ORed in. Everything but the bit words and the
+// row ids is loaded up front, so a chunk costs two dependent round trips.
+// Build-time tuning knobs of the bit mode (defaults: the measured best).
+#ifndef GMR_BIT_CHUNKS
+#define GMR_BIT_CHUNKS 4
+#endif
+#ifndef GMR_BIT_RANGE_LOG2
+#define GMR_BIT_RANGE_LOG2 19
+#endif
+#ifndef GMR_BIT_BLOCK
+#define GMR_BIT_BLOCK 1024
+#endif
+#ifndef GMR_BIT_WG_PER_CU
+#define GMR_BIT_WG_PER_CU 2
+#endif
+constexpr int kBitChunks = GMR_BIT_CHUNKS;
+
+__device__ void ph_bits_gather(const SweepArgs& a, long tid, long nth) {
+    const int lane = static_cast<int>(tid & 63);
+    int* Y = static_cast<int*>(a.Y);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
+    const long nch = a.n_fchunk;
+    const long wid = tid >> 6, nw = nth >> 6;
+    for (long c0 = wid * kBitChunks; c0 < nch; c0 += nw * kBitChunks) {
+        int sym[kBitChunks], ord[kBitChunks];
+        unsigned long long fb[kBitChunks], fnext[kBitChunks];
+#pragma unroll
+        for (int j = 0; j < kBitChunks; ++j) {
+            const long c = c0 + j, p = c * 64 + lane;
+            sym[j] = p < a.n_csym ? a.csym[p] : -1;  // terminal 0: always 0
+            fb[j] = c < nch ? a.fbits[c] : 0ull;
+            fnext[j] = c + 1 < nch ? a.fbits[c + 1] : 1ull;
+            ord[j] = c < nch ? a.ford[c] : 0;
+        }
+        bool b[kBitChunks];
+        int row[kBitChunks];
+#pragma unroll
+        for (int j = 0; j < kBitChunks; ++j) {
+            b[j] = value_bit(a.vbits, a.tw, sym[j]);
+            const unsigned long long st = fb[j] & upto;
+            const long p = (c0 + j) * 64 + lane;
+            const bool last = p < a.n_csym && (lane == 63 || p + 1 == a.n_csym || ((fb[j] >> (lane + 1)) & 1ull) != 0ull);
+            row[j] = last ? a.frow[ord[j] - 1 + __popcll(st)] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kBitChunks; ++j) {
+            if (c0 + j >= nch) break;  // wave-uniform
+            const unsigned long long m = __ballot(b[j]);
+            const long p = (c0 + j) * 64 + lane;
+            const bool last = p < a.n_csym && (lane == 63 || p + 1 == a.n_csym || ((fb[j] >> (lane + 1)) & 1ull) != 0ull);
+            if (last) {
+                const unsigned long long st = fb[j] & upto;
+                const int first = st ? 63 - __clzll(st) : 0;
+                const bool v = (m & upto & (~0ull << first)) != 0ull;
+                // lane 63 ends its row iff the next symbol starts one (or none follows)
+                const bool ends_here = lane < 63 || p + 1 == a.n_csym || (fnext[j] & 1ull) != 0ull;
+                if (st != 0ull && ends_here) Y[row[j]] = v ? 1 : 0;
+                else if (v) Y[row[j]] = 1;  // a piece: OR into the zeroed row (BooleanOp::emit)
+            }
+        }
+    }
+}
+
+// Gather from LDS: list p holds the symbols whose bits lie in range p
+// ([p * kBitRangeBits, (p + 1) * kBitRangeBits), 64 KiB). Workgroup w takes
+// the w-th equal share of all lists' chunks; for each list its share touches
+// it copies that range of the bitset into LDS, then its waves sweep the chunks
+// (kBitChunks per wave in flight) looking bits up in LDS. A row's symbols sit
+// in several lists, so every row piece that is 1 is ORed into the zeroed y
+// (by storing 1, as BooleanOp::emit does).
+constexpr int kBitRangeBits = 1 << GMR_BIT_RANGE_LOG2;
+constexpr int kBitGatherBlock = GMR_BIT_BLOCK;
+constexpr int kBitWgPerCu = GMR_BIT_WG_PER_CU;
+
+__global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a) {
+    __shared__ unsigned lds[kBitRangeBits / 32];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    int* Y = static_cast<int*>(a.Y);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
+    const long tbits = static_cast<long>(a.tw) << 6;
+    const long N = a.xpoff[a.nbl];
+    const long g0 = N * blockIdx.x / gridDim.x, g1 = N * (blockIdx.x + 1) / gridDim.x;
+    int p = 0;
+    for (long c0 = g0; c0 < g1;) {
+        while (a.xpoff[p + 1] <= c0) ++p;  // ends below nbl, since c0 < N
+        const long c1 = a.xpoff[p + 1] < g1 ? static_cast<long>(a.xpoff[p + 1]) : g1;
+        __syncthreads();  // the previous range's lookups are done
+        const long w0 = static_cast<long>(p) * (kBitRangeBits / 64);
+        for (int i = threadIdx.x; i < kBitRangeBits / 64; i += blockDim.x) {
+            const unsigned long long v = w0 + i < a.nvw ? a.vbits[w0 + i] : 0ull;
+            lds[2 * i] = static_cast<unsigned>(v);
+            lds[2 * i + 1] = static_cast<unsigned>(v >> 32);
+        }
+        __syncthreads();
+        const long base = static_cast<long>(p) * kBitRangeBits;
+        for (long c = c0 + static_cast<long>(wave) * kBitChunks; c < c1; c += static_cast<long>(nwave) * kBitChunks) {
+            int sym[kBitChunks], ord[kBitChunks];
+            unsigned long long fb[kBitChunks];
+#pragma unroll
+            for (int j = 0; j < kBitChunks; ++j) {
+                const bool in = c + j < c1;
+                sym[j] = in ? a.xsym[(c + j) * 64 + lane] : -1;
+                fb[j] = in ? a.xbits[c + j] : 0ull;
+                ord[j] = in ? a.xord[c + j] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < kBitChunks; ++j) {
+                if (c + j >= c1) break;  // wave-uniform
+                const long k = sym[j] >= 0 ? tbits + sym[j] : static_cast<long>(-sym[j] - 1);
+                const unsigned long long local = static_cast<unsigned long long>(k - base);
+                const bool b = local < static_cast<unsigned long long>(kBitRangeBits) &&
+                               ((lds[local >> 5] >> (local & 31)) & 1u) != 0u;
+                const unsigned long long m = __ballot(b);
+                const bool last = lane == 63 || ((fb[j] >> (lane + 1)) & 1ull) != 0ull;
+                if (last) {
+                    const unsigned long long st = fb[j] & upto;
+                    const int first = st ? 63 - __clzll(st) : 0;
+                    if ((m & upto & (~0ull << first)) != 0ull) Y[a.xrow[ord[j] - 1 + __popcll(st)]] = 1;
+                }
+            }
+        }
+        c0 = c1;
+    }
+}
+
+__global__ void k_bits_init(SweepArgs a) { ph_bits_init(a, xcd_tid(), static_cast<long>(gridDim.x) * blockDim.x); }
+
+__global__ void k_bits_eval(SweepArgs a, int k) {
+    ph_bits_eval(a, k, xcd_tid(), static_cast<long>(gridDim.x) * blockDim.x);
+}
+
+__global__ void k_bits_gather(SweepArgs a) {
+    ph_bits_gather(a, xcd_tid(), static_cast<long>(gridDim.x) * blockDim.x);
 }
 
 // ---- persistent variant: the whole sweep in one launch ----

@@ -11,19 +11,21 @@ import hashlib
 import json
 import os
 
-from . import (EMIT_GATHER, EMIT_SPOT, RESIDENT, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, VARIANT_GRAPH, VARIANT_LEVELS,
-               VARIANT_PERSISTENT)
+from . import (BOOLEAN, EMIT_GATHER, EMIT_SPOT, RESIDENT, RUN_BITS, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, VARIANT_GRAPH,
+               VARIANT_LEVELS, VARIANT_PERSISTENT)
 
 VARIANT_NAMES = {VARIANT_LEVELS: "levels", VARIANT_GRAPH: "graph", VARIANT_PERSISTENT: "persistent"}
 EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
 
 
-def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), (1024, 0), (256, 256))):
+def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), (1024, 0), (256, 256)),
+               semiring=None):
     """The search space for one engine and batch width: variant x emission x
     vector tile x launch shape (x gather order for row-gather emission of
     several vectors; x XCD-partitioned (1, 2, 4 or automatic passes),
     single-list or row-segment gather for one vector; x XCD-partitioned or
-    row-segment gather for tiles of 2-4 vectors).
+    row-segment gather for tiles of 2-4 vectors; the Boolean semiring on one
+    vector adds the bit mode to the levels and graph variants).
     Persistent shapes are (block, grid cap); a cap of 0 means the engine's
     default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
@@ -40,6 +42,8 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
                         narrow = e == EMIT_GATHER and 2 <= t <= 4 and layout == RESIDENT
                         if e == EMIT_GATHER and B == 1:
                             flags = (0, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST)
+                            if semiring == BOOLEAN and v != VARIANT_PERSISTENT and layout == RESIDENT:
+                                flags += (RUN_BITS,)
                         elif narrow:
                             flags = (0, RUN_ROW_SEGMENTS) if order == 0 else (RUN_ROW_SEGMENTS,)
                         else:
@@ -72,6 +76,7 @@ def describe(c):
             + (f"/grid<={cap}" if cap else "") + ("/order=rule" if c.get("gather_order") == 1 else "")
             + ("/segments" if c.get("flags", 0) & RUN_ROW_SEGMENTS else "")
             + ("/single-list" if c.get("flags", 0) & RUN_SINGLE_LIST else "")
+            + ("/bits" if c.get("flags", 0) & RUN_BITS else "")
             + (f"/xcd-passes={c['xcd_passes'] or 'auto'}" if "xcd_passes" in c else ""))
 
 

@@ -59,7 +59,11 @@ enum gmr_run_flags {
      * atomically into a zeroed y (in a fused multi-grammar persistent launch the
      * single-vector gather uses the single list below and 2-4 vectors the row segments). */
     GMR_RUN_ROW_SEGMENTS = 2, /* lanes per row segment instead */
-    GMR_RUN_SINGLE_LIST = 4   /* one vector: the whole top sequence as one chunked list instead */
+    GMR_RUN_SINGLE_LIST = 4,  /* one vector: the whole top sequence as one chunked list instead */
+    /* Boolean semiring, one vector, gather emission, levels or graph variant: every terminal
+     * and rule value is one bit of a bitset ((U + rules) / 8 bytes, small enough to stay in
+     * L2), node updates and the gather read bits, and y gets 0/1 as int32. */
+    GMR_RUN_BITS = 8
 };
 
 typedef struct gmr_run_opts {

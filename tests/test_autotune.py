@@ -90,3 +90,14 @@ def test_candidates_for_narrow_tiles_cover_both_gathers():
         assert "levels/gather/tile=B/blk=256/order=rule/segments" in d
     d16 = {autotune.describe(c) for c in autotune.candidates(16, gmr.RESIDENT)}
     assert not any("xcd-passes" in x or "segments" in x for x in d16)
+
+
+def test_bit_mode_candidates_only_for_boolean_single_vectors():
+    """The Boolean bit mode is searched for one vector with gather emission in
+    the levels and graph variants, and nowhere else."""
+    d = {autotune.describe(c) for c in autotune.candidates(1, gmr.RESIDENT, semiring=gmr.BOOLEAN)}
+    assert {"levels/gather/tile=B/blk=256/bits", "graph/gather/tile=B/blk=256/bits"} <= {x for x in d if "bits" in x}
+    assert len([x for x in d if "bits" in x]) == 2
+    for B, sr, lay in ((1, gmr.PLUS_TIMES, gmr.RESIDENT), (1, gmr.TROPICAL, gmr.RESIDENT), (4, gmr.BOOLEAN, gmr.RESIDENT),
+                       (1, gmr.BOOLEAN, gmr.WAVE64), (1, None, gmr.RESIDENT)):
+        assert not any(c.get("flags", 0) & gmr.RUN_BITS for c in autotune.candidates(B, lay, semiring=sr))

@@ -552,3 +552,69 @@ def test_narrow_tiles_on_the_partitioned_gather(hap, sr):
                                flags=flags, xcd_passes=passes)
                 check(sr, Y, Y_star, den if sr == 0 else None,
                       f"narrow sr={sr} B={B} tile={tile} {var} flags={flags} p={passes}")
+
+
+def test_boolean_bit_mode(hap):
+    """Boolean, one vector, values as bits (GMR_RUN_BITS; LDS-range lists,
+    and the flat list with GMR_RUN_SINGLE_LIST): bit-exact against the oracle
+    and the int32 path on the haplotype grammar (as built and with its top
+    rules inlined), ragged rows (empty, exactly 64, crossing several 64-symbol
+    chunks), an R-MAT graph, a weighted R-MAT graph whose bitset spans three
+    LDS ranges, and a grammar without rules; frontiers of 1%, 30% and all
+    ones; levels and graph variants, the graph replayed into the same buffers
+    with a new x in between."""
+    m, g, seq, V = hap
+    rng = np.random.default_rng(4242)
+    rows, cols = 40, 300
+    lens = [0, 1, 64, 63, 65, 0, 0, 200, 1, 129, 2, 0, 300, 5] + list(rng.integers(0, 90, rows - 14))
+    M = np.zeros((rows, cols), np.float64)
+    for r, n in enumerate(lens):
+        M[r, rng.choice(cols, int(n), replace=False)] = 1.0
+    ragged = gmr.Matrix.from_dense(M)
+    rmat = gmr.Matrix.rmat(2027, 13, 60000)
+    # weighted edges: ~1.5M distinct terminals, so the bitset spans three LDS ranges
+    wide = gmr.Matrix.rmat(99, 20, 1_500_000, nweights=100)
+    cases = [("haplotype", m, g), ("haplotype inlined", m, g.inline_top()), ("ragged", ragged, ragged.repair()),
+             ("rmat", rmat, rmat.repair()), ("weighted rmat, three ranges", wide, wide.repair())]
+    for name, mm, gg in cases:
+        sq, VV = mm.arrays()
+        eng = gg.schedule(mm.rows, mm.cols, "resident").engine()
+        for dens in (0.01, 0.3, 1.0):
+            x = (rng.random(mm.cols) < dens).astype(np.float32)
+            y_star, _ = O.brute_force(1, mm.rows, mm.cols, sq, VV, x)
+            for var in ("levels", "graph"):
+                for fl in (gmr.RUN_BITS, gmr.RUN_BITS | gmr.RUN_SINGLE_LIST):
+                    _say("bits", name, dens, var, fl)
+                    y = run_engine(eng, 1, x, variant=gmr.VARIANTS[var], emit=gmr.EMIT_GATHER, flags=fl)
+                    check(1, y, y_star, None, f"bits {name} {dens} {var} flags={fl}")
+            y_int = run_engine(eng, 1, x, variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER)
+            check(1, y_int, y_star, None, f"int32 path {name} {dens}")
+        # graph replays into the same buffers, a new x written in between
+        xs = [(rng.random(mm.cols) < 0.05).astype(np.float32) for _ in range(3)]
+        Xt = torch.zeros(mm.cols, dtype=torch.float32, device="cuda")
+        Yt = torch.full((mm.rows,), 7, dtype=torch.int32, device="cuda")
+        for rep, x in enumerate(xs + xs[:1]):
+            _say("bits graph replay", name, rep)
+            Xt.copy_(torch.from_numpy(x))
+            eng.run_torch(1, Xt, Yt, variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS)
+            torch.cuda.synchronize()
+            y_star, _ = O.brute_force(1, mm.rows, mm.cols, sq, VV, x)
+            check(1, Yt.cpu().numpy(), y_star, None, f"bits replay {name} {rep}")
+    # a grammar without rules: terminals only, empty rows
+    t = gmr.Grammar.from_arrays(4, [], [], [1, 2, 0, 0, 3, 0], [1.0])
+    eng = t.schedule(3, 3, "resident").engine()
+    for x, want in (([1.0, 0.0, 0.0], [1, 0, 0]), ([0.0, 0.0, 5.0], [0, 0, 1]), ([0.0, 2.0, 0.0], [1, 0, 0])):
+        y = run_engine(eng, 1, np.array(x, np.float32), emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS)
+        np.testing.assert_array_equal(y, want)
+    # what the mode does not run is refused before any launch
+    eng = g.schedule(4096, 2048, "resident").engine()
+    X1 = torch.zeros(2048, dtype=torch.float32, device="cuda")
+    X4 = torch.zeros((2048, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(gmr.GmrError, match="Boolean"):
+        eng.run_torch(0, X1, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS)
+    with pytest.raises(gmr.GmrError, match="one vector"):
+        eng.run_torch(1, X4, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS)
+    with pytest.raises(gmr.GmrError, match="one vector"):
+        eng.run_torch(1, X1, emit=gmr.EMIT_SPOT, flags=gmr.RUN_BITS)
+    with pytest.raises(gmr.GmrError, match="one vector"):
+        eng.run_torch(1, X1, variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS)
