@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE CSR baseline and the energy reads")
     ap.add_argument("--no-tune", action="store_true", help="use the default schedule instead of autotuning")
+    ap.add_argument("--semiring", default="all", choices=["all", "boolean", "minplus"],
+                    help="rmat workload: which of its two products to run")
     ap.add_argument("--variant", default=None, help="force a variant (levels|graph|persistent)")
     ap.add_argument("--emit", default=None, help="force an emission strategy (spot|gather)")
     ap.add_argument("--b-tile", type=int, default=None)
@@ -680,6 +682,8 @@ def run_rmat(args):
     scale, edges = 23, 1 << 27
     res = {}
     for tag, nw, sr in (("boolean", 1, gmr.BOOLEAN), ("minplus", 100, gmr.TROPICAL)):
+        if args.semiring not in ("all", tag):
+            continue
         m = gmr.Matrix.rmat(20261206 + nw, scale, edges, nweights=nw)
         nnz, n = m.nnz(), m.rows
         g = m.repair()
@@ -698,8 +702,31 @@ def run_rmat(args):
             x[rng.choice(n, 1024, replace=False)] = 0.0
         xt = torch.from_numpy(x).cuda()
         y = torch.empty(n, dtype=torch.int32, device="cuda")
-        form, best, tuned = tune_forms(forms, xt, y, autotune.candidates(1, gmr.RESIDENT, semiring=sr), reps=2, semiring=sr)
+        if args.no_tune:
+            # Boolean: values as bits (the LDS-range gather); min/plus: the XCD-partitioned gather
+            form, tuned = untuned_form(args, forms, 1), []
+            best = dict(variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER, b_tile=0, block=256, grid_cap=0,
+                        gather_order=0)
+            if sr == gmr.BOOLEAN:
+                best["flags"] = gmr.RUN_BITS
+            else:
+                best["xcd_passes"] = 0
+        else:
+            form, best, tuned = tune_forms(forms, xt, y, autotune.candidates(1, gmr.RESIDENT, semiring=sr), reps=2,
+                                           semiring=sr)
         ms = _time_steps(form["engine"], sr, xt, y, best, args.steps, args.warmup)
+        # phases of the chosen schedule (levels variant, HIP events around every launch)
+        # and the row gather's rate on its algorithmic bytes: 4 per top-sequence symbol
+        # read, 4 per y row written
+        kw = autotune.apply(form["engine"], best)
+        kw.pop("variant", None)
+        ph = form["engine"].time_levels(sr, xt, y, reps=5, **kw)
+        finfo = form["engine"].info()
+        gbytes = 4 * (finfo["emit_total"] + finfo["num_term_emit"]) + 4 * n
+        phases = {"init_ms": round(float(ph[0]), 4), "levels_ms": round(float(ph[1:-1].sum()), 4),
+                  "gather_ms": round(float(ph[-1]), 4), "gather_bytes": int(gbytes),
+                  "gather_GBps": round(gbytes / (float(ph[-1]) * 1e-3) / 1e9, 1)}
+        log(tag, "phases", phases)
         seq, V = m.arrays()
         y_star, _ = O.brute_force(sr, n, n, seq, V, x)
         exact = bool(np.array_equal(y.cpu().numpy().astype(np.int64), y_star.astype(np.int64)))
@@ -741,7 +768,7 @@ def run_rmat(args):
             log(tag, "traversal", trav)
         info = s_res.info()
         res[tag] = {"nnz": nnz, "ms": ms, "nnz_per_s": nnz / (ms * 1e-3), "schedule": autotune.describe(best),
-                    "grammar_form": form["label"],
+                    "grammar_form": form["label"], "phases": phases,
                     "traversal": trav,
                     "bit_exact": exact, "plain_ms": plain_ms, "plain_nnz_per_s": nnz / (plain_ms * 1e-3),
                     "plain_bit_exact": exact_plain,

@@ -388,7 +388,8 @@ const Engine::XParts& Engine::xparts_for(int passes) {
 // Lists of the Boolean bit-mode gather: the top-sequence symbols regrouped by
 // bit index (terminal q -> q, rule slot s -> 64 * ceil(U / 64) + s) into ranges
 // of dev::kBitRangeBits bits, one range per list, so a workgroup can hold its
-// list's bits in LDS. Padding reads terminal 0 (always 0).
+// list's bits in LDS. Each entry is stored as its bit's index inside the
+// range; padding is -1 (past every range: reads 0).
 const Engine::XParts& Engine::bit_lists() {
     if (bparts.d_sym) return bparts;
     if (!resident) throw std::invalid_argument("the bit-mode gather needs the Resident layout");
@@ -405,6 +406,13 @@ const Engine::XParts& Engine::bit_lists() {
     const int P = static_cast<int>(std::max(1L, (nkey + span - 1) / span));
     PartLists L;
     build_part_lists(0, rows, crow, csym, key, nkey, P, -1, L, span);
+    // Entries become bit indices inside their list's range; padding (and the
+    // zero terminal, whose bit is 0) becomes an index past every range.
+    for (int p = 0; p < P; ++p)
+        for (long q = static_cast<long>(L.poff[p]) * 64; q < static_cast<long>(L.poff[p + 1]) * 64; ++q) {
+            const int c = L.sym[q];
+            L.sym[q] = c == -1 ? -1 : static_cast<int>(key(c) - static_cast<long>(p) * span);
+        }
     bparts.poff = L.poff;
     bparts.d_sym = dev_upload(L.sym.data(), L.sym.size(), &struct_bytes);
     bparts.d_bits = dev_upload(L.bits.data(), L.bits.size(), &struct_bytes);

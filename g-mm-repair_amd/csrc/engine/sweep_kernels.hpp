@@ -1060,6 +1060,9 @@ __device__ void ph_bits_eval(const SweepArgs& a, int k, long tid, long nth) {
 #ifndef GMR_BIT_WG_PER_CU
 #define GMR_BIT_WG_PER_CU 2
 #endif
+#ifndef GMR_BIT_NT
+#define GMR_BIT_NT 1
+#endif
 constexpr int kBitChunks = GMR_BIT_CHUNKS;
 
 __device__ void ph_bits_gather(const SweepArgs& a, long tid, long nth) {
@@ -1121,10 +1124,13 @@ constexpr int kBitWgPerCu = GMR_BIT_WG_PER_CU;
 
 __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a) {
     __shared__ unsigned lds[kBitRangeBits / 32];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63;
+    // wave-uniform, so the per-chunk words (row starts, row order) are scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int nwave = blockDim.x >> 6;
     int* Y = static_cast<int*>(a.Y);
+    const unsigned* loc = reinterpret_cast<const unsigned*>(a.xsym);  // bit index inside the list's range
     const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
-    const long tbits = static_cast<long>(a.tw) << 6;
     const long N = a.xpoff[a.nbl];
     const long g0 = N * blockIdx.x / gridDim.x, g1 = N * (blockIdx.x + 1) / gridDim.x;
     int p = 0;
@@ -1139,32 +1145,45 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
             lds[2 * i + 1] = static_cast<unsigned>(v >> 32);
         }
         __syncthreads();
-        const long base = static_cast<long>(p) * kBitRangeBits;
         for (long c = c0 + static_cast<long>(wave) * kBitChunks; c < c1; c += static_cast<long>(nwave) * kBitChunks) {
-            int sym[kBitChunks], ord[kBitChunks];
+            // Every load is unconditional (chunk indices clamped into the
+            // segment, row indices to a valid entry), so the loads of all
+            // chunks are in flight together: a conditional load would end in
+            // a wait for everything outstanding.
+            unsigned l[kBitChunks];
             unsigned long long fb[kBitChunks];
+            int ord[kBitChunks];
 #pragma unroll
             for (int j = 0; j < kBitChunks; ++j) {
-                const bool in = c + j < c1;
-                sym[j] = in ? a.xsym[(c + j) * 64 + lane] : -1;
-                fb[j] = in ? a.xbits[c + j] : 0ull;
-                ord[j] = in ? a.xord[c + j] : 0;
+                const long cj = c + j < c1 ? c + j : c1 - 1;
+#if GMR_BIT_NT
+                l[j] = __builtin_nontemporal_load(loc + cj * 64 + lane);
+#else
+                l[j] = loc[cj * 64 + lane];
+#endif
+                fb[j] = a.xbits[cj];
+                ord[j] = a.xord[cj];
             }
+            bool hit[kBitChunks];
+            int ri[kBitChunks];
 #pragma unroll
             for (int j = 0; j < kBitChunks; ++j) {
-                if (c + j >= c1) break;  // wave-uniform
-                const long k = sym[j] >= 0 ? tbits + sym[j] : static_cast<long>(-sym[j] - 1);
-                const unsigned long long local = static_cast<unsigned long long>(k - base);
-                const bool b = local < static_cast<unsigned long long>(kBitRangeBits) &&
-                               ((lds[local >> 5] >> (local & 31)) & 1u) != 0u;
+                const bool in_range = l[j] < static_cast<unsigned>(kBitRangeBits);
+                const unsigned li = in_range ? l[j] : 0u;
+                const bool b = (((lds[li >> 5] >> (li & 31)) & 1u) & (in_range ? 1u : 0u)) != 0u;
                 const unsigned long long m = __ballot(b);
-                const bool last = lane == 63 || ((fb[j] >> (lane + 1)) & 1ull) != 0ull;
-                if (last) {
-                    const unsigned long long st = fb[j] & upto;
-                    const int first = st ? 63 - __clzll(st) : 0;
-                    if ((m & upto & (~0ull << first)) != 0ull) Y[a.xrow[ord[j] - 1 + __popcll(st)]] = 1;
-                }
+                const unsigned long long ends = (fb[j] >> 1) | (1ull << 63);  // lane l ends a row piece
+                const unsigned long long st = fb[j] & upto;
+                const int first = st ? 63 - __clzll(st) : 0;  // the piece's first lane
+                hit[j] = c + j < c1 && ((ends >> lane) & 1ull) != 0ull && ((m & upto) >> first) != 0ull;
+                ri[j] = hit[j] ? ord[j] - 1 + __popcll(st) : 0;
             }
+            int row[kBitChunks];
+#pragma unroll
+            for (int j = 0; j < kBitChunks; ++j) row[j] = a.xrow[ri[j]];
+#pragma unroll
+            for (int j = 0; j < kBitChunks; ++j)
+                if (hit[j]) Y[row[j]] = 1;
         }
         c0 = c1;
     }
