@@ -1145,25 +1145,32 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
             lds[2 * i + 1] = static_cast<unsigned>(v >> 32);
         }
         __syncthreads();
-        for (long c = c0 + static_cast<long>(wave) * kBitChunks; c < c1; c += static_cast<long>(nwave) * kBitChunks) {
-            // Every load is unconditional (chunk indices clamped into the
-            // segment, row indices to a valid entry), so the loads of all
-            // chunks are in flight together: a conditional load would end in
-            // a wait for everything outstanding.
-            unsigned l[kBitChunks];
-            unsigned long long fb[kBitChunks];
-            int ord[kBitChunks];
+        // Every load is unconditional (chunk indices clamped into the segment,
+        // row indices to a valid entry), so a stage's loads are in flight
+        // together: a conditional load would end in a wait for everything
+        // outstanding. Software-pipelined: an iteration issues its row-id loads,
+        // then the next iteration's symbol and chunk-word loads, so the two
+        // round trips overlap.
+        const long step = static_cast<long>(nwave) * kBitChunks;
+        long c = c0 + static_cast<long>(wave) * kBitChunks;
+        unsigned l[kBitChunks];
+        unsigned long long fb[kBitChunks];
+        int ord[kBitChunks];
+        auto load_chunks = [&](long cc, unsigned* lo, unsigned long long* fo, int* oo) {
 #pragma unroll
             for (int j = 0; j < kBitChunks; ++j) {
-                const long cj = c + j < c1 ? c + j : c1 - 1;
+                const long cj = cc + j < c1 ? cc + j : c1 - 1;
 #if GMR_BIT_NT
-                l[j] = __builtin_nontemporal_load(loc + cj * 64 + lane);
+                lo[j] = __builtin_nontemporal_load(loc + cj * 64 + lane);
 #else
-                l[j] = loc[cj * 64 + lane];
+                lo[j] = loc[cj * 64 + lane];
 #endif
-                fb[j] = a.xbits[cj];
-                ord[j] = a.xord[cj];
+                fo[j] = a.xbits[cj];
+                oo[j] = a.xord[cj];
             }
+        };
+        load_chunks(c, l, fb, ord);  // (past the segment: its last chunk again, unused)
+        while (c < c1) {
             bool hit[kBitChunks];
             int ri[kBitChunks];
 #pragma unroll
@@ -1181,9 +1188,17 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
             int row[kBitChunks];
 #pragma unroll
             for (int j = 0; j < kBitChunks; ++j) row[j] = a.xrow[ri[j]];
+            const long cn = c + step;
+            load_chunks(cn, l, fb, ord);  // unconditional: a branch here would make the stores wait for it
+            // The row ids are used here, after the prefetch, so the compiler cannot
+            // sink their loads into the stores' branches (where the wait for them
+            // would also wait for the prefetch). An empty statement: no code.
+#pragma unroll
+            for (int j = 0; j < kBitChunks; ++j) asm volatile("" : : "v"(row[j]));
 #pragma unroll
             for (int j = 0; j < kBitChunks; ++j)
                 if (hit[j]) Y[row[j]] = 1;
+            c = cn;
         }
         c0 = c1;
     }
