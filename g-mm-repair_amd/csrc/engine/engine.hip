@@ -79,12 +79,13 @@ Engine::~Engine() {
                     static_cast<void*>(d_lrow), ws_trav, static_cast<void*>(d_lxsym), static_cast<void*>(d_lxbits),
                     static_cast<void*>(d_lxord), static_cast<void*>(d_lxrow), static_cast<void*>(d_lxpoff)})
         if (p) (void)hipFree(p);
-    xparts.emplace(-1, bparts);  // the bit-mode lists are freed with the others
-    for (auto& kv : xparts)
-        for (void* p : {static_cast<void*>(kv.second.d_sym), static_cast<void*>(kv.second.d_bits),
-                        static_cast<void*>(kv.second.d_ord), static_cast<void*>(kv.second.d_row),
-                        static_cast<void*>(kv.second.d_poff)})
+    auto free_lists = [](const XParts& x) {
+        for (void* p : {static_cast<void*>(x.d_sym), static_cast<void*>(x.d_bits), static_cast<void*>(x.d_ord),
+                        static_cast<void*>(x.d_row), static_cast<void*>(x.d_poff)})
             if (p) (void)hipFree(p);
+    };
+    for (auto& kv : xparts) free_lists(kv.second);
+    free_lists(bparts);
     void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_zrows, d_fbits, d_ford,
                     d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds,
                     static_cast<void*>(ws_bits)};
