@@ -42,3 +42,31 @@ def test_struct_layouts_match_the_reference():
     # reference (two ints, two pointers, three pointers + int, two pointers + int).
     assert ctypes.sizeof(_lib.LayeredRule) == 12
     assert ctypes.sizeof(_lib.GPUSchedule) == 8 + 2 * 8 + 3 * 8 + 8 + 2 * 8 + 8
+
+
+def test_python_constants_match_the_header_enums():
+    """Every enumerator of the C headers has the value the Python bindings pass
+    (GMR_PLUS_TIMES -> gmr.PLUS_TIMES, GMR_LAYOUT_RESIDENT -> gmr.RESIDENT,
+    GMR_RUN_BITS -> gmr.RUN_BITS, GMR_CSR_ALG_ROW -> vendor.ALGS['row'], ...)."""
+    import gmr
+    from gmr import vendor
+    enum = re.compile(r"\b(GMR_[A-Z0-9_]+)\s*=\s*(\d+)")
+    values = {}
+    for h in ("gmr.h", "gmr_host.h", "gmr_vendor.h"):
+        with open(os.path.join(INC, h)) as f:
+            values.update({k: int(v) for k, v in enum.findall(f.read())})
+    checked = 0
+    for name, v in values.items():
+        short = name[len("GMR_"):]
+        if short.startswith("LAYOUT_"):
+            short = short[len("LAYOUT_"):]
+        if short.startswith("CSR_ALG_"):
+            assert vendor.ALGS[short[len("CSR_ALG_"):].lower()] == v, name
+            checked += 1
+        elif hasattr(gmr, short):
+            assert getattr(gmr, short) == v, name
+            checked += 1
+    for must in ("GMR_RUN_BITS", "GMR_RUN_SINGLE_LIST", "GMR_VARIANT_PERSISTENT", "GMR_EMIT_GATHER", "GMR_TROPICAL",
+                 "GMR_EVAL_LDS", "GMR_LAYOUT_RESIDENT"):
+        assert must in values, must
+    assert checked >= 20, checked
