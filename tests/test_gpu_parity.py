@@ -606,8 +606,17 @@ def test_boolean_bit_mode(hap):
     for x, want in (([1.0, 0.0, 0.0], [1, 0, 0]), ([0.0, 0.0, 5.0], [0, 0, 1]), ([0.0, 2.0, 0.0], [1, 0, 0])):
         y = run_engine(eng, 1, np.array(x, np.float32), emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS)
         np.testing.assert_array_equal(y, want)
-    # what the mode does not run is refused before any launch
+    # per-level timing runs the same launches (the bench's phase breakdown)
     eng = g.schedule(4096, 2048, "resident").engine()
+    x = (rng.random(2048) < 0.05).astype(np.float32)
+    Xt = torch.from_numpy(x).cuda()
+    Yt = torch.zeros(4096, dtype=torch.int32, device="cuda")
+    t = eng.time_levels(1, Xt, Yt, reps=2, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS)
+    assert len(t) == eng.info()["max_depth"] + 2 and t[0] > 0 and t[-1] > 0
+    torch.cuda.synchronize()
+    y_star, _ = O.brute_force(1, 4096, 2048, seq, V, x)
+    check(1, Yt.cpu().numpy(), y_star, None, "bits time_levels")
+    # what the mode does not run is refused before any launch
     X1 = torch.zeros(2048, dtype=torch.float32, device="cuda")
     X4 = torch.zeros((2048, 4), dtype=torch.float32, device="cuda")
     with pytest.raises(gmr.GmrError, match="Boolean"):
@@ -618,3 +627,12 @@ def test_boolean_bit_mode(hap):
         eng.run_torch(1, X1, emit=gmr.EMIT_SPOT, flags=gmr.RUN_BITS)
     with pytest.raises(gmr.GmrError, match="one vector"):
         eng.run_torch(1, X1, variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS)
+    batch = gmr.Batch([eng, g.schedule(4096, 2048, "resident").engine()])
+    with pytest.raises(gmr.GmrError, match="levels variant"):
+        batch.run_torch(1, [X1[:, None], X1[:, None]], variant=gmr.VARIANT_GRAPH,
+                        opts=[{"emit": gmr.EMIT_GATHER, "flags": gmr.RUN_BITS}] * 2)
+    Ys = batch.run_torch(1, [Xt[:, None], Xt[:, None]], variant=gmr.VARIANT_LEVELS,
+                         opts=[{"emit": gmr.EMIT_GATHER, "flags": gmr.RUN_BITS}] * 2)
+    torch.cuda.synchronize()
+    for Yb in Ys:
+        check(1, Yb.cpu().numpy()[:, 0], y_star, None, "bits in a levels batch")
