@@ -727,6 +727,13 @@ def run_rmat(args):
                   "gather_ms": round(float(ph[-1]), 4), "gather_bytes": int(gbytes),
                   "gather_GBps": round(gbytes / (float(ph[-1]) * 1e-3) / 1e9, 1)}
         log(tag, "phases", phases)
+        # the gather kernel against HBM peak; traffic as counted by rocprofv3 for the
+        # Boolean bit-mode gather on the grammar as built (profiles/traffic.txt)
+        bits = bool(best.get("flags", 0) & gmr.RUN_BITS) and not (best.get("flags", 0) & gmr.RUN_SINGLE_LIST)
+        kernel = "k_bits_gather_lds" if bits else "k_gather"
+        roof = {"kernel": kernel, "bound": "hbm", "achieved": phases["gather_GBps"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(phases["gather_GBps"] / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(f"rmat{scale}_{tag}", kernel) if form["label"] == "as built" else None}
         seq, V = m.arrays()
         y_star, _ = O.brute_force(sr, n, n, seq, V, x)
         exact = bool(np.array_equal(y.cpu().numpy().astype(np.int64), y_star.astype(np.int64)))
@@ -768,7 +775,7 @@ def run_rmat(args):
             log(tag, "traversal", trav)
         info = s_res.info()
         res[tag] = {"nnz": nnz, "ms": ms, "nnz_per_s": nnz / (ms * 1e-3), "schedule": autotune.describe(best),
-                    "grammar_form": form["label"], "phases": phases,
+                    "grammar_form": form["label"], "phases": phases, "roofline": roof,
                     "traversal": trav,
                     "bit_exact": exact, "plain_ms": plain_ms, "plain_nnz_per_s": nnz / (plain_ms * 1e-3),
                     "plain_bit_exact": exact_plain,
