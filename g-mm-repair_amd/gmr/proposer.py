@@ -11,7 +11,7 @@ minimises ceil(B/Bt) * (predicted plan time, corrected by the chosen kernels'
 predicted gain per level). The network is trained on the GPU by the engine's own
 kernels (gmr_mlp_*) on timings the sweep itself produces. Exhaustive timing of
 every (plan, level, kernel, tile) stays available as the fallback and as the
-judge of the proposer's regret. Every choice computes the same Y.
+yardstick for the proposer's regret. Every choice computes the same Y.
 """
 import ctypes as C
 import math
