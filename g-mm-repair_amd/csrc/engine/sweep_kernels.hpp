@@ -1125,7 +1125,7 @@ constexpr int kBitWgPerCu = GMR_BIT_WG_PER_CU;
 __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a) {
     __shared__ unsigned lds[kBitRangeBits / 32];
     const int lane = threadIdx.x & 63;
-    // wave-uniform, so the per-chunk words (row starts, row order) are scalar loads
+    // wave-uniform, so a chunk's words (row starts, row order) are read at one uniform address
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int nwave = blockDim.x >> 6;
     int* Y = static_cast<int*>(a.Y);
