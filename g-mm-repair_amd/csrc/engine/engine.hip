@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -84,16 +85,67 @@ Engine::~Engine() {
                         static_cast<void*>(x.d_row), static_cast<void*>(x.d_poff)})
             if (p) (void)hipFree(p);
     };
-    for (auto& kv : xparts) free_lists(kv.second);
-    free_lists(bparts);
+    for (auto* m : {&xparts, &bparts, &lparts})
+        for (auto& kv : *m) free_lists(kv.second);
     void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_zrows, d_fbits, d_ford,
-                    d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_barrier, d_kinds,
+                    d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_kinds,
                     static_cast<void*>(ws_bits)};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    barrier.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
+}
+
+void BarrierState::init(int khz) {
+    wall_khz = khz > 0 ? khz : 100000;
+    GMR_HIP(hipMalloc(&d, sizeof(dev::GridBarrier)));
+    GMR_HIP(hipMemset(d, 0, sizeof(dev::GridBarrier)));
+    GMR_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_fail), sizeof(int), hipHostMallocMapped | hipHostMallocCoherent));
+    *h_fail = 0;
+    GMR_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_fail), h_fail, 0));
+    GMR_HIP(hipDeviceSynchronize());
+    base = 0;
+}
+
+void BarrierState::release() noexcept {
+    if (d) (void)hipFree(d);
+    if (h_fail) (void)hipHostFree(h_fail);
+    d = nullptr;
+    h_fail = d_fail = nullptr;
+}
+
+dev::BarrierRef BarrierState::ref() const {
+    dev::BarrierRef r;
+    r.gb = d;
+    r.fail_host = d_fail;
+    r.base = base;
+    const unsigned long long us = timeout_us > 0 ? static_cast<unsigned long long>(timeout_us) : 2000000ULL;
+    r.deadline = static_cast<unsigned long long>(wall_khz) * us / 1000ULL;  // ticks
+    return r;
+}
+
+bool BarrierState::take_failure() {
+    if (!h_fail) return false;
+    if (__atomic_load_n(h_fail, __ATOMIC_ACQUIRE) == 0) return false;
+    __atomic_store_n(h_fail, 0, __ATOMIC_RELEASE);
+    return true;
+}
+
+int BarrierState::check() {
+    dev::GridBarrier h{};
+    GMR_HIP(hipDeviceSynchronize());
+    GMR_HIP(hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost));
+    const bool failed = take_failure();
+    if (failed || h.arrived != base) {
+        // Restart the barrier from a clean state.
+        GMR_HIP(hipMemset(d, 0, sizeof(dev::GridBarrier)));
+        GMR_HIP(hipDeviceSynchronize());
+        base = 0;
+        return 1;
+    }
+    return 0;
 }
 
 void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRule* rules,
@@ -272,10 +324,8 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
     GMR_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
     GMR_HIP(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev));
     if (wall_khz <= 0) wall_khz = 100000;
-    GMR_HIP(hipMalloc(&d_barrier, sizeof(dev::GridBarrier)));
-    GMR_HIP(hipMemset(d_barrier, 0, sizeof(dev::GridBarrier)));
-    GMR_HIP(hipDeviceSynchronize());
-    barrier_base = 0;
+    GMR_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    barrier.init(wall_khz);
 }
 
 // Passes that keep each XCD's share of one value range near 3 MB (a 4 MB L2
@@ -386,55 +436,61 @@ const Engine::XParts& Engine::xparts_for(int passes) {
     return xparts.emplace(passes, std::move(x)).first->second;
 }
 
-// Lists of the Boolean bit-mode gather: the top-sequence symbols regrouped by
-// bit index (terminal q -> q, rule slot s -> 64 * ceil(U / 64) + s) into ranges
-// of dev::kBitRangeBits bits, one range per list, so a workgroup can hold its
-// list's bits in LDS. Each entry is stored as its bit's index inside the
-// range; padding is -1 (past every range: reads 0).
-const Engine::XParts& Engine::bit_lists() {
-    if (bparts.d_sym) return bparts;
-    if (!resident) throw std::invalid_argument("the bit-mode gather needs the Resident layout");
-    ensure_bits();
-    const long nsym = crow_host[0];
-    std::vector<int> crow(static_cast<std::size_t>(rows) + 1), csym(static_cast<std::size_t>(nsym));
+namespace {
+
+// Lists by fixed key ranges of `span` keys (one list per range, each entry
+// stored as its key's index inside the range; padding -1), uploaded.
+Engine::XParts range_lists(Engine& en, long nkey, long span, const std::function<long(int)>& key) {
+    const long nsym = en.crow_host[0];
+    std::vector<int> crow(static_cast<std::size_t>(en.rows) + 1), csym(static_cast<std::size_t>(nsym));
     GMR_HIP(hipDeviceSynchronize());
-    GMR_HIP(hipMemcpy(crow.data(), d_crow, sizeof(int) * crow.size(), hipMemcpyDeviceToHost));
-    if (nsym) GMR_HIP(hipMemcpy(csym.data(), d_csym, sizeof(int) * nsym, hipMemcpyDeviceToHost));
-    const long tbits = static_cast<long>(bits_tw) * 64;
-    auto key = [&](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : tbits + c; };
-    const long nkey = tbits + NL;
-    const long span = dev::kBitRangeBits;
+    GMR_HIP(hipMemcpy(crow.data(), en.d_crow, sizeof(int) * crow.size(), hipMemcpyDeviceToHost));
+    if (nsym) GMR_HIP(hipMemcpy(csym.data(), en.d_csym, sizeof(int) * nsym, hipMemcpyDeviceToHost));
     const int P = static_cast<int>(std::max(1L, (nkey + span - 1) / span));
     PartLists L;
-    build_part_lists(0, rows, crow, csym, key, nkey, P, -1, L, span);
-    // Entries become bit indices inside their list's range; padding (and the
-    // zero terminal, whose bit is 0) becomes an index past every range.
+    build_part_lists(0, en.rows, crow, csym, key, nkey, P, -1, L, span);
+    // Entries become indices inside their list's range; padding (-1) becomes an
+    // index past every range (it reads the semiring zero).
     for (int p = 0; p < P; ++p)
         for (long q = static_cast<long>(L.poff[p]) * 64; q < static_cast<long>(L.poff[p + 1]) * 64; ++q) {
             const int c = L.sym[q];
             L.sym[q] = c == -1 ? -1 : static_cast<int>(key(c) - static_cast<long>(p) * span);
         }
-    bparts.poff = L.poff;
-    bparts.d_sym = dev_upload(L.sym.data(), L.sym.size(), &struct_bytes);
-    bparts.d_bits = dev_upload(L.bits.data(), L.bits.size(), &struct_bytes);
-    bparts.d_ord = dev_upload(L.ord.data(), L.ord.size(), &struct_bytes);
-    bparts.d_row = dev_upload(L.row.data(), L.row.size(), &struct_bytes);
-    bparts.d_poff = dev_upload(bparts.poff.data(), bparts.poff.size(), &struct_bytes);
-    return bparts;
+    Engine::XParts x;
+    x.poff = L.poff;
+    x.d_sym = dev_upload(L.sym.data(), L.sym.size(), &en.struct_bytes);
+    x.d_bits = dev_upload(L.bits.data(), L.bits.size(), &en.struct_bytes);
+    x.d_ord = dev_upload(L.ord.data(), L.ord.size(), &en.struct_bytes);
+    x.d_row = dev_upload(L.row.data(), L.row.size(), &en.struct_bytes);
+    x.d_poff = dev_upload(x.poff.data(), x.poff.size(), &en.struct_bytes);
+    return x;
 }
 
-int Engine::check_barrier() {
-    dev::GridBarrier h{};
-    GMR_HIP(hipDeviceSynchronize());
-    GMR_HIP(hipMemcpy(&h, d_barrier, sizeof(h), hipMemcpyDeviceToHost));
-    if (h.error || h.arrived != barrier_base) {
-        // Restart the barrier from a clean state.
-        GMR_HIP(hipMemset(d_barrier, 0, sizeof(dev::GridBarrier)));
-        GMR_HIP(hipDeviceSynchronize());
-        barrier_base = 0;
-        return 1;
-    }
-    return 0;
+}  // namespace
+
+// Lists of the Boolean bit-mode gather: the top-sequence symbols regrouped by
+// bit index (terminal q -> q, rule slot s -> 64 * ceil(U / 64) + s) into ranges
+// of `range_bits` bits, one range per list, so a workgroup can hold its list's
+// bits in LDS. The zero terminal's bit is 0, so it reads 0 like the padding.
+const Engine::XParts& Engine::bit_lists(int range_bits) {
+    auto it = bparts.find(range_bits);
+    if (it != bparts.end()) return it->second;
+    if (!resident) throw std::invalid_argument("the bit-mode gather needs the Resident layout");
+    ensure_bits();
+    const long tbits = static_cast<long>(bits_tw) * 64;
+    auto key = [tbits](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : tbits + c; };
+    return bparts.emplace(range_bits, range_lists(*this, tbits + NL, range_bits, key)).first->second;
+}
+
+// Lists of the LDS-range gather of one vector: value keys (terminal t -> t,
+// rule slot s -> U + s) in ranges of `span` values.
+const Engine::XParts& Engine::lds_lists(int span) {
+    auto it = lparts.find(span);
+    if (it != lparts.end()) return it->second;
+    if (!resident) throw std::invalid_argument("the LDS-range gather needs the Resident layout");
+    const long u = U;
+    auto key = [u](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : u + c; };
+    return lparts.emplace(span, range_lists(*this, u + NL, span, key)).first->second;
 }
 
 void Engine::ensure_workspace(int Bt, int vbytes) {
@@ -517,6 +573,14 @@ inline unsigned eval_grid(const SweepArgs& a, int kind, long w, int block) {
 
 inline int host_kind(const gmr_run_opts& o, int k) { return o.level_kind ? o.level_kind[k] : o.eval_kind; }
 
+// Workgroups of an LDS-range gather (1,024 lanes, `lds` bytes of LDS each):
+// two per CU when a range takes at most 64 KiB, else one, and no more than
+// one per 64 chunks.
+inline unsigned lds_gather_grid(const Engine& en, long nch, std::size_t lds) {
+    const long per_cu = lds <= 65536 ? 2 : 1;
+    return static_cast<unsigned>(std::max(1L, std::min<long>(per_cu * en.num_cus, (nch + 63) / 64)));
+}
+
 template <class S>
 void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, int block, hipStream_t st,
                    hipEvent_t* level_ev = nullptr) {
@@ -535,11 +599,17 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         }
         if (!a.flat) {
             // one LDS range per list; workgroups take equal shares of all chunks
-            const long nch = en.bparts.poff.back();
-            const long g = std::max(1L, std::min<long>(static_cast<long>(dev::kBitWgPerCu) * en.num_cus, (nch + 63) / 64));
-            if (nch > 0)
-                hipLaunchKernelGGL(dev::k_bits_gather_lds, dim3(static_cast<unsigned>(g)), dim3(dev::kBitGatherBlock), 0,
-                                   st, a);
+            const long nch = en.bparts.at(a.bit_range).poff.back();
+            const std::size_t lds = static_cast<std::size_t>(a.bit_range) / 8;
+            const unsigned g = lds_gather_grid(en, nch, lds);
+            if (nch > 0) {
+                if (a.chunks == 2)
+                    hipLaunchKernelGGL(dev::k_bits_gather_lds<2>, dim3(g), dim3(dev::kBitGatherBlock), lds, st, a);
+                else if (a.chunks == 8)
+                    hipLaunchKernelGGL(dev::k_bits_gather_lds<8>, dim3(g), dim3(dev::kBitGatherBlock), lds, st, a);
+                else
+                    hipLaunchKernelGGL(dev::k_bits_gather_lds<4>, dim3(g), dim3(dev::kBitGatherBlock), lds, st, a);
+            }
         } else if (en.n_fchunk > 0) {
             hipLaunchKernelGGL(dev::k_bits_gather,
                                dim3(grid_for((static_cast<long>(en.n_fchunk) + dev::kBitChunks - 1) / dev::kBitChunks * 64,
@@ -560,8 +630,11 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         const int w = en.level_off[k] - en.level_off[k - 1];
         const int kind = host_kind(o, k);
         if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * k - 1], st));
-        if (w > 0)
-            hipLaunchKernelGGL(dev::k_eval<S>, dim3(eval_grid(a, kind, w, block)), dim3(block), 0, st, a, k, kind);
+        if (w > 0) {
+            const bool lds = kind == dev::EV_LDS && a.Bt == 1;
+            hipLaunchKernelGGL(dev::k_eval<S>, dim3(eval_grid(a, kind, w, block)), dim3(block),
+                               lds ? static_cast<std::size_t>(a.lds_window) * sizeof(typename S::T) : 0, st, a, k, kind);
+        }
         if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * k], st));
         const int ne = en.emit_off[k] - en.emit_off[k - 1];
         if (!a.gather && ne > 0) {
@@ -580,10 +653,23 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
             ap.xpass1 = pass + 1;
             long most = 0;
             for (int q = 0; q < 8; ++q) most = std::max<long>(most, poff[8 * pass + q + 1] - poff[8 * pass + q]);
-            if (most > 0)  // kPartChunks chunks per wave
-                hipLaunchKernelGGL(dev::k_gather<S>,
-                                   dim3(grid_for((most + dev::kPartChunks - 1) / dev::kPartChunks * 8 * 64, block)),
-                                   dim3(block), 0, st, ap);
+            const int K = a.chunks > 0 ? a.chunks : dev::kPartChunks;
+            if (most > 0)  // K chunks per wave
+                hipLaunchKernelGGL(dev::k_gather<S>, dim3(grid_for((most + K - 1) / K * 8 * 64, block)), dim3(block), 0,
+                                   st, ap);
+        }
+    } else if (a.gather && a.lspan) {
+        // one LDS range per list; workgroups take equal shares of all chunks
+        const long nch = en.lparts.at(a.lspan).poff.back();
+        const std::size_t lds = static_cast<std::size_t>(a.lspan) * sizeof(typename S::T);
+        const unsigned g = lds_gather_grid(en, nch, lds);
+        if (nch > 0) {
+            if (a.chunks == 2)
+                hipLaunchKernelGGL((dev::k_gather_lds<S, 2>), dim3(g), dim3(dev::kLdsGatherBlock), lds, st, a);
+            else if (a.chunks == 8)
+                hipLaunchKernelGGL((dev::k_gather_lds<S, 8>), dim3(g), dim3(dev::kLdsGatherBlock), lds, st, a);
+            else
+                hipLaunchKernelGGL((dev::k_gather_lds<S, 4>), dim3(g), dim3(dev::kLdsGatherBlock), lds, st, a);
         }
     } else if (a.gather) {
         // (the B-wide row gather takes two segments per lane group at once)
@@ -596,30 +682,39 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
     GMR_HIP(hipGetLastError());
 }
 
+// Workgroups per CU of a persistent launch. The barrier needs every workgroup
+// resident at once, and the occupancy query can promise one workgroup per CU
+// more than the dispatcher actually keeps resident (measured: 7/CU reported,
+// fewer held, at 71 VGPRs x 256 lanes), so one slot of the reported occupancy
+// stays free, and at most 16 waves per CU are aimed for. A block size at which
+// only one workgroup fits per CU leaves no such slot and is refused.
+inline int persistent_per_cu(int occ, int block) {
+    if (occ < 2)
+        throw std::invalid_argument(
+            "persistent variant: only " + std::to_string(occ) + " workgroup(s) of " + std::to_string(block) +
+            " lanes fit per CU, which leaves no residency headroom for the grid barrier; use a smaller block");
+    return std::min(occ - 1, std::max(1, 1024 / block));
+}
+
 // One launch for the whole sweep, sized to one resident wave of workgroups
-// (occupancy x CUs, capped) so every workgroup is on the chip while the
-// barrier waits; a wait that outlives its deadline ends with the error word set.
+// (occupancy x CUs with the headroom above, capped) so every workgroup is on
+// the chip while the barrier waits; a wait that outlives its deadline marks
+// the launch failed (reported by the next gmr_engine_run or gmr_engine_check).
 template <class S>
 void launch_persistent(Engine& en, const SweepArgs& a, int block, int grid_cap, hipStream_t st, int tiles = 1,
                        int tile_w = 0, int vbytes = 4) {
-    // The barrier needs every workgroup resident at once. The occupancy query
-    // can promise one workgroup per CU more than the dispatcher actually keeps
-    // resident (measured: 7/CU reported, fewer held, at 71 VGPRs x 256 lanes),
-    // so keep one slot of headroom and aim for 16 waves per CU.
     int occ = 0;
     GMR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, dev::k_persistent<S>, block, 0));
-    if (occ < 1) throw DeviceError("persistent kernel does not fit on a CU");
-    const int per_cu = std::max(1, std::min(occ - 1, std::max(1, 1024 / block)));
+    const int per_cu = persistent_per_cu(occ, block);
     int grid = per_cu * en.num_cus;
     if (grid_cap > 0) grid = std::min(grid, grid_cap);
-    const unsigned long long deadline = static_cast<unsigned long long>(en.wall_khz) * 1000ULL * 2ULL;  // 2 s
     en.last_grid = grid;
     en.last_per_cu = per_cu;
-    hipLaunchKernelGGL(dev::k_persistent<S>, dim3(grid), dim3(block), 0, st, a, en.d_barrier, en.barrier_base,
-                       deadline, tiles, tile_w > 0 ? tile_w : a.Bt, vbytes);
+    hipLaunchKernelGGL(dev::k_persistent<S>, dim3(grid), dim3(block), 0, st, a, en.barrier.ref(), tiles,
+                       tile_w > 0 ? tile_w : a.Bt, vbytes);
     GMR_HIP(hipGetLastError());
-    en.barrier_base += static_cast<unsigned long long>(grid) *
-                       (static_cast<unsigned long long>(tiles) * static_cast<unsigned long long>(a.L + 2) - 1ULL);
+    en.barrier.advance(static_cast<unsigned long long>(grid) *
+                       (static_cast<unsigned long long>(tiles) * static_cast<unsigned long long>(a.L + 2) - 1ULL));
 }
 
 template <class S>
@@ -632,7 +727,9 @@ void run_tile(Engine& en, int semiring, SweepArgs a, const gmr_run_opts& o, hipS
         // level, gather form and pass count.
         auto key = std::make_tuple(semiring * 1000003 + kinds_hash(en, o), reinterpret_cast<uintptr_t>(a.X),
                                    reinterpret_cast<uintptr_t>(a.Y), a.B, a.Bt,
-                                   a.gather | (o.flags << 1) | (o.xcd_passes << 8), block,
+                                   a.gather | (o.flags << 1) | (o.xcd_passes << 8) | (o.chunks << 12) |
+                                       (o.lds_values > 0 ? (__builtin_ctz(o.lds_values) << 16) : 0),
+                                   block,
                                    reinterpret_cast<uintptr_t>(st));
         auto it = en.graphs.find(key);
         if (it == en.graphs.end()) {
@@ -715,10 +812,32 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
         a.xpass0 = 0;
         a.xpass1 = passes;
     }
+    const bool lds_gather = (o.flags & (GMR_RUN_BITS | GMR_RUN_LDS_RANGES)) != 0;
+    if ((o.flags & GMR_RUN_BITS) && (o.flags & GMR_RUN_LDS_RANGES))
+        throw std::invalid_argument("GMR_RUN_BITS and GMR_RUN_LDS_RANGES exclude each other");
+    if (o.lds_values != 0 && (o.lds_values < 0 || (o.lds_values % 1024) != 0))
+        throw std::invalid_argument("lds_values must be 0 or a multiple of 1024");
+    if (o.chunks != 0) {
+        const bool ok = lds_gather ? (o.chunks == 2 || o.chunks == 4 || o.chunks == 8)
+                                   : (o.chunks == 1 || o.chunks == 2 || o.chunks == 4);
+        if (!ok) throw std::invalid_argument(lds_gather ? "chunks must be 0, 2, 4 or 8 for the LDS-range gathers"
+                                                        : "chunks must be 0, 1, 2 or 4");
+    }
+    a.chunks = o.chunks > 0 ? o.chunks : (lds_gather ? 4 : dev::kPartChunks);
+    // EV_LDS window of the per-level launches (lds_values, unless it sizes an LDS-range gather)
+    a.lds_window = dev::kLdsWindow;
+    if (!lds_gather && o.lds_values > 0) {
+        // (the kernel also holds a small static array for its block min/max)
+        if (o.lds_values < 1024 || o.lds_values > 16384 ||
+            static_cast<long>(o.lds_values) * 4 + 1024 > static_cast<long>(en.max_lds))
+            throw std::invalid_argument("lds_values (the EV_LDS window) must be in [1024, 16384]");
+        a.lds_window = o.lds_values;
+    }
+    if (lds_gather && (!gather || !en.resident || B != 1 || o.variant == GMR_VARIANT_PERSISTENT))
+        throw std::invalid_argument(
+            "GMR_RUN_BITS and GMR_RUN_LDS_RANGES need one vector, gather emission (Resident layout) and the levels or "
+            "graph variant");
     if (o.flags & GMR_RUN_BITS) {
-        if (!gather || !en.resident || B != 1 || o.variant == GMR_VARIANT_PERSISTENT)
-            throw std::invalid_argument(
-                "GMR_RUN_BITS needs one vector, gather emission (Resident layout) and the levels or graph variant");
         a.bits = 1;
         a.vbits = en.ensure_bits();
         a.tw = en.bits_tw;
@@ -727,7 +846,12 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
         a.xparts = 0;
         if (!(o.flags & GMR_RUN_SINGLE_LIST)) {
             // the LDS-range lists (the default); GMR_RUN_SINGLE_LIST keeps the flat list
-            const Engine::XParts& x = en.bit_lists();
+            const int range = o.lds_values > 0 ? o.lds_values : dev::kBitRangeBits;
+            if (range < 131072 || range > 1048576 || range % 64 != 0 ||
+                static_cast<long>(range / 8) > static_cast<long>(en.max_lds))
+                throw std::invalid_argument("lds_values (bits per range of GMR_RUN_BITS) must be in [131072, 1048576]");
+            const Engine::XParts& x = en.bit_lists(range);
+            a.bit_range = range;
             a.xsym = x.d_sym;
             a.xbits = x.d_bits;
             a.xord = x.d_ord;
@@ -736,6 +860,24 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
             a.nbl = static_cast<int>(x.poff.size()) - 1;
             a.flat = 0;
         }
+    }
+    if (o.flags & GMR_RUN_LDS_RANGES) {
+        const int span = o.lds_values > 0 ? o.lds_values : 32768;
+        if (span < 4096 || static_cast<long>(span) * vbytes > static_cast<long>(en.max_lds))
+            throw std::invalid_argument(
+                "lds_values (values per range of GMR_RUN_LDS_RANGES) must be at least 4096 and fit the LDS of a "
+                "workgroup (" + std::to_string(en.max_lds) + " bytes)");
+        const Engine::XParts& x = en.lds_lists(span);
+        a.lspan = span;
+        a.nkey = static_cast<long>(en.U) + en.NL;
+        a.xsym = x.d_sym;
+        a.xbits = x.d_bits;
+        a.xord = x.d_ord;
+        a.xrow = x.d_row;
+        a.xpoff = x.d_poff;
+        a.nbl = static_cast<int>(x.poff.size()) - 1;
+        a.flat = 0;
+        a.xparts = 0;
     }
     a.GR = en.gather_lanes;
     a.eval_kind = o.eval_kind;
@@ -1328,8 +1470,7 @@ struct Batch {
     dev::Tile* d_tiles = nullptr;
     long* d_phase_off = nullptr;
     std::size_t cap_args = 0, cap_tiles = 0, cap_phase = 0;
-    dev::GridBarrier* d_barrier = nullptr;
-    unsigned long long barrier_base = 0;
+    BarrierState barrier;
     int num_cus = 0, wall_khz = 100000;
     hipStream_t capture_stream = nullptr;
     std::vector<hipStream_t> branch;
@@ -1347,9 +1488,9 @@ Batch::~Batch() {
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
         if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
-    for (void* p : {static_cast<void*>(d_args), static_cast<void*>(d_tiles), static_cast<void*>(d_phase_off),
-                    static_cast<void*>(d_barrier)})
+    for (void* p : {static_cast<void*>(d_args), static_cast<void*>(d_tiles), static_cast<void*>(d_phase_off)})
         if (p) (void)hipFree(p);
+    barrier.release();
     for (auto st : branch) (void)hipStreamDestroy(st);
     for (auto ev : branch_ev) (void)hipEventDestroy(ev);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
@@ -1424,7 +1565,8 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
         // the XCD-partitioned gather (which needs workgroup b on XCD b % 8) is
         // replaced there by the single chunked list.
         if (variant == GMR_VARIANT_PERSISTENT) args[g].xparts = 0;
-        if (args[g].bits) throw std::invalid_argument("GMR_RUN_BITS runs only in the levels variant of a batch");
+        if (args[g].bits || args[g].lspan)
+            throw std::invalid_argument("GMR_RUN_BITS and GMR_RUN_LDS_RANGES run only in the levels variant of a batch");
     }
     if (variant == GMR_VARIANT_GRAPH) {
         std::size_t key = mix(static_cast<std::size_t>(semiring), static_cast<std::size_t>(block));
@@ -1530,15 +1672,13 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
     grow_upload(&bt.d_args, &bt.cap_args, args);
     int occ = 0;
     GMR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, dev::k_multi<S>, block, 0));
-    if (occ < 1) throw DeviceError("multi-grammar kernel does not fit on a CU");
-    const int per_cu = std::max(1, std::min(occ - 1, std::max(1, 1024 / block)));
+    const int per_cu = persistent_per_cu(occ, block);
     int grid = per_cu * bt.num_cus;
     if (o[0].grid_cap > 0) grid = std::min(grid, o[0].grid_cap);
-    const unsigned long long deadline = static_cast<unsigned long long>(bt.wall_khz) * 2000ULL;
     hipLaunchKernelGGL(dev::k_multi<S>, dim3(grid), dim3(block), 0, st, bt.d_args, bt.d_tiles, bt.d_phase_off,
-                       bt.last_phases, bt.d_barrier, bt.barrier_base, deadline);
+                       bt.last_phases, bt.barrier.ref());
     GMR_HIP(hipGetLastError());
-    bt.barrier_base += static_cast<unsigned long long>(grid) * static_cast<unsigned long long>(bt.last_phases - 1);
+    bt.barrier.advance(static_cast<unsigned long long>(grid) * static_cast<unsigned long long>(bt.last_phases - 1));
 }
 
 }  // namespace
@@ -1619,14 +1759,34 @@ int gmr_engine_info(const gmr_engine* h, long long info[16]) {
     return GMR_OK;
 }
 
+namespace {
+const char* const kBarrierMsg =
+    "a persistent launch's grid barrier timed out (its workgroups were not all resident): the Y it wrote is "
+    "invalid; this call did not run";
+}
+
 int gmr_engine_run(gmr_engine* h, int semiring, const void* d_X, int B, void* d_Y,
                    const gmr_run_opts* opts, void* stream) {
     if (!h) return dev_fail(GMR_ERR_INVALID, "engine is NULL");
+    // A failure of an earlier launch (seen without synchronising) is reported
+    // once, by the next call, which then does not run.
+    if (h->e.barrier.take_failure()) return dev_fail(GMR_ERR_BARRIER, kBarrierMsg);
     gmr_run_opts o{};
     if (opts) o = *opts;
-    return dev_guarded([&] {
+    int rc = dev_guarded([&] {
         h->e.run(semiring, static_cast<const float*>(d_X), B, d_Y, o, static_cast<hipStream_t>(stream));
     });
+    // A timed run has synchronised its stream: its own failure is known now.
+    if (rc == GMR_OK && (o.flags & GMR_RUN_TIMED) && h->e.barrier.take_failure())
+        return dev_fail(GMR_ERR_BARRIER, "this persistent launch's grid barrier timed out: the Y it wrote is invalid");
+    return rc;
+}
+
+int gmr_engine_set_barrier_timeout(gmr_engine* h, long long microseconds) {
+    if (!h) return dev_fail(GMR_ERR_INVALID, "engine is NULL");
+    if (microseconds < 0) return dev_fail(GMR_ERR_INVALID, "timeout must be >= 0");
+    h->e.barrier.timeout_us = microseconds;
+    return GMR_OK;
 }
 
 int gmr_engine_time_levels(gmr_engine* h, int semiring, const void* d_X, int B, void* d_Y,
@@ -1677,8 +1837,9 @@ int gmr_engine_debug(gmr_engine* h, long long out[8]) {
     return dev_guarded([&] {
         gmr::dev::GridBarrier b{};
         GMR_HIP(hipDeviceSynchronize());
-        GMR_HIP(hipMemcpy(&b, h->e.d_barrier, sizeof(b), hipMemcpyDeviceToHost));
-        long long v[8] = {static_cast<long long>(b.arrived), b.error, static_cast<long long>(h->e.barrier_base),
+        GMR_HIP(hipMemcpy(&b, h->e.barrier.d, sizeof(b), hipMemcpyDeviceToHost));
+        const long long failed = h->e.barrier.h_fail ? *h->e.barrier.h_fail : 0;
+        long long v[8] = {static_cast<long long>(b.arrived), failed, static_cast<long long>(h->e.barrier.base),
                           h->e.wall_khz, h->e.num_cus, h->e.last_grid, h->e.last_per_cu,
                           static_cast<long long>(h->e.graphs.size())};
         std::memcpy(out, v, sizeof(v));
@@ -1713,9 +1874,7 @@ int gmr_batch_create(gmr_engine* const* engines, int n, gmr_batch** out) {
         GMR_HIP(hipDeviceGetAttribute(&h->b.num_cus, hipDeviceAttributeMultiprocessorCount, dev));
         GMR_HIP(hipDeviceGetAttribute(&h->b.wall_khz, hipDeviceAttributeWallClockRate, dev));
         if (h->b.wall_khz <= 0) h->b.wall_khz = 100000;
-        GMR_HIP(hipMalloc(&h->b.d_barrier, sizeof(gmr::dev::GridBarrier)));
-        GMR_HIP(hipMemset(h->b.d_barrier, 0, sizeof(gmr::dev::GridBarrier)));
-        GMR_HIP(hipDeviceSynchronize());
+        h->b.barrier.init(h->b.wall_khz);
     });
     if (rc != GMR_OK) {
         delete h;
@@ -1729,6 +1888,7 @@ int gmr_batch_run(gmr_batch* h, int semiring, const void* const* d_X, const int*
                   const gmr_run_opts* opts, void* stream) {
     if (!h || !d_X || !B || !d_Y) return dev_fail(GMR_ERR_INVALID, "NULL argument");
     if (variant < 0 || variant > 2) return dev_fail(GMR_ERR_INVALID, "unknown variant");
+    if (h->b.barrier.take_failure()) return dev_fail(GMR_ERR_BARRIER, kBarrierMsg);
     return dev_guarded([&] {
         const float* const* X = reinterpret_cast<const float* const*>(d_X);
         auto st = static_cast<hipStream_t>(stream);
@@ -1743,18 +1903,14 @@ int gmr_batch_run(gmr_batch* h, int semiring, const void* const* d_X, const int*
 
 int gmr_batch_check(gmr_batch* h, int* barrier_error) {
     if (!h || !barrier_error) return dev_fail(GMR_ERR_INVALID, "NULL argument");
-    return dev_guarded([&] {
-        gmr::dev::GridBarrier b{};
-        GMR_HIP(hipDeviceSynchronize());
-        GMR_HIP(hipMemcpy(&b, h->b.d_barrier, sizeof(b), hipMemcpyDeviceToHost));
-        *barrier_error = 0;
-        if (b.error || b.arrived != h->b.barrier_base) {
-            GMR_HIP(hipMemset(h->b.d_barrier, 0, sizeof(b)));
-            GMR_HIP(hipDeviceSynchronize());
-            h->b.barrier_base = 0;
-            *barrier_error = 1;
-        }
-    });
+    return dev_guarded([&] { *barrier_error = h->b.barrier.check(); });
+}
+
+int gmr_batch_set_barrier_timeout(gmr_batch* h, long long microseconds) {
+    if (!h) return dev_fail(GMR_ERR_INVALID, "batch is NULL");
+    if (microseconds < 0) return dev_fail(GMR_ERR_INVALID, "timeout must be >= 0");
+    h->b.barrier.timeout_us = microseconds;
+    return GMR_OK;
 }
 
 void gmr_batch_destroy(gmr_batch* h) { delete h; }

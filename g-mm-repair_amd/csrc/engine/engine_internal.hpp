@@ -15,6 +15,29 @@
 
 namespace gmr {
 
+// The grid barrier of the persistent launches (one engine's, or one batch's):
+// the device counter and failed epoch, and a word of pinned host memory that a
+// timed-out wait sets, so a failure is seen on the host without synchronising.
+struct BarrierState {
+    dev::GridBarrier* d = nullptr;
+    int* h_fail = nullptr;           // pinned host word (mapped)
+    int* d_fail = nullptr;           // its device address
+    unsigned long long base = 0;     // arrivals made by earlier launches
+    long long timeout_us = 0;        // a wait's limit (0: 2 s)
+    int wall_khz = 100000;
+    void init(int wall_khz);
+    void release() noexcept;
+    // The view a launch of `grid` workgroups passes to its kernel; then
+    // advance() by the arrivals it makes.
+    dev::BarrierRef ref() const;
+    void advance(unsigned long long arrivals) { base += arrivals; }
+    // A timed-out wait since the last call (clears the word; no device sync).
+    bool take_failure();
+    // Synchronises the device: 1 if a wait timed out or the arrivals disagree
+    // with what the launches made (the state is then reset), else 0.
+    int check();
+};
+
 struct Engine {
     // structure (host copies of what the launch loop needs)
     int rows = 0, cols = 0, U = 0, NL = 0, L = 0, nvals = 0, layout = 0;
@@ -54,8 +77,11 @@ struct Engine {
         std::vector<int> poff;       // 8 x passes + 1 chunk offsets
     };
     std::map<int, XParts> xparts;
-    XParts bparts;                   // Boolean bit mode: lists by LDS-sized bit range
-    const XParts& bit_lists();
+    std::map<int, XParts> bparts;    // Boolean bit mode: lists by LDS-sized bit range, per range size
+    const XParts& bit_lists(int range_bits);
+    std::map<int, XParts> lparts;    // one vector, GMR_RUN_LDS_RANGES: lists by value range, per range size
+    const XParts& lds_lists(int span);
+    int max_lds = 65536;             // LDS bytes a workgroup may use
     const XParts& xparts_for(int passes);
     int auto_xcd_passes(int Bs = 1, int Bt = 1) const;  // pass count of the XCD-partitioned gather
     std::vector<int> crow_host;      // [0] = number of top-sequence symbols
@@ -73,8 +99,7 @@ struct Engine {
 
     // device facts and timing
     int num_cus = 0, wall_khz = 0;
-    ::gmr::dev::GridBarrier* d_barrier = nullptr;
-    unsigned long long barrier_base = 0;  // arrivals recorded by earlier launches
+    BarrierState barrier;
     int last_grid = 0, last_per_cu = 0;
     int* d_kinds = nullptr;          // per-level eval kinds of the latest run
     std::vector<int> kinds_host;
@@ -140,7 +165,7 @@ struct Engine {
     void run(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st);
     // 1 if a persistent launch's barrier timed out since the last check (and
     // the barrier state was reset), else 0. Synchronises the device.
-    int check_barrier();
+    int check_barrier() { return barrier.check(); }
     const int* device_kinds(const int* host_kinds);
     void set_gather_order(int order);
     void time_levels(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, int reps, float* out,

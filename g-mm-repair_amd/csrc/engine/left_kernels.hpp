@@ -142,7 +142,7 @@ __global__ void k_left_fold(LeftArgs a) {
             const long col = a.fcol[a.ford[c] - 1 + __popcll(bits & upto)];
             const bool starts_here = (bits & upto) != 0ull;
             const bool ends_here = lane < 63 || a.col_off[col + 1] == p + 1;
-            if (starts_here && ends_here) Y[col * a.B] = v;
+            if (starts_here && ends_here) Y[col * a.B] = S::fin(v);
             else if (!S::skip(v)) S::emit(Y + col * a.B, v);
         }
     }

@@ -28,6 +28,9 @@ struct PlusTimesOp {
     static __device__ __forceinline__ T leaf(float v, float x) { return v * x; }
     static __device__ __forceinline__ T apply(T v, float c) { return c * v; }
     static __device__ __forceinline__ bool skip(T) { return false; }
+    // A whole row's sum stored into y: what (+)-combining it into a y that
+    // starts at zero() gives (0 + v = v; only the sign of a zero differs).
+    static __device__ __forceinline__ T fin(T v) { return v; }
     // Hardware global_atomic_add_f32 (no compare-and-swap loop).
     static __device__ __forceinline__ void emit(T* y, T v) { unsafeAtomicAdd(y, v); }
 };
@@ -39,6 +42,7 @@ struct BooleanOp {
     static __device__ __forceinline__ T leaf(float v, float x) { return (v != 0.0f && x != 0.0f) ? 1 : 0; }
     static __device__ __forceinline__ T apply(T v, float) { return v; }
     static __device__ __forceinline__ bool skip(T v) { return v == 0; }  // OR with 0 is a no-op
+    static __device__ __forceinline__ T fin(T v) { return v; }            // 0 | v
     // Values are 0 or 1 and y starts at 0, so OR-ing a 1 in is storing 1: any
     // number of writers store the same value, with no read-modify-write (a
     // device-scope atomic would travel to the memory side; a store stays in L2).
@@ -59,6 +63,9 @@ struct TropicalOp {
     static __device__ __forceinline__ T apply(T v, float) { return v; }
     // y starts at +inf, so min with anything >= +inf leaves it unchanged.
     static __device__ __forceinline__ bool skip(T v) { return v >= kTropicalInf; }
+    // min(+inf, v): y starts at +inf, so a row whose finite leaf sums all exceed
+    // 1e9 still reads +inf, as with atomicMin into the filled y.
+    static __device__ __forceinline__ T fin(T v) { return v < kTropicalInf ? v : kTropicalInf; }
     static __device__ __forceinline__ void emit(T* y, T v) { atomicMin(y, v); }
 };
 
@@ -123,6 +130,15 @@ struct SweepArgs {
     int tw;
     int bits;
     int nbl;                 // bit mode, LDS ranges: lists (in xsym ... xpoff)
+    // One vector, values staged in LDS by range (GMR_RUN_LDS_RANGES): list p of
+    // xsym ... xpoff holds the symbols whose value key (terminal t -> t, rule
+    // slot s -> U + s) lies in [p * lspan, (p + 1) * lspan), stored as the key's
+    // index inside the range.
+    int lspan;               // values per range (0: not this gather)
+    long nkey;               // U + rule slots
+    int lds_window;          // EV_LDS left-child window of the per-level launches (values)
+    int bit_range;           // bit mode: bits per LDS range
+    int chunks;              // 64-symbol chunks per wave in flight (chunked single-vector gathers)
 };
 
 // Per-level node-update kernels (one is chosen for each level):
@@ -185,6 +201,14 @@ struct Op4 {
         if (n > 2 && !S::skip(v.z)) S::emit(y + 2, v.z);
         if (n > 3 && !S::skip(v.w)) S::emit(y + 3, v.w);
     }
+    static __device__ __forceinline__ T4 fin(T4 a) {
+        T4 r;
+        r.x = S::fin(a.x);
+        r.y = S::fin(a.y);
+        r.z = S::fin(a.z);
+        r.w = S::fin(a.w);
+        return r;
+    }
     static __device__ __forceinline__ void st_n(T* y, T4 v, int n, int aligned) {
         if (n >= 4 && aligned) {
             st(y, v);
@@ -209,7 +233,7 @@ __host__ __device__ __forceinline__ int lanes_for(int Bs, int vec) {
 // Whether the init phase zeroes only the listed split rows: a row-segment
 // gather stores every other row whole.
 __host__ __device__ __forceinline__ bool zero_list(const SweepArgs& a) {
-    return a.gather && !a.flat && !a.xparts;
+    return a.gather && !a.flat && !a.xparts && !a.lspan;
 }
 
 // T[p*Bs + b] = leaf(V, X[col*B + b]) for b < Bt, zero for p = 0 and for the
@@ -378,7 +402,7 @@ __device__ __forceinline__ void ph_emit(const SweepArgs& a, const int2* em, int 
 // row add into the zeroed row with the semiring's atomic.
 template <class S>
 __device__ __forceinline__ void gather_out(typename S::T* y, typename S::T v, int split) {
-    if (!split) *y = v;
+    if (!split) *y = S::fin(v);
     else if (!S::skip(v)) S::emit(y, v);
 }
 
@@ -422,7 +446,7 @@ __device__ void ph_gather_rows_v4(const SweepArgs& a, long q0, long q1, long gid
     };
     auto store = [&](const int4& sg, int b, int nb, const T4& v) {
         T* y = Y + static_cast<long>(sg.x) * a.B + b;
-        if (!sg.w) O::st_n(y, v, nb, a.yvec);
+        if (!sg.w) O::st_n(y, O::fin(v), nb, a.yvec);
         else O::emit_n(y, v, nb);
     };
     auto one = [&](const int4& sg) {
@@ -551,7 +575,7 @@ __device__ void ph_gather_flat(const SweepArgs& a, long c0, long c1, long wid, l
             const int r = a.frow[a.ford[c] - 1 + __popcll(bits & upto)];
             const bool starts_here = (bits & upto) != 0ull;
             const bool ends_here = lane < 63 || a.crow[r + 1] == p + 1;
-            if (starts_here && ends_here) Y[r] = v;
+            if (starts_here && ends_here) Y[r] = S::fin(v);
             else if (!S::skip(v)) S::emit(Y + r, v);
         }
     }
@@ -567,7 +591,7 @@ constexpr int kPartChunks = 2;
 // the chunked gather; every row piece is combined atomically into the zeroed y,
 // since the row's other symbols live in other lists. `blk`/`nblk` are the raw
 // workgroup index and count (not renumbered).
-template <class S>
+template <class S, int K>
 __device__ void ph_gather_parts(const SweepArgs& a, long blk, long nblk, int tid_in_blk, int bdim) {
     using T = typename S::T;
     const T* W = tp<S>(a.W0);
@@ -600,9 +624,8 @@ __device__ void ph_gather_parts(const SweepArgs& a, long blk, long nblk, int tid
             if (!S::skip(v)) S::emit(Y + r, v);
         }
     };
-    // kPartChunks chunks per wave at once: all symbol ids, then all values,
-    // then the scans (the later chunks' loads hide behind the first scan).
-    constexpr int K = kPartChunks;
+    // K chunks per wave at once: all symbol ids, then all values, then the
+    // scans (the later chunks' loads hide behind the first scan).
     for (long pass = a.xpass0; pass < a.xpass1; ++pass)
     for (long q = p0; q < 8; q += pstep) {
         const long p = pass * 8 + q;
@@ -625,7 +648,7 @@ __device__ void ph_gather_parts(const SweepArgs& a, long blk, long nblk, int tid
 // with a 16 NV-byte value row per symbol (NV dwordx4 per lane) and the
 // segmented scan and the atomics on its 4 NV components. Used for Bs = 4; at
 // Bs = 8 (32-byte rows) it measured no faster than the row segments.
-template <class S, int NV>
+template <class S, int NV, int K>
 __device__ void ph_gather_parts4(const SweepArgs& a, long blk, long nblk, int tid_in_blk, int bdim) {
     using T = typename S::T;
     using O = Op4<S>;
@@ -670,7 +693,6 @@ __device__ void ph_gather_parts4(const SweepArgs& a, long blk, long nblk, int ti
                 if (4 * n < a.Bt) O::emit_n(Y + r * a.B + 4 * n, v[n], a.Bt - 4 * n);
         }
     };
-    constexpr int K = kPartChunks;
     for (long pass = a.xpass0; pass < a.xpass1; ++pass)
     for (long q = p0; q < 8; q += pstep) {
         const long p = pass * 8 + q;
@@ -693,11 +715,20 @@ __device__ void ph_gather_parts4(const SweepArgs& a, long blk, long nblk, int ti
     }
 }
 
+// Chunks per wave in flight: a.chunks of 1, 2 or 4 (0: kPartChunks).
 template <class S>
 __device__ __forceinline__ void ph_gather_parts_any(const SweepArgs& a, long blk, long nblk, int tid_in_blk,
                                                     int bdim) {
-    if (a.vec) ph_gather_parts4<S, 1>(a, blk, nblk, tid_in_blk, bdim);  // Bs = 4 only
-    else ph_gather_parts<S>(a, blk, nblk, tid_in_blk, bdim);
+    const int k = a.chunks > 0 ? a.chunks : kPartChunks;
+    if (a.vec) {  // Bs = 4 only
+        if (k == 1) ph_gather_parts4<S, 1, 1>(a, blk, nblk, tid_in_blk, bdim);
+        else if (k == 4) ph_gather_parts4<S, 1, 4>(a, blk, nblk, tid_in_blk, bdim);
+        else ph_gather_parts4<S, 1, 2>(a, blk, nblk, tid_in_blk, bdim);
+    } else {
+        if (k == 1) ph_gather_parts<S, 1>(a, blk, nblk, tid_in_blk, bdim);
+        else if (k == 4) ph_gather_parts<S, 4>(a, blk, nblk, tid_in_blk, bdim);
+        else ph_gather_parts<S, 2>(a, blk, nblk, tid_in_blk, bdim);
+    }
 }
 
 template <class S>
@@ -864,9 +895,8 @@ __device__ __forceinline__ void block_minmax(int& lo, int& hi) {
 // blockDim-sized chunks, all threads of a block run the same iterations.
 template <class S>
 __device__ void ph_eval_lds(const SweepArgs& a, int i0, int i1, const typename S::T* src,
-                            typename S::T* dst, long bid, long nb) {
+                            typename S::T* dst, long bid, long nb, typename S::T* win, int wsize) {
     using T = typename S::T;
-    __shared__ T win[kLdsWindow];
     const T* Tt = tp<S>(a.Tv);
     for (long base = i0 + bid * blockDim.x; base < i1; base += nb * blockDim.x) {
         const long i = base + threadIdx.x;
@@ -878,7 +908,7 @@ __device__ void ph_eval_lds(const SweepArgs& a, int i0, int i1, const typename S
         }
         int lo = k.x >= 0 ? k.x : 0x7fffffff, hi = k.x >= 0 ? k.x : -1;
         block_minmax(lo, hi);
-        const bool staged = hi >= lo && hi - lo < kLdsWindow;
+        const bool staged = hi >= lo && hi - lo < wsize;
         if (staged)
             for (int t = threadIdx.x; t <= hi - lo; t += blockDim.x) win[t] = src[lo + t];
         __syncthreads();
@@ -891,16 +921,18 @@ __device__ void ph_eval_lds(const SweepArgs& a, int i0, int i1, const typename S
     }
 }
 
-// The level's kernel: kind as given, EV_LDS only for a single vector.
+// The level's kernel: kind as given, EV_LDS only for a single vector. `win`
+// (wsize values of LDS) is the EV_LDS window: dynamic LDS sized by the launch
+// in the per-level kernels, a fixed kLdsWindow array in the persistent ones.
 template <class S>
 __device__ __forceinline__ void ph_eval_kind(const SweepArgs& a, int kind, int i0, int i1,
                                              const typename S::T* src, typename S::T* dst, long tid,
-                                             long nth) {
+                                             long nth, typename S::T* win, int wsize) {
     if (kind == EV_LDS && a.Bt != 1) kind = EV_GROUP;
     switch (kind) {
         case EV_THREAD: ph_eval_thread<S>(a, i0, i1, src, dst, tid, nth); break;
         case EV_WAVE: ph_eval_wave<S>(a, i0, i1, src, dst, tid / 64, nth / 64, static_cast<int>(tid & 63)); break;
-        case EV_LDS: ph_eval_lds<S>(a, i0, i1, src, dst, tid / blockDim.x, nth / blockDim.x); break;
+        case EV_LDS: ph_eval_lds<S>(a, i0, i1, src, dst, tid / blockDim.x, nth / blockDim.x, win, wsize); break;
         default:
             if (a.vec) ph_eval_v4x4<S, 4>(a, i0, i1, src, dst, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
             else if (a.Bs == 1) ph_eval_1xU<S, 4>(a, i0, i1, src, dst, tid, nth);
@@ -944,12 +976,16 @@ __global__ void k_init(SweepArgs a) {
     ph_init<S>(a, tid, static_cast<long>(gridDim.x) * blockDim.x);
 }
 
+// Dynamic LDS of the per-level launches (the EV_LDS window, the range of the
+// LDS gathers); the launch gives its size.
+extern __shared__ __align__(16) unsigned char dyn_lds[];
+
 template <class S>
 __global__ void k_eval(SweepArgs a, int k, int kind) {
     long tid = xcd_tid();
     long nth = static_cast<long>(gridDim.x) * blockDim.x;
     ph_eval_kind<S>(a, kind, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k), tid,
-                    nth);
+                    nth, reinterpret_cast<typename S::T*>(dyn_lds), a.lds_window);
 }
 
 // k = 0: terminal emissions; k >= 1: level k's emissions.
@@ -1048,9 +1084,6 @@ __device__ void ph_bits_eval(const SweepArgs& a, int k, long tid, long nth) {
 // stored, pieces of longer rows ORed in. Everything but the bit words and the
 // row ids is loaded up front, so a chunk costs two dependent round trips.
 // Build-time tuning knobs of the bit mode (defaults: the measured best).
-#ifndef GMR_BIT_CHUNKS
-#define GMR_BIT_CHUNKS 4
-#endif
 #ifndef GMR_BIT_RANGE_LOG2
 #define GMR_BIT_RANGE_LOG2 19
 #endif
@@ -1063,7 +1096,7 @@ __device__ void ph_bits_eval(const SweepArgs& a, int k, long tid, long nth) {
 #ifndef GMR_BIT_NT
 #define GMR_BIT_NT 1
 #endif
-constexpr int kBitChunks = GMR_BIT_CHUNKS;
+constexpr int kBitChunks = 4;  // chunks per wave of the flat bit gather
 
 __device__ void ph_bits_gather(const SweepArgs& a, long tid, long nth) {
     const int lane = static_cast<int>(tid & 63);
@@ -1112,18 +1145,20 @@ __device__ void ph_bits_gather(const SweepArgs& a, long tid, long nth) {
 }
 
 // Gather from LDS: list p holds the symbols whose bits lie in range p
-// ([p * kBitRangeBits, (p + 1) * kBitRangeBits), 64 KiB). Workgroup w takes
-// the w-th equal share of all lists' chunks; for each list its share touches
-// it copies that range of the bitset into LDS, then its waves sweep the chunks
-// (kBitChunks per wave in flight) looking bits up in LDS. A row's symbols sit
-// in several lists, so every row piece that is 1 is ORed into the zeroed y
-// (by storing 1, as BooleanOp::emit does).
+// ([p * a.bit_range, (p + 1) * a.bit_range), 64 KiB by default). Workgroup w
+// takes the w-th equal share of all lists' chunks; for each list its share
+// touches it copies that range of the bitset into LDS (dynamic, a.bit_range / 8
+// bytes), then its waves sweep the chunks (K per wave in flight) looking bits
+// up in LDS. A row's symbols sit in several lists, so every row piece that is
+// 1 is ORed into the zeroed y (by storing 1, as BooleanOp::emit does).
 constexpr int kBitRangeBits = 1 << GMR_BIT_RANGE_LOG2;
 constexpr int kBitGatherBlock = GMR_BIT_BLOCK;
 constexpr int kBitWgPerCu = GMR_BIT_WG_PER_CU;
 
+template <int K>
 __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a) {
-    __shared__ unsigned lds[kBitRangeBits / 32];
+    unsigned* lds = reinterpret_cast<unsigned*>(dyn_lds);
+    const int range = a.bit_range;
     const int lane = threadIdx.x & 63;
     // wave-uniform, so a chunk's words (row starts, row order) are read at one uniform address
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
@@ -1138,8 +1173,8 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
         while (a.xpoff[p + 1] <= c0) ++p;  // ends below nbl, since c0 < N
         const long c1 = a.xpoff[p + 1] < g1 ? static_cast<long>(a.xpoff[p + 1]) : g1;
         __syncthreads();  // the previous range's lookups are done
-        const long w0 = static_cast<long>(p) * (kBitRangeBits / 64);
-        for (int i = threadIdx.x; i < kBitRangeBits / 64; i += blockDim.x) {
+        const long w0 = static_cast<long>(p) * (range / 64);
+        for (int i = threadIdx.x; i < range / 64; i += blockDim.x) {
             const unsigned long long v = w0 + i < a.nvw ? a.vbits[w0 + i] : 0ull;
             lds[2 * i] = static_cast<unsigned>(v);
             lds[2 * i + 1] = static_cast<unsigned>(v >> 32);
@@ -1151,14 +1186,14 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
         // outstanding. Software-pipelined: an iteration issues its row-id loads,
         // then the next iteration's symbol and chunk-word loads, so the two
         // round trips overlap.
-        const long step = static_cast<long>(nwave) * kBitChunks;
-        long c = c0 + static_cast<long>(wave) * kBitChunks;
-        unsigned l[kBitChunks];
-        unsigned long long fb[kBitChunks];
-        int ord[kBitChunks];
+        const long step = static_cast<long>(nwave) * K;
+        long c = c0 + static_cast<long>(wave) * K;
+        unsigned l[K];
+        unsigned long long fb[K];
+        int ord[K];
         auto load_chunks = [&](long cc, unsigned* lo, unsigned long long* fo, int* oo) {
 #pragma unroll
-            for (int j = 0; j < kBitChunks; ++j) {
+            for (int j = 0; j < K; ++j) {
                 const long cj = cc + j < c1 ? cc + j : c1 - 1;
 #if GMR_BIT_NT
                 lo[j] = __builtin_nontemporal_load(loc + cj * 64 + lane);
@@ -1171,11 +1206,11 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
         };
         load_chunks(c, l, fb, ord);  // (past the segment: its last chunk again, unused)
         while (c < c1) {
-            bool hit[kBitChunks];
-            int ri[kBitChunks];
+            bool hit[K];
+            int ri[K];
 #pragma unroll
-            for (int j = 0; j < kBitChunks; ++j) {
-                const bool in_range = l[j] < static_cast<unsigned>(kBitRangeBits);
+            for (int j = 0; j < K; ++j) {
+                const bool in_range = l[j] < static_cast<unsigned>(range);
                 const unsigned li = in_range ? l[j] : 0u;
                 const bool b = (((lds[li >> 5] >> (li & 31)) & 1u) & (in_range ? 1u : 0u)) != 0u;
                 const unsigned long long m = __ballot(b);
@@ -1185,19 +1220,108 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
                 hit[j] = c + j < c1 && ((ends >> lane) & 1ull) != 0ull && ((m & upto) >> first) != 0ull;
                 ri[j] = hit[j] ? ord[j] - 1 + __popcll(st) : 0;
             }
-            int row[kBitChunks];
+            int row[K];
 #pragma unroll
-            for (int j = 0; j < kBitChunks; ++j) row[j] = a.xrow[ri[j]];
+            for (int j = 0; j < K; ++j) row[j] = a.xrow[ri[j]];
             const long cn = c + step;
             load_chunks(cn, l, fb, ord);  // unconditional: a branch here would make the stores wait for it
             // The row ids are used here, after the prefetch, so the compiler cannot
             // sink their loads into the stores' branches (where the wait for them
             // would also wait for the prefetch). An empty statement: no code.
 #pragma unroll
-            for (int j = 0; j < kBitChunks; ++j) asm volatile("" : : "v"(row[j]));
+            for (int j = 0; j < K; ++j) asm volatile("" : : "v"(row[j]));
 #pragma unroll
-            for (int j = 0; j < kBitChunks; ++j)
+            for (int j = 0; j < K; ++j)
                 if (hit[j]) Y[row[j]] = 1;
+            c = cn;
+        }
+        c0 = c1;
+    }
+}
+
+// One vector, 4-byte values staged in LDS by range (GMR_RUN_LDS_RANGES): the
+// same structure as the bit-mode gather above with a value per entry. List p
+// holds the symbols whose value key lies in [p * lspan, (p + 1) * lspan); a
+// workgroup copies that range of the terminal and rule values into LDS
+// (dynamic, lspan values) once per list its share of chunks touches, looks
+// every symbol's value up there, reduces each 64-symbol chunk per row piece
+// with the segmented shuffle scan, and combines every row piece atomically
+// into the zeroed y (a row's symbols sit in many lists). No value lookup
+// reaches the L1/L2 caches; the cost moves to one atomic per row piece.
+constexpr int kLdsGatherBlock = 1024;
+
+template <class S, int K>
+__global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
+    using T = typename S::T;
+    T* lds = reinterpret_cast<T*>(dyn_lds);
+    const int span = a.lspan;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int nwave = blockDim.x >> 6;
+    const T* Tt = tp<S>(a.Tv);
+    const T* W = tp<S>(a.W0);
+    T* Y = tp<S>(a.Y);
+    const unsigned* loc = reinterpret_cast<const unsigned*>(a.xsym);  // key index inside the list's range
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
+    const long N = a.xpoff[a.nbl];
+    const long g0 = N * blockIdx.x / gridDim.x, g1 = N * (blockIdx.x + 1) / gridDim.x;
+    int p = 0;
+    for (long c0 = g0; c0 < g1;) {
+        while (a.xpoff[p + 1] <= c0) ++p;  // ends below nbl, since c0 < N
+        const long c1 = a.xpoff[p + 1] < g1 ? static_cast<long>(a.xpoff[p + 1]) : g1;
+        __syncthreads();  // the previous range's lookups are done
+        const long k0 = static_cast<long>(p) * span;
+        for (int i = threadIdx.x; i < span; i += blockDim.x) {
+            const long k = k0 + i;
+            lds[i] = k < a.U ? Tt[k] : (k < a.nkey ? W[k - a.U] : S::zero());
+        }
+        __syncthreads();
+        // As in the bit-mode gather: unconditional, clamped loads, and the next
+        // iteration's chunk loads issued before this one's emits.
+        const long step = static_cast<long>(nwave) * K;
+        long c = c0 + static_cast<long>(wave) * K;
+        unsigned l[K];
+        unsigned long long fb[K];
+        int ord[K];
+        auto load_chunks = [&](long cc) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const long cj = cc + j < c1 ? cc + j : c1 - 1;
+                l[j] = __builtin_nontemporal_load(loc + cj * 64 + lane);
+                fb[j] = a.xbits[cj];
+                ord[j] = a.xord[cj];
+            }
+        };
+        load_chunks(c);
+        while (c < c1) {
+            T v[K];
+            bool end[K];
+            int ri[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) v[j] = lds[l[j] < static_cast<unsigned>(span) ? l[j] : 0u];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (l[j] >= static_cast<unsigned>(span)) v[j] = S::zero();  // padding
+                const unsigned long long bits = fb[j];
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const T t = __shfl_up(v[j], d, 64);
+                    if (lane >= d && ((bits >> (lane - d + 1)) & ((1ull << d) - 1ull)) == 0ull) v[j] = S::oplus(v[j], t);
+                }
+                const bool ends = lane == 63 || ((bits >> (lane + 1)) & 1ull) != 0ull;
+                end[j] = c + j < c1 && ends && !S::skip(v[j]);
+                ri[j] = end[j] ? ord[j] - 1 + __popcll(bits & upto) : 0;
+            }
+            int row[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) row[j] = a.xrow[ri[j]];
+            const long cn = c + step;
+            load_chunks(cn);
+#pragma unroll
+            for (int j = 0; j < K; ++j) asm volatile("" : : "v"(row[j]));
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (end[j]) S::emit(Y + row[j], v[j]);
             c = cn;
         }
         c0 = c1;
@@ -1220,25 +1344,38 @@ __global__ void k_bits_gather(SweepArgs a) {
 // workgroup's writes (agent-scope release), adds one arrival to a monotonic
 // 64-bit counter and waits for the counter to reach `target` (all workgroups of
 // this barrier), then acquires. Every wait is bounded: after `deadline` ticks
-// of the 100 MHz wall clock, or as soon as another workgroup has given up, it
-// stops waiting and sets the sticky error word, so the kernel always ends.
+// of the 100 MHz wall clock it stops waiting, records the launch's epoch as
+// failed and sets a word in pinned host memory (so the host sees the failure
+// without synchronising); waits of the same launch see the failed epoch and
+// stop too, so the kernel always ends. Every workgroup still makes all of its
+// arrivals, so the counter stays consistent across launches, and a later
+// launch (another epoch) is not affected by an earlier one's failure.
 struct GridBarrier {
     unsigned long long arrived;  // monotonic across launches
-    int error;                   // 1 = a wait timed out (results invalid)
-    int pad;
+    unsigned long long failed;   // epoch of the last launch whose wait timed out (0: none)
 };
 
-__device__ __forceinline__ void grid_barrier(GridBarrier* gb, unsigned long long target,
-                                             unsigned long long deadline) {
+// One launch's view of the barrier.
+struct BarrierRef {
+    GridBarrier* gb;
+    int* fail_host;               // device address of the pinned host word set on a timeout
+    unsigned long long base;      // arrivals made by earlier launches
+    unsigned long long deadline;  // wall-clock ticks a wait may last
+};
+
+__device__ __forceinline__ void grid_barrier(const BarrierRef& br, unsigned long long target) {
     __syncthreads();
     if (threadIdx.x == 0) {
+        GridBarrier* gb = br.gb;
+        const unsigned long long epoch = br.base + 1;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_fetch_add(&gb->arrived, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long t0 = wall_clock64();
         while (__hip_atomic_load(&gb->arrived, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(&gb->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-            if (wall_clock64() - t0 > deadline) {
-                __hip_atomic_store(&gb->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_load(&gb->failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) break;
+            if (wall_clock64() - t0 > br.deadline) {
+                __hip_atomic_store(&gb->failed, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(br.fail_host, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
@@ -1256,14 +1393,13 @@ __device__ __forceinline__ void grid_barrier(GridBarrier* gb, unsigned long long
 // `base` is the arrival count before this launch.
 // One sweep of the tile described by `a`; `nbar` counts the barriers passed.
 template <class S>
-__device__ void persistent_sweep(const SweepArgs& a, GridBarrier* gb, unsigned long long base, unsigned long long& nbar,
-                                 unsigned long long deadline) {
+__device__ void persistent_sweep(const SweepArgs& a, const BarrierRef& br, unsigned long long& nbar) {
     const long tid = xcd_tid();
     const long nth = static_cast<long>(gridDim.x) * blockDim.x;
     const int lane = static_cast<int>(tid % a.G);
     const unsigned long long nb = gridDim.x;
     ph_init<S>(a, tid, nth);
-    grid_barrier(gb, base + nb * (++nbar), deadline);
+    grid_barrier(br, br.base + nb * (++nbar));
     for (int k = 1; k <= a.L; ++k) {
         if (!a.gather) {
             if (k == 1) {
@@ -1274,9 +1410,10 @@ __device__ void persistent_sweep(const SweepArgs& a, GridBarrier* gb, unsigned l
                 ph_emit<S>(a, a.emits + e0, e1 - e0, src, tid, nth, lane);
             }
         }
+        __shared__ typename S::T win[kLdsWindow];
         ph_eval_kind<S>(a, level_kind<S>(a, k), a.level_off[k - 1], a.level_off[k], level_src<S>(a, k),
-                        level_dst<S>(a, k), tid, nth);
-        grid_barrier(gb, base + nb * (++nbar), deadline);
+                        level_dst<S>(a, k), tid, nth, win, kLdsWindow);
+        grid_barrier(br, br.base + nb * (++nbar));
     }
     if (a.gather && a.xparts) {
         ph_gather_parts_any<S>(a, blockIdx.x, gridDim.x, threadIdx.x, blockDim.x);
@@ -1297,13 +1434,12 @@ __device__ void persistent_sweep(const SweepArgs& a, GridBarrier* gb, unsigned l
 // describes the first tile; the others differ only in their column base and
 // width. Barriers per launch: tiles * (L + 2) - 1.
 template <class S>
-__global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long base, unsigned long long deadline,
-                             int tiles, int tile_w, int vbytes) {
+__global__ void k_persistent(SweepArgs a, BarrierRef br, int tiles, int tile_w, int vbytes) {
     unsigned long long nbar = 0;
     for (int t = 0; t < tiles; ++t) {
         if (t > 0) {
             const unsigned long long nb = gridDim.x;
-            grid_barrier(gb, base + nb * (++nbar), deadline);
+            grid_barrier(br, br.base + nb * (++nbar));
             const int b0 = t * tile_w;
             SweepArgs at = a;
             at.X = a.X + b0;
@@ -1313,9 +1449,9 @@ __global__ void k_persistent(SweepArgs a, GridBarrier* gb, unsigned long long ba
             at.vec = at.Bt > 1 ? 1 : 0;
             at.yvec = (a.B % 4 == 0 && b0 % 4 == 0) ? 1 : 0;
             at.G = lanes_for(at.Bs, at.vec);
-            persistent_sweep<S>(at, gb, base, nbar, deadline);
+            persistent_sweep<S>(at, br, nbar);
         } else {
-            persistent_sweep<S>(a, gb, base, nbar, deadline);
+            persistent_sweep<S>(a, br, nbar);
         }
     }
 }
@@ -1344,10 +1480,13 @@ __device__ void run_tile(const SweepArgs& a, const Tile& t) {
     const long tid = threadIdx.x, nth = blockDim.x;
     switch (t.task) {
         case TT_INIT: ph_init_range<S>(a, t.begin, t.end, tid, nth); break;
-        case TT_EVAL:
+        case TT_EVAL: {
+            __shared__ typename S::T win[kLdsWindow];
             ph_eval_kind<S>(a, t.kind, static_cast<int>(t.begin), static_cast<int>(t.end), level_src<S>(a, t.level),
-                            level_dst<S>(a, t.level) + (t.begin - a.level_off[t.level - 1]) * a.Bs, tid, nth);
+                            level_dst<S>(a, t.level) + (t.begin - a.level_off[t.level - 1]) * a.Bs, tid, nth, win,
+                            kLdsWindow);
             break;
+        }
         case TT_EMIT: {
             const int lane = static_cast<int>(tid % a.G);
             if (t.level == 0) {
@@ -1364,15 +1503,14 @@ __device__ void run_tile(const SweepArgs& a, const Tile& t) {
 }
 
 template <class S>
-__global__ void k_multi(const SweepArgs* args, const Tile* tiles, const long* phase_off, int phases,
-                        GridBarrier* gb, unsigned long long base, unsigned long long deadline) {
+__global__ void k_multi(const SweepArgs* args, const Tile* tiles, const long* phase_off, int phases, BarrierRef br) {
     const unsigned long long nb = gridDim.x;
     for (int p = 0; p < phases; ++p) {
         for (long t = phase_off[p] + blockIdx.x; t < phase_off[p + 1]; t += gridDim.x) {
             const Tile tile = tiles[t];
             run_tile<S>(args[tile.g], tile);
         }
-        if (p + 1 < phases) grid_barrier(gb, base + nb * static_cast<unsigned long long>(p + 1), deadline);
+        if (p + 1 < phases) grid_barrier(br, br.base + nb * static_cast<unsigned long long>(p + 1));
     }
 }
 

@@ -33,20 +33,28 @@ VARIANTS = {"levels": VARIANT_LEVELS, "graph": VARIANT_GRAPH, "persistent": VARI
 EMITS = {"spot": EMIT_SPOT, "gather": EMIT_GATHER}
 # Per-level node-update kernels (gmr.h gmr_eval_kind).
 EVAL_GROUP, EVAL_THREAD, EVAL_WAVE, EVAL_LDS = 0, 1, 2, 3
-RUN_TIMED, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, RUN_BITS = 1, 2, 4, 8
+RUN_TIMED, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, RUN_BITS, RUN_LDS_RANGES = 1, 2, 4, 8, 16
 EVAL_KINDS = {"group": EVAL_GROUP, "thread": EVAL_THREAD, "wave": EVAL_WAVE, "lds": EVAL_LDS}
 
 TROPICAL_INF = 1000000000
 
 
+# gmr_status codes (gmr_host.h)
+ERR_INVALID, ERR_IO, ERR_NOMEM, ERR_DEVICE, ERR_INTERNAL, ERR_BARRIER = 1, 2, 3, 4, 5, 6
+
+
 class GmrError(RuntimeError):
-    pass
+    """A C-ABI call failed; .code is its gmr_status."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 def _check(rc, lib=None):
     if rc != 0:
         msg = _lib.host().gmr_last_error()
-        raise GmrError(f"gmr error {rc}: {msg.decode() if msg else ''}")
+        raise GmrError(f"gmr error {rc}: {msg.decode() if msg else ''}", rc)
 
 
 def _ptr(a, ctype):
@@ -292,8 +300,8 @@ class Engine:
 
     @staticmethod
     def _opts(variant=VARIANT_LEVELS, emit=EMIT_SPOT, b_tile=0, block=0, grid_cap=0, eval_kind=0,
-              level_kind=None, flags=0, xcd_passes=0):
-        o = RunOpts(variant, emit, b_tile, block, grid_cap, eval_kind, flags, xcd_passes, None)
+              level_kind=None, flags=0, xcd_passes=0, lds_values=0, chunks=0):
+        o = RunOpts(variant, emit, b_tile, block, grid_cap, eval_kind, flags, xcd_passes, None, lds_values, chunks)
         keep = None
         if level_kind is not None:
             keep = np.ascontiguousarray(level_kind, dtype=np.int32)
@@ -336,7 +344,11 @@ class Engine:
         err = C.c_int()
         _check(_lib.dev().gmr_engine_check(self._h, C.byref(err)))
         if err.value:
-            raise GmrError("persistent sweep: a grid-barrier wait timed out; results are invalid")
+            raise GmrError("persistent sweep: a grid-barrier wait timed out; results are invalid", ERR_BARRIER)
+
+    def set_barrier_timeout(self, microseconds):
+        """Limit of one grid-barrier wait of the persistent variant (0: 2 s)."""
+        _check(_lib.dev().gmr_engine_set_barrier_timeout(self._h, int(microseconds)))
 
     def set_gather_order(self, order):
         """0: row order; 1: segments sharing their highest rule together."""
@@ -459,7 +471,11 @@ class Batch:
         err = C.c_int()
         _check(_lib.dev().gmr_batch_check(self._h, C.byref(err)))
         if err.value:
-            raise GmrError("batched persistent sweep: a grid-barrier wait timed out; results are invalid")
+            raise GmrError("batched persistent sweep: a grid-barrier wait timed out; results are invalid",
+                           ERR_BARRIER)
+
+    def set_barrier_timeout(self, microseconds):
+        _check(_lib.dev().gmr_batch_set_barrier_timeout(self._h, int(microseconds)))
 
     def __del__(self):
         lib = getattr(_lib, "_dev", None) if _lib is not None else None

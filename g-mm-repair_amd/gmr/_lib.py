@@ -41,7 +41,7 @@ class RunOpts(C.Structure):
     _fields_ = [
         ("variant", C.c_int), ("emit", C.c_int), ("b_tile", C.c_int), ("block", C.c_int),
         ("grid_cap", C.c_int), ("eval_kind", C.c_int), ("flags", C.c_int), ("xcd_passes", C.c_int),
-        ("level_kind", c_int_p),
+        ("level_kind", c_int_p), ("lds_values", C.c_int), ("chunks", C.c_int),
     ]
 
 
@@ -110,6 +110,7 @@ def dev():
     _sig(L, "gmr_engine_run", C.c_int, [vp, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(RunOpts), C.c_void_p])
     _sig(L, "gmr_engine_last_kernel_ms", C.c_int, [vp, c_double_p])
     _sig(L, "gmr_engine_check", C.c_int, [vp, c_int_p])
+    _sig(L, "gmr_engine_set_barrier_timeout", C.c_int, [vp, C.c_longlong])
     _sig(L, "gmr_engine_debug", C.c_int, [vp, c_ll_p])
     _sig(L, "gmr_engine_set_gather_order", C.c_int, [vp, C.c_int])
     _sig(L, "gmr_engine_traverse", C.c_int, [vp, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(RunOpts), C.c_void_p])
@@ -120,6 +121,7 @@ def dev():
     _sig(L, "gmr_batch_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int, vpp])
     _sig(L, "gmr_batch_run", C.c_int, [vp, C.c_int, C.POINTER(C.c_void_p), c_int_p, C.POINTER(C.c_void_p), C.c_int, C.POINTER(RunOpts), C.c_void_p])
     _sig(L, "gmr_batch_check", C.c_int, [vp, c_int_p])
+    _sig(L, "gmr_batch_set_barrier_timeout", C.c_int, [vp, C.c_longlong])
     _sig(L, "gmr_batch_destroy", None, [vp])
     _sig(L, "gmr_mlp_create", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, vpp])
     _sig(L, "gmr_mlp_num_params", C.c_longlong, [vp])

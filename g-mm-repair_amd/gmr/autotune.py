@@ -11,8 +11,8 @@ import hashlib
 import json
 import os
 
-from . import (BOOLEAN, EMIT_GATHER, EMIT_SPOT, RESIDENT, RUN_BITS, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, VARIANT_GRAPH,
-               VARIANT_LEVELS, VARIANT_PERSISTENT)
+from . import (BOOLEAN, EMIT_GATHER, EMIT_SPOT, EVAL_LDS, RESIDENT, RUN_BITS, RUN_LDS_RANGES, RUN_ROW_SEGMENTS,
+               RUN_SINGLE_LIST, VARIANT_GRAPH, VARIANT_LEVELS, VARIANT_PERSISTENT, GmrError)
 
 VARIANT_NAMES = {VARIANT_LEVELS: "levels", VARIANT_GRAPH: "graph", VARIANT_PERSISTENT: "persistent"}
 EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
@@ -25,9 +25,13 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
     several vectors; x XCD-partitioned (1, 2, 4 or automatic passes),
     single-list or row-segment gather for one vector; x XCD-partitioned or
     row-segment gather for tiles of 2-4 vectors; the Boolean semiring on one
-    vector adds the bit mode to the levels and graph variants).
-    Persistent shapes are (block, grid cap); a cap of 0 means the engine's
-    default of 16 waves per CU."""
+    vector adds the bit mode to the levels and graph variants), and the
+    level-chunk sizes: for one vector, the chunks per wave of the
+    XCD-partitioned gather, the LDS-range gather (levels and graph variants)
+    at several range sizes and chunk counts, the bit mode at several range
+    sizes, and the LDS-staged node update (GMR_EVAL_LDS) at several window
+    sizes. Persistent shapes are (block, grid cap); a cap of 0 means the
+    engine's default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
     tiles = sorted({B} | {t for t in (16, 32, 64, 128) if t < B})
     out = []
@@ -42,22 +46,43 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
                         narrow = e == EMIT_GATHER and 2 <= t <= 4 and layout == RESIDENT
                         if e == EMIT_GATHER and B == 1:
                             flags = (0, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST)
-                            if semiring == BOOLEAN and v != VARIANT_PERSISTENT and layout == RESIDENT:
-                                flags += (RUN_BITS,)
+                            if v != VARIANT_PERSISTENT and layout == RESIDENT:
+                                flags += (RUN_LDS_RANGES,)
+                                if semiring == BOOLEAN:
+                                    flags += (RUN_BITS,)
                         elif narrow:
                             flags = (0, RUN_ROW_SEGMENTS) if order == 0 else (RUN_ROW_SEGMENTS,)
                         else:
                             flags = (0,)
                         for fl in flags:
-                            ps = (0, 1, 2, 4) if (e == EMIT_GATHER and (B == 1 or narrow) and fl == 0) else (None,)
-                            for passes in ps:
-                                c = dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap,
-                                         gather_order=order)
-                                if fl:
-                                    c["flags"] = fl
-                                if passes is not None:
-                                    c["xcd_passes"] = passes
-                                out.append(c)
+                            base = dict(variant=v, emit=e, b_tile=0 if t == B else t, block=blk, grid_cap=cap,
+                                        gather_order=order)
+                            if fl:
+                                base["flags"] = fl
+                            if e == EMIT_GATHER and (B == 1 or narrow) and fl == 0:
+                                # XCD-partitioned gather: passes x chunks per wave (one vector)
+                                for passes in (0, 1, 2, 4):
+                                    for ch in ((2, 4) if B == 1 else (0,)):
+                                        c = dict(base, xcd_passes=passes)
+                                        if ch:
+                                            c["chunks"] = ch
+                                        out.append(c)
+                            elif fl == RUN_LDS_RANGES:
+                                for span in (16384, 32768, 40960):
+                                    for ch in (2, 4):
+                                        out.append(dict(base, lds_values=span, chunks=ch))
+                            elif fl == RUN_BITS:
+                                for rng in (262144, 524288, 1048576):
+                                    out.append(dict(base, lds_values=rng))
+                            else:
+                                out.append(base)
+    if B == 1:
+        # the LDS-staged node update at several windows, with the default emission forms
+        for v in (VARIANT_LEVELS, VARIANT_GRAPH):
+            for e in emits:
+                for win in (2048, 4096, 16384):
+                    out.append(dict(variant=v, emit=e, b_tile=0, block=blocks[0], grid_cap=0, gather_order=0,
+                                    eval_kind=EVAL_LDS, lds_values=win))
     return out
 
 
@@ -72,12 +97,17 @@ def apply(engine, cand):
 
 def describe(c):
     cap = c.get("grid_cap", 0)
+    fl = c.get("flags", 0)
+    lds = c.get("lds_values", 0)
     return (f"{VARIANT_NAMES[c['variant']]}/{EMIT_NAMES[c['emit']]}/tile={c['b_tile'] or 'B'}/blk={c['block']}"
             + (f"/grid<={cap}" if cap else "") + ("/order=rule" if c.get("gather_order") == 1 else "")
-            + ("/segments" if c.get("flags", 0) & RUN_ROW_SEGMENTS else "")
-            + ("/single-list" if c.get("flags", 0) & RUN_SINGLE_LIST else "")
-            + ("/bits" if c.get("flags", 0) & RUN_BITS else "")
-            + (f"/xcd-passes={c['xcd_passes'] or 'auto'}" if "xcd_passes" in c else ""))
+            + ("/segments" if fl & RUN_ROW_SEGMENTS else "")
+            + ("/single-list" if fl & RUN_SINGLE_LIST else "")
+            + (f"/bits(range={lds or 'default'})" if fl & RUN_BITS else "")
+            + (f"/lds-ranges(values={lds or 'default'})" if fl & RUN_LDS_RANGES else "")
+            + (f"/xcd-passes={c['xcd_passes'] or 'auto'}" if "xcd_passes" in c else "")
+            + (f"/chunks={c['chunks']}" if c.get("chunks") else "")
+            + (f"/eval=lds(window={lds or 'default'})" if c.get("eval_kind") == EVAL_LDS else ""))
 
 
 def shape_key(info, B, semiring):
@@ -125,9 +155,16 @@ def tune(engine, semiring, X, Y, cands, cache_path=None, reps=3, log=None):
             log(f"  tune: {len(results)} timings from {cache_path}")
     if not results:
         for c in cands:
-            ms = time_candidate(engine, semiring, X, Y, c, reps=reps)
+            try:
+                ms = time_candidate(engine, semiring, X, Y, c, reps=reps)
+            except GmrError as e:
+                # a schedule this engine or device refuses (an LDS range larger than a
+                # workgroup may hold, a persistent block without residency headroom)
+                ms = float("inf")
+                if log:
+                    log(f"  tune {describe(c):40s} refused: {e}")
             results.append((c, ms))
-            if log:
+            if log and ms != float("inf"):
                 log(f"  tune {describe(c):40s} {ms * 1e3:9.1f} us")
     best = min(results, key=lambda r: r[1])[0]
     if cache_path:

@@ -34,7 +34,7 @@ enum gmr_semiring { GMR_PLUS_TIMES = 0, GMR_BOOLEAN = 1, GMR_TROPICAL = 2 };
 enum gmr_variant {
     GMR_VARIANT_LEVELS = 0,     /* one launch per phase and level (the plain sweep) */
     GMR_VARIANT_GRAPH = 1,      /* the same launches replayed from a captured hipGraph */
-    GMR_VARIANT_PERSISTENT = 2  /* one cooperative launch, grid barrier between levels */
+    GMR_VARIANT_PERSISTENT = 2  /* one launch, the engine's own grid barrier between levels */
 };
 
 /* How top-sequence symbols reach y. */
@@ -63,7 +63,12 @@ enum gmr_run_flags {
     /* Boolean semiring, one vector, gather emission, levels or graph variant: every terminal
      * and rule value is one bit of a bitset ((U + rules) / 8 bytes, small enough to stay in
      * L2), node updates and the gather read bits, and y gets 0/1 as int32. */
-    GMR_RUN_BITS = 8
+    GMR_RUN_BITS = 8,
+    /* One vector, gather emission, levels or graph variant: the top-sequence symbols are
+     * regrouped by value into ranges of lds_values values, one list per range; a workgroup
+     * copies a range of the value array into LDS once and looks its symbols' values up
+     * there, and every row piece combines atomically into a zeroed y. */
+    GMR_RUN_LDS_RANGES = 16
 };
 
 typedef struct gmr_run_opts {
@@ -78,6 +83,17 @@ typedef struct gmr_run_opts {
                                many ranges, swept in this many launches so each XCD's L2 holds the
                                values of one range at a time (0: sized from the value count) */
     const int* level_kind;  /* optional host array, max_depth + 1 entries ([0] unused) */
+    /* Level-chunk sizes (0: the defaults below; the autotuner searches them):
+     * lds_values  values a workgroup stages in LDS, a multiple of 1024: with GMR_RUN_LDS_RANGES
+     *             the values per range (default 32768; at least 4096, at most what one
+     *             workgroup's LDS holds); with GMR_RUN_BITS the bits per range (default 524288,
+     *             131072..1048576); otherwise the left-child window of GMR_EVAL_LDS levels in the
+     *             levels and graph variants (default 4096, 1024..16384)
+     * chunks      64-symbol chunks a wave of a chunked single-vector gather has in flight:
+     *             the XCD-partitioned gather 1, 2 or 4 (default 2), GMR_RUN_LDS_RANGES and
+     *             GMR_RUN_BITS 2, 4 or 8 (default 4) */
+    int lds_values;
+    int chunks;
 } gmr_run_opts;
 
 typedef struct gmr_engine gmr_engine;
@@ -122,12 +138,28 @@ int gmr_engine_run_left(gmr_engine* e, int semiring, const void* d_X, int B, voi
  * (reference cugraph_bench.py:53-61). */
 int gmr_engine_traverse(gmr_engine* e, int semiring, void* d_x, int max_iters, int* iters_out,
                         const gmr_run_opts* opts, void* stream);
-/* Persistent variant health check: *barrier_error = 1 if a grid-barrier wait
- * of an earlier persistent launch on this engine timed out (its results are
- * invalid; the barrier has been reset), else 0. Synchronises the device. */
+/* The persistent variant. Its grid barrier needs every workgroup of the launch
+ * resident at once; no cooperative launch guarantees it, so the grid is sized
+ * from the occupancy query with one workgroup per CU of headroom (the query
+ * has reported one more than the dispatcher keeps resident): min(occupancy - 1,
+ * 1024 / block) workgroups per CU, times the CU count, capped by grid_cap. A
+ * block size at which fewer than two workgroups fit per CU is refused
+ * (GMR_ERR_INVALID). Every barrier wait is bounded (2 s by default); a wait that
+ * times out marks that launch failed, and the kernel still ends. A failure is
+ * reported as GMR_ERR_BARRIER by the next gmr_engine_run on the engine (which
+ * then does not run), by the same call when it was timed (GMR_RUN_TIMED
+ * synchronises), and by gmr_engine_check. Later launches are not affected.
+ *
+ * Health check: *barrier_error = 1 if a grid-barrier wait of an earlier
+ * persistent launch on this engine timed out since the last report (its
+ * results are invalid; the barrier has been reset), else 0. Synchronises the
+ * device. */
 int gmr_engine_check(gmr_engine* e, int* barrier_error);
+/* Limit of one grid-barrier wait in microseconds (0: the default of 2 s); for
+ * tests of the failure path, which set a tiny limit so the waits give up. */
+int gmr_engine_set_barrier_timeout(gmr_engine* e, long long microseconds);
 /* Diagnostics of the persistent variant: out[0..6] = barrier arrivals on the
- * device, barrier error word, arrivals the host expects, wall-clock rate (kHz),
+ * device, unreported barrier failure (0/1), arrivals the host expects, wall-clock rate (kHz),
  * CU count, workgroups of the last persistent launch, workgroups per CU,
  * captured graphs held. Synchronises the device. */
 int gmr_engine_debug(gmr_engine* e, long long out[8]);
@@ -158,8 +190,10 @@ int gmr_batch_create(gmr_engine* const* engines, int n, gmr_batch** out);
  * kinds per grammar; block and grid_cap of entry 0 apply to the launch) or NULL. */
 int gmr_batch_run(gmr_batch* b, int semiring, const void* const* d_X, const int* B, void* const* d_Y,
                   int variant, const gmr_run_opts* opts, void* stream);
-/* As gmr_engine_check, for the batch's persistent launches. */
+/* As gmr_engine_check and gmr_engine_set_barrier_timeout, for the batch's
+ * persistent launches (a failure is also reported by the next gmr_batch_run). */
 int gmr_batch_check(gmr_batch* b, int* barrier_error);
+int gmr_batch_set_barrier_timeout(gmr_batch* b, long long microseconds);
 void gmr_batch_destroy(gmr_batch* b);
 
 /* ---- the schedule proposer's network, trained on the GPU by its own kernels ----

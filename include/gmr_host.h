@@ -36,7 +36,8 @@ enum gmr_status {
     GMR_ERR_IO = 2,       /* missing, truncated or unwritable file */
     GMR_ERR_NOMEM = 3,    /* host or device allocation failed */
     GMR_ERR_DEVICE = 4,   /* HIP runtime error */
-    GMR_ERR_INTERNAL = 5
+    GMR_ERR_INTERNAL = 5,
+    GMR_ERR_BARRIER = 6   /* a persistent launch's grid barrier timed out: the Y it wrote is invalid */
 };
 
 enum gmr_layout { GMR_LAYOUT_REFCOMPAT = 0, GMR_LAYOUT_WAVE64 = 1, GMR_LAYOUT_RESIDENT = 2 };

@@ -96,8 +96,31 @@ def test_bit_mode_candidates_only_for_boolean_single_vectors():
     """The Boolean bit mode is searched for one vector with gather emission in
     the levels and graph variants, and nowhere else."""
     d = {autotune.describe(c) for c in autotune.candidates(1, gmr.RESIDENT, semiring=gmr.BOOLEAN)}
-    assert {"levels/gather/tile=B/blk=256/bits", "graph/gather/tile=B/blk=256/bits"} <= {x for x in d if "bits" in x}
-    assert len([x for x in d if "bits" in x]) == 2
+    bits = {x for x in d if "bits" in x}
+    for v in ("levels", "graph"):
+        for r in (262144, 524288, 1048576):
+            assert f"{v}/gather/tile=B/blk=256/bits(range={r})" in bits
+    assert len(bits) == 6
     for B, sr, lay in ((1, gmr.PLUS_TIMES, gmr.RESIDENT), (1, gmr.TROPICAL, gmr.RESIDENT), (4, gmr.BOOLEAN, gmr.RESIDENT),
                        (1, gmr.BOOLEAN, gmr.WAVE64), (1, None, gmr.RESIDENT)):
         assert not any(c.get("flags", 0) & gmr.RUN_BITS for c in autotune.candidates(B, lay, semiring=sr))
+
+
+def test_candidates_search_the_level_chunk_sizes():
+    """One vector: the XCD-partitioned gather's chunks per wave, the LDS-range
+    gather's range sizes and chunks (levels and graph, every semiring), and
+    the LDS-staged node update's windows; several vectors: none of these."""
+    for sr in (gmr.PLUS_TIMES, gmr.BOOLEAN, gmr.TROPICAL):
+        d = {autotune.describe(c) for c in autotune.candidates(1, gmr.RESIDENT, semiring=sr)}
+        for v in ("levels", "graph"):
+            for span in (16384, 32768, 40960):
+                for ch in (2, 4):
+                    assert f"{v}/gather/tile=B/blk=256/lds-ranges(values={span})/chunks={ch}" in d
+            for win in (2048, 4096, 16384):
+                assert f"{v}/spot/tile=B/blk=256/eval=lds(window={win})" in d
+        for ch in (2, 4):
+            assert f"levels/gather/tile=B/blk=256/xcd-passes=2/chunks={ch}" in d
+        assert not any("lds-ranges" in x for x in d if x.startswith("persistent"))
+    for B in (4, 64):
+        d = {autotune.describe(c) for c in autotune.candidates(B, gmr.RESIDENT)}
+        assert not any("lds" in x or "chunks" in x for x in d)

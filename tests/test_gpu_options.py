@@ -1,0 +1,247 @@
+"""GPU tests of the run options that change how, not what, a sweep computes:
+the level-chunk sizes (LDS window of the EV_LDS node update, LDS range and
+chunks per wave of the single-vector gathers), the LDS-range gather of 4-byte
+values, the Tropical cap of whole-row stores, and the reporting of a
+persistent launch whose grid barrier timed out. Every result is checked
+against the oracle (FP64 brute force from the uncompressed matrix): Boolean
+and Tropical bit-exact, (+,x) max_i |y_i - y*_i| / sum_j |M_ij x_j| <= 1e-5."""
+import sys
+
+import numpy as np
+import pytest
+
+import gmr
+from gmr import _lib
+from helpers import rel_err, x_for
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+torch = pytest.importorskip("torch")
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    _lib.dev()  # the HIP library must load; no fallback exists
+
+
+def run(eng, sr, X, **kw):
+    Xt = torch.from_numpy(np.ascontiguousarray(X, np.float32)).cuda()
+    Y = eng.run_torch(sr, Xt, **kw)
+    torch.cuda.synchronize()
+    if kw.get("variant") == gmr.VARIANT_PERSISTENT:
+        eng.check()
+    return Y.cpu().numpy()
+
+
+def check(sr, y, y_star, den, what):
+    assert np.shape(y) == np.shape(y_star), what
+    if sr == 0:
+        e = rel_err(y, y_star, den)
+        assert e <= TOL, f"{what}: FP32 error {e} > {TOL}"
+    else:
+        diff = int((np.asarray(y, np.int64) != np.asarray(y_star, np.int64)).sum())
+        assert diff == 0, f"{what}: {diff} rows differ"
+
+
+@pytest.fixture(scope="module")
+def hap():
+    m = gmr.Matrix.haplotype(20261204, 4096, 2048)
+    g = m.repair()
+    seq, V = m.arrays()
+    return g.schedule(4096, 2048, "resident").engine(), seq, V
+
+
+@pytest.fixture(scope="module")
+def geno():
+    # enough values (rules + terminals) for dozens of LDS ranges
+    m = gmr.Matrix.genotype(20261206, 6000, 10000)
+    g = m.repair()
+    seq, V = m.arrays()
+    return g.schedule(6000, 10000, "resident").engine(), seq, V
+
+
+@pytest.mark.parametrize("sr", [0, 1, 2])
+def test_lds_range_gather(hap, geno, sr):
+    """GMR_RUN_LDS_RANGES at every range size and chunk count, levels and graph."""
+    for name, (eng, seq, V), rows, cols in (("hap", hap, 4096, 2048), ("geno", geno, 6000, 10000)):
+        x = x_for(sr, cols, 300 + sr)
+        y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
+        for span in (4096, 8192, 16384, 32768):
+            for ch in (2, 4, 8):
+                for var in (gmr.VARIANT_LEVELS, gmr.VARIANT_GRAPH):
+                    _say(name, "lds ranges", sr, span, ch, var)
+                    y = run(eng, sr, x, variant=var, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES,
+                            lds_values=span, chunks=ch)
+                    check(sr, y, y_star, den, f"{name} lds span={span} chunks={ch} var={var}")
+        # default range and chunks
+        y = run(eng, sr, x, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES)
+        check(sr, y, y_star, den, f"{name} lds default")
+        # 160 KiB ranges: only where one workgroup may hold that much LDS
+        try:
+            y = run(eng, sr, x, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=40960, chunks=2)
+        except gmr.GmrError as e:
+            assert e.code == gmr.ERR_INVALID and "fit the LDS" in str(e), str(e)
+        else:
+            check(sr, y, y_star, den, f"{name} lds span=40960")
+
+
+def test_level_chunk_options(hap, geno):
+    """EV_LDS windows, XCD-partitioned gather chunks, bit-mode ranges and chunks."""
+    for name, (eng, seq, V), rows, cols in (("hap", hap, 4096, 2048), ("geno", geno, 6000, 10000)):
+        for sr in (0, 2):
+            x = x_for(sr, cols, 400 + sr)
+            y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
+            for win in (1024, 2048, 8192, 16384):
+                for em in (gmr.EMIT_SPOT, gmr.EMIT_GATHER):
+                    _say(name, "EV_LDS window", sr, win, em)
+                    y = run(eng, sr, x, emit=em, eval_kind=gmr.EVAL_LDS, lds_values=win)
+                    check(sr, y, y_star, den, f"{name} window={win} emit={em}")
+            for ch in (1, 2, 4):
+                for passes in (1, 2, 4):
+                    _say(name, "xcd chunks", sr, ch, passes)
+                    y = run(eng, sr, x, emit=gmr.EMIT_GATHER, chunks=ch, xcd_passes=passes)
+                    check(sr, y, y_star, den, f"{name} xcd chunks={ch} passes={passes}")
+        xb = x_for(1, cols, 500)
+        yb_star, _ = O.brute_force(1, rows, cols, seq, V, xb)
+        for rng in (131072, 262144, 524288, 1048576):
+            for ch in (2, 4, 8):
+                _say(name, "bits", rng, ch)
+                y = run(eng, 1, xb, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS, lds_values=rng, chunks=ch)
+                check(1, y, yb_star, None, f"{name} bits range={rng} chunks={ch}")
+
+
+def test_option_errors(hap):
+    eng, seq, V = hap
+    x = torch.zeros(2048, device="cuda")
+    bad = [dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=3000),          # not a multiple of 1024
+           dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=1 << 20),       # too large
+           dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, chunks=3),
+           dict(emit=gmr.EMIT_SPOT, flags=gmr.RUN_LDS_RANGES),                              # needs gather
+           dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, variant=gmr.VARIANT_PERSISTENT),
+           dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_BITS),
+           dict(emit=gmr.EMIT_GATHER, chunks=8),                                            # XCD gather: 1, 2, 4
+           dict(eval_kind=gmr.EVAL_LDS, lds_values=65536)]                                  # window too large
+    for kw in bad:
+        with pytest.raises(gmr.GmrError) as e:
+            eng.run_torch(0, x, **kw)
+        assert e.value.code == gmr.ERR_INVALID, kw
+    X4 = torch.zeros(2048, 4, device="cuda")
+    with pytest.raises(gmr.GmrError):
+        eng.run_torch(0, X4, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES)  # one vector only
+    torch.cuda.synchronize()
+
+
+def test_tropical_large_finite_values_cap_at_infinity():
+    """Finite leaf sums above 1e9 (large weights, large distances): the
+    reference's y starts at 1e9 and is min-combined, so such a row reads 1e9.
+    Every emission form stores that, the whole-row stores included."""
+    rng = np.random.default_rng(20261207)
+    rows, cols = 600, 300
+    M = np.where(rng.random((rows, cols)) < 0.05, rng.choice([5e8, 6e8, 7e8, 3.0], (rows, cols)), 0.0)
+    M[:40] = 0.0
+    M[:40, :3] = 6e8  # rows whose every leaf sum exceeds 1e9 below
+    m = gmr.Matrix.from_dense(M)
+    g = m.repair()
+    seq, V = m.arrays()
+    for B in (1, 5):
+        X = rng.uniform(3e8, 9e8, (cols, B)).astype(np.float32)
+        X[:3] = 6e8
+        X[rng.random((cols, B)) < 0.05] = 1e9
+        Xs = X[:, 0] if B == 1 else X
+        y_star = O.brute_force_batched(2, rows, cols, seq, V, X)[0]
+        y_star = y_star[:, 0] if B == 1 else y_star
+        assert (y_star[:40] == 1e9).all()
+        for gg in (g, g.inline_top(2)):
+            eng = gg.schedule(rows, cols, "resident").engine()
+            for var in ("levels", "graph", "persistent"):
+                for em in ("spot", "gather"):
+                    flags = (0, gmr.RUN_ROW_SEGMENTS, gmr.RUN_SINGLE_LIST, gmr.RUN_LDS_RANGES) \
+                        if (em == "gather" and B == 1 and var != "persistent") else (0,)
+                    for fl in flags:
+                        _say("tropical cap", B, var, em, fl)
+                        y = run(eng, 2, Xs, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em], flags=fl)
+                        check(2, y, y_star, None, f"tropical cap B={B} {var}/{em}/flags={fl}")
+    # the left product's fold stores whole columns the same way
+    eng = g.schedule(rows, cols, "resident").engine()
+    xr = rng.uniform(3e8, 9e8, rows).astype(np.float32)
+    yl_star = O.brute_force_left(2, rows, cols, seq, V, xr[:, None])[0][:, 0]
+    yl = eng.run_left_torch(2, torch.from_numpy(xr).cuda()).cpu().numpy()
+    check(2, yl, yl_star, None, "tropical cap, left product")
+
+
+def _fail_then_recover(run_once, check_rc, what):
+    """Run with a 1 us barrier limit until a failure is reported (a wait of
+    some workgroup outlives 1 us essentially always)."""
+    for attempt in range(3):
+        run_once()
+        torch.cuda.synchronize()  # the failed launch has ended; its host word is set
+        try:
+            check_rc()
+        except gmr.GmrError as e:
+            assert e.code == gmr.ERR_BARRIER, f"{what}: code {e.code}"
+            return
+    pytest.fail(f"{what}: no barrier timeout was reported with a 1 us limit")
+
+
+def test_persistent_barrier_timeout_is_reported(hap):
+    """A grid-barrier wait that gives up (forced from the host with a 1 us
+    limit, so every kernel still ends normally) is returned as GMR_ERR_BARRIER
+    by the next run, which does not run; later runs are correct again."""
+    eng, seq, V = hap
+    x = x_for(0, 2048, 600)
+    y_star, den = O.brute_force(0, 4096, 2048, seq, V, x)
+    xt = torch.from_numpy(x).cuda()
+    kw = dict(variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_GATHER)
+    eng.set_barrier_timeout(1)
+    _fail_then_recover(lambda: eng.run_torch(0, xt, **kw), lambda: eng.run_torch(0, xt, **kw), "next run")
+    # a timed run reports its own failure
+    _fail_then_recover(lambda: None, lambda: eng.run_torch(0, xt, flags=gmr.RUN_TIMED, **kw), "timed run")
+    # gmr_engine_check reports it too
+    _fail_then_recover(lambda: eng.run_torch(0, xt, **kw), eng.check, "check")
+    eng.set_barrier_timeout(0)
+    eng.check()
+    check(0, run(eng, 0, x, **kw), y_star, den, "after recovery")
+    for _ in range(3):  # and stays correct
+        check(0, run(eng, 0, x, **kw), y_star, den, "after recovery, again")
+
+
+def test_batch_barrier_timeout_is_reported(hap, geno):
+    (e1, s1, V1), (e2, s2, V2) = hap, geno
+    b = gmr.Batch([e1, e2])
+    X1 = torch.from_numpy(x_for(0, 2048, 700, B=3)).cuda()
+    X2 = torch.from_numpy(x_for(0, 10000, 701, B=2)).cuda()
+    b.set_barrier_timeout(1)
+    go = lambda: b.run_torch(0, [X1, X2], variant=gmr.VARIANT_PERSISTENT)  # noqa: E731
+    _fail_then_recover(go, go, "batch next run")
+    b.set_barrier_timeout(0)
+    b.check()
+    Ys = b.run_torch(0, [X1, X2], variant=gmr.VARIANT_PERSISTENT)
+    torch.cuda.synchronize()
+    b.check()
+    for Y, X, seq, V, rows, cols in ((Ys[0], X1, s1, V1, 4096, 2048), (Ys[1], X2, s2, V2, 6000, 10000)):
+        Y_star, den = O.brute_force_batched(0, rows, cols, seq, V, X.cpu().numpy())
+        check(0, Y.cpu().numpy(), Y_star, den, "batch after recovery")
+
+
+def test_persistent_block_without_headroom_is_refused_or_correct(hap):
+    """block = 1024: runs when at least two workgroups fit per CU (one stays
+    free as headroom), otherwise is refused with GMR_ERR_INVALID."""
+    eng, seq, V = hap
+    x = x_for(0, 2048, 800)
+    y_star, den = O.brute_force(0, 4096, 2048, seq, V, x)
+    try:
+        y = run(eng, 0, x, variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_GATHER, block=1024)
+    except gmr.GmrError as e:
+        assert e.code == gmr.ERR_INVALID and "headroom" in str(e)
+        return
+    check(0, y, y_star, den, "persistent block=1024")
+    assert eng.debug()["per_cu"] >= 1
