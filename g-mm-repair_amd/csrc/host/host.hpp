@@ -8,6 +8,8 @@
 
 #include "grammar.h"
 
+struct BuiltSchedule;
+
 namespace gmr {
 
 // A matrix as its CSRV string: for every row, the terminals of its non-zeros in
@@ -74,5 +76,13 @@ grammar::Grammar inline_top_rules(const grammar::Grammar& g, int max_emits, long
 void write_grammar_files(const std::string& base, const grammar::Grammar& g,
                          const std::vector<double>& V);
 grammar::Grammar read_grammar_files(const std::string& base, std::vector<double>* V);
+
+// CPU layered sweep (FP32 for semiring 0, int32 for 1 Boolean and 2 min/plus):
+// y = M x over any layout of a built schedule; val_idx / col_idx decode compact
+// terminal q >= 1 (value V[val_idx[q-1]] in column col_idx[q-1]). threads > 0
+// sets the OpenMP thread count for the call. Throws std::invalid_argument on
+// an unknown semiring.
+void cpu_layered_sweep(int semiring, const BuiltSchedule& bs, const int* val_idx, const int* col_idx, const float* V,
+                       const float* x, void* y, int threads);
 
 }  // namespace gmr

@@ -319,6 +319,23 @@ int gmr_schedule_copy(const gmr_schedule* s, LayeredRule* rules, int* level_offs
     return GMR_OK;
 }
 
+int gmr_schedule_cpu_sweep(const gmr_schedule* s, int semiring, const float* V, int nvals, const float* x, void* y,
+                           int threads) {
+    if (!s || !V || !x || !y) return fail(GMR_ERR_INVALID, "NULL argument");
+    return guarded([&] {
+        const BuiltSchedule& b = s->s;
+        const int U = b.alpha;
+        std::vector<int> vi(U > 0 ? U - 1 : 0), ci(vi.size());
+        for (int q = 1; q < U; ++q) {
+            const int t = b.term_orig[q];
+            vi[q - 1] = (t - 1) / b.cols;
+            ci[q - 1] = (t - 1) % b.cols;
+            if (vi[q - 1] >= nvals) throw std::invalid_argument("a terminal's value index is beyond nvals");
+        }
+        gmr::cpu_layered_sweep(semiring, b, vi.data(), ci.data(), V, x, y, threads);
+    });
+}
+
 int gmr_schedule_view(gmr_schedule* s, GPUSchedule* out) {
     if (!s || !out) return fail(GMR_ERR_INVALID, "NULL argument");
     *out = s->s.gpu_schedule();

@@ -269,6 +269,21 @@ class Schedule:
                     level_offsets=lo, level_widths=lw, emit_offsets=eo, emit_pos=ep, emit_row=er,
                     term_sym=ts, term_row=tr, term_orig=to, info=i)
 
+    def cpu_sweep(self, semiring, x, V=None, threads=0):
+        """y = M x on the host CPU over this schedule (the host library's own
+        OpenMP level sweep; FP32 for (+,x), int32 otherwise). Not the GPU path:
+        it is what the driver checks the device result against."""
+        if isinstance(semiring, str):
+            semiring = SEMIRINGS[semiring]
+        V = self.V if V is None else V
+        Vf = np.ascontiguousarray(V, dtype=np.float32)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        rows = self.info()["rows"]
+        y = np.empty(rows, np.float32 if semiring == PLUS_TIMES else np.int32)
+        _check(_lib.host().gmr_schedule_cpu_sweep(self._h, semiring, _ptr(Vf, C.c_float), len(Vf),
+                                                  _ptr(x, C.c_float), C.c_void_p(y.ctypes.data), threads))
+        return y
+
     def engine(self, V=None):
         """Upload to the GPU (device-resident engine)."""
         V = self.V if V is None else V

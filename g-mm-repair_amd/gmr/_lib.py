@@ -89,6 +89,7 @@ def host():
     _sig(L, "gmr_schedule_info", C.c_int, [vp, c_ll_p])
     _sig(L, "gmr_schedule_copy", C.c_int, [vp, C.POINTER(LayeredRule), c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p])
     _sig(L, "gmr_schedule_view", C.c_int, [vp, C.POINTER(GPUSchedule)])
+    _sig(L, "gmr_schedule_cpu_sweep", C.c_int, [vp, C.c_int, c_float_p, C.c_int, c_float_p, C.c_void_p, C.c_int])
     _sig(L, "gmr_schedule_free", None, [vp])
     _host = L
     return L

@@ -105,8 +105,20 @@ def energy_between_updates(batch, seconds=1.5, device=0):
     """Energy of work between two updates of the counter at least `seconds`
     apart (it advances about once a second): `batch()` runs some products,
     synchronises and returns how many it ran. Returns (millijoules per
-    product, mean watts) or None when the counter cannot be read or does not
-    advance within 2 x seconds + 5 s."""
+    product, mean watts) or None when the counter cannot be read, does not
+    advance within 2 x seconds + 5 s, or jumps backwards in three attempts
+    (a backwards jump of energy and timestamp has been seen between reads)."""
+    for _ in range(3):
+        r = _energy_interval(batch, seconds, device)
+        if r is None:
+            return None
+        uj, secs, n = r
+        if uj >= 0 and secs > 0:
+            return uj * 1e-3 / n, uj * 1e-6 / secs
+    return None
+
+
+def _energy_interval(batch, seconds, device):
     import time
     r = energy_read(device)
     if r is None:
@@ -126,7 +138,6 @@ def energy_between_updates(batch, seconds=1.5, device=0):
         r1 = energy_read(device)
         if r1 is None or time.time() - t0 > 2 * seconds + 5:
             return None
-        if r1[1] != r0[1] and (r1[1] - r0[1]) * 1e-9 >= seconds:
+        if r1[1] != r0[1] and (r1[1] < r0[1] or (r1[1] - r0[1]) * 1e-9 >= seconds):
             break
-    uj, secs = r1[0] - r0[0], (r1[1] - r0[1]) * 1e-9
-    return uj * 1e-3 / n, uj * 1e-6 / secs
+    return r1[0] - r0[0], (r1[1] - r0[1]) * 1e-9, n

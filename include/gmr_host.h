@@ -105,6 +105,15 @@ int gmr_schedule_copy(const gmr_schedule* s, LayeredRule* rules, int* level_offs
                       int* term_sym, int* term_row, int* term_orig);
 /* Borrowed view (valid while s lives). */
 int gmr_schedule_view(gmr_schedule* s, GPUSchedule* out);
+/* y = M x on the host CPU (OpenMP; threads 0 = the OpenMP default) over the
+ * schedule's own layout, with the values the GPU path uses: float y for
+ * semiring 0 (+,x), int32 y for 1 (Boolean) and 2 (min/plus, 1e9 = +inf). x has
+ * cols floats, V the grammar's nvals values. The same level sweep as the
+ * reference driver's OpenMP CPU reference (reference gpu_engine_test.cu:49-104);
+ * each level's row-sorted emissions are split at row boundaries, so no atomics
+ * are needed. */
+int gmr_schedule_cpu_sweep(const gmr_schedule* s, int semiring, const float* V, int nvals, const float* x, void* y,
+                           int threads);
 void gmr_schedule_free(gmr_schedule* s);
 
 #ifdef __cplusplus

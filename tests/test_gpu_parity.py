@@ -417,6 +417,9 @@ def test_gpu_test_driver_crosscheck_files(tmp_path):
                            timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         assert "GPU kernel:" in r.stdout and "Batched verification" in r.stdout and "SUCCESS" in r.stdout
+        # the in-run CPU check of the reference driver (gpu_engine_test.cu:143-176, :251-274)
+        assert "CPU OpenMP vs sequential: max_rel_diff=" in r.stdout
+        assert "SUCCESS: GPU and CPU reference results match" in r.stdout, r.stdout
         # energy per vector between two counter updates >= 0.5 s apart: a
         # positive reading, or -1 where the counter cannot be read
         mj = float(r.stdout.split("ENERGY mJ/vector:")[1].split()[0])

@@ -331,3 +331,26 @@ def test_row_blocks_partition_the_csrv_string():
     for r0, r1 in ((-1, 3), (5, 4), (0, 24)):
         with pytest.raises(gmr.GmrError):
             m.row_block(r0, r1)
+
+
+@pytest.mark.parametrize("sr", [0, 1, 2])
+def test_host_cpu_sweep_matches_the_oracle(sr):
+    """The host library's CPU layered sweep (what bin/gpu_test checks the GPU
+    against in the same run) against the oracle's brute force, on every layout,
+    with 1 and 4 threads: Boolean / min-plus exact, (+,x) within FP32."""
+    from helpers import rel_err, x_for
+    for m, cols in ((gmr.Matrix.haplotype(20261204, 2048, 1024), 1024),
+                    (gmr.Matrix.rmat(77, 12, 40000, nweights=100 if sr == 2 else 1), 4096)):
+        g = m.repair()
+        seq, V = m.arrays()
+        rows = m.rows
+        x = x_for(sr, cols, 900 + sr)
+        y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
+        for lay in ("refcompat", "wave64", "resident"):
+            s = g.schedule(rows, cols, lay)
+            for threads in (1, 4):
+                y = s.cpu_sweep(sr, x, threads=threads)
+                if sr == 0:
+                    assert rel_err(y, y_star, den) <= 1e-5, (lay, threads)
+                else:
+                    assert np.array_equal(y.astype(np.int64), y_star.astype(np.int64)), (lay, threads)
