@@ -321,6 +321,7 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
     GMR_HIP(hipEventCreate(&ev1));
     int dev = 0;
     GMR_HIP(hipGetDevice(&dev));
+    device = dev;
     GMR_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
     GMR_HIP(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev));
     if (wall_khz <= 0) wall_khz = 100000;
@@ -417,6 +418,7 @@ void build_part_lists(long r0, long r1, const Off& off, const std::vector<int>& 
 const Engine::XParts& Engine::xparts_for(int passes) {
     auto it = xparts.find(passes);
     if (it != xparts.end()) return it->second;
+    forbid_alloc("the XCD-partitioned gather's lists for this pass count");
     if (!resident) throw std::invalid_argument("the partitioned gather needs the Resident layout");
     const long nsym = crow_host[0];
     std::vector<int> crow(static_cast<std::size_t>(rows) + 1), csym(static_cast<std::size_t>(nsym));
@@ -475,6 +477,7 @@ Engine::XParts range_lists(Engine& en, long nkey, long span, const std::function
 const Engine::XParts& Engine::bit_lists(int range_bits) {
     auto it = bparts.find(range_bits);
     if (it != bparts.end()) return it->second;
+    forbid_alloc("the bit-mode lists for this range size");
     if (!resident) throw std::invalid_argument("the bit-mode gather needs the Resident layout");
     ensure_bits();
     const long tbits = static_cast<long>(bits_tw) * 64;
@@ -487,6 +490,7 @@ const Engine::XParts& Engine::bit_lists(int range_bits) {
 const Engine::XParts& Engine::lds_lists(int span) {
     auto it = lparts.find(span);
     if (it != lparts.end()) return it->second;
+    forbid_alloc("the LDS-range gather's lists for this range size");
     if (!resident) throw std::invalid_argument("the LDS-range gather needs the Resident layout");
     const long u = U;
     auto key = [u](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : u + c; };
@@ -494,6 +498,7 @@ const Engine::XParts& Engine::lds_lists(int span) {
 }
 
 void Engine::ensure_workspace(int Bt, int vbytes) {
+    if (no_alloc) return;  // the workspace is the scratch array
     const std::size_t tb = static_cast<std::size_t>(U) * Bt * vbytes;
     const std::size_t wn = resident ? static_cast<std::size_t>(NL) : static_cast<std::size_t>(max_width);
     const std::size_t wb = std::max<std::size_t>(wn, 1) * Bt * vbytes;
@@ -526,6 +531,7 @@ void Engine::ensure_workspace(int Bt, int vbytes) {
 // word per 64 rule slots. Its size is fixed by the grammar, so it never moves.
 unsigned long long* Engine::ensure_bits() {
     if (!ws_bits) {
+        forbid_alloc("the bit-mode bitset");
         bits_tw = (U + 63) / 64;
         const std::size_t words = static_cast<std::size_t>(bits_tw) + (static_cast<std::size_t>(NL) + 63) / 64 + 1;
         GMR_HIP(hipMalloc(&ws_bits, words * sizeof(unsigned long long)));
@@ -733,6 +739,7 @@ void run_tile(Engine& en, int semiring, SweepArgs a, const gmr_run_opts& o, hipS
                                    reinterpret_cast<uintptr_t>(st));
         auto it = en.graphs.find(key);
         if (it == en.graphs.end()) {
+            en.forbid_alloc("a captured graph");
             // Capture on the engine's own stream (the caller's may be the
             // NULL stream, which cannot be captured); replay on the caller's.
             if (!en.capture_stream) GMR_HIP(hipStreamCreateWithFlags(&en.capture_stream, hipStreamNonBlocking));
@@ -936,6 +943,61 @@ void Engine::run(int semiring, const float* X, int B, void* Y, const gmr_run_opt
     }
 }
 
+void Engine::forbid_alloc(const char* what) const {
+    if (no_alloc)
+        throw std::invalid_argument(std::string("no-allocation run: ") + what +
+                                    " would be allocated now; run gmr_engine_run once with the same options first "
+                                    "(it builds them), or choose options that need none");
+}
+
+// The sweep with its workspace (terminal values, then the value array or the
+// two frontier buffers, each 256-byte aligned) carved from `scratch`.
+void Engine::run_noalloc(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st,
+                         void* scratch, std::size_t scratch_bytes) {
+    if (B < 1) throw std::invalid_argument("B must be >= 1");
+    if (o.variant == GMR_VARIANT_GRAPH)
+        throw std::invalid_argument("no-allocation run: the levels or persistent variant (a captured graph allocates)");
+    int dev = 0;
+    GMR_HIP(hipGetDevice(&dev));
+    if (dev != device) throw std::invalid_argument("no-allocation run: the engine belongs to another device");
+    const int Bt = (o.b_tile > 0 && o.b_tile < B) ? o.b_tile : B;
+    const std::size_t Bs = static_cast<std::size_t>(Bt <= 1 ? 1 : (Bt + 3) / 4 * 4);
+    auto up = [](std::size_t n) { return (n + 255) / 256 * 256; };
+    const std::size_t tb = up(static_cast<std::size_t>(U) * Bs * 4);
+    const std::size_t wn = resident ? static_cast<std::size_t>(NL) : static_cast<std::size_t>(max_width);
+    const std::size_t wb = up(std::max<std::size_t>(wn, 1) * Bs * 4);
+    const std::size_t need = tb + wb * (resident ? 1 : 2);
+    if (need > scratch_bytes)
+        throw std::invalid_argument("no-allocation run: the sweep needs " + std::to_string(need) +
+                                    " bytes of workspace (vector tile of " + std::to_string(Bt) +
+                                    "), more than the static scratch array's " + std::to_string(scratch_bytes) +
+                                    "; use a narrower b_tile or gmr_engine_run");
+    void* const saved[3] = {ws_T, ws_W0, ws_W1};
+    const std::size_t saved_b[2] = {ws_T_bytes, ws_W_bytes};
+    char* base = static_cast<char*>(scratch);
+    ws_T = base;
+    ws_W0 = base + tb;
+    ws_W1 = resident ? nullptr : base + tb + wb;
+    ws_T_bytes = tb;
+    ws_W_bytes = wb;
+    no_alloc = true;
+    auto restore = [&] {
+        no_alloc = false;
+        ws_T = saved[0];
+        ws_W0 = saved[1];
+        ws_W1 = saved[2];
+        ws_T_bytes = saved_b[0];
+        ws_W_bytes = saved_b[1];
+    };
+    try {
+        run(semiring, X, B, Y, o, st);
+    } catch (...) {
+        restore();
+        throw;
+    }
+    restore();
+}
+
 void Engine::set_gather_order(int order) {
     if (!resident || n_gseg == 0) return;
     std::vector<int4> seg(n_gseg);
@@ -974,6 +1036,7 @@ void Engine::set_gather_order(int order) {
 const int* Engine::device_kinds(const int* host_kinds) {
     std::vector<int> v(host_kinds, host_kinds + L + 1);
     if (v != kinds_host || !d_kinds) {
+        forbid_alloc("a per-level kernel table (level_kind)");
         if (!d_kinds) GMR_HIP(hipMalloc(&d_kinds, sizeof(int) * (L + 1)));
         // The previous table may still be read by queued work.
         GMR_HIP(hipDeviceSynchronize());
@@ -1777,6 +1840,51 @@ int gmr_engine_run(gmr_engine* h, int semiring, const void* d_X, int B, void* d_
         h->e.run(semiring, static_cast<const float*>(d_X), B, d_Y, o, static_cast<hipStream_t>(stream));
     });
     // A timed run has synchronised its stream: its own failure is known now.
+    if (rc == GMR_OK && (o.flags & GMR_RUN_TIMED) && h->e.barrier.take_failure())
+        return dev_fail(GMR_ERR_BARRIER, "this persistent launch's grid barrier timed out: the Y it wrote is invalid");
+    return rc;
+}
+
+// ---- no-allocation runs ----
+// The workspace of gmr_engine_run_noalloc: a file-scope device array. Each
+// GPU the library runs on holds its own copy (GMR_NOALLOC_SCRATCH_BYTES) until
+// the process ends. In host code `scratch` names a host-side stand-in, not a
+// device address: its address on a device comes from hipGetSymbolAddress with
+// that device current, looked up once per device and kept in g_scratch_ptr,
+// and only that pointer is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[GMR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void* scratch_on_current_device() {
+    int dev = 0;
+    GMR_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::invalid_argument("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        GMR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw gmr::DeviceError("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+int gmr_engine_run_noalloc(gmr_engine* h, int semiring, const void* d_X, int B, void* d_Y,
+                           const gmr_run_opts* opts, void* stream) {
+    if (!h) return dev_fail(GMR_ERR_INVALID, "engine is NULL");
+    if (h->e.barrier.take_failure()) return dev_fail(GMR_ERR_BARRIER, kBarrierMsg);
+    gmr_run_opts o{};
+    if (opts) o = *opts;
+    int rc = dev_guarded([&] {
+        void* ws = scratch_on_current_device();
+        h->e.run_noalloc(semiring, static_cast<const float*>(d_X), B, d_Y, o, static_cast<hipStream_t>(stream), ws,
+                         GMR_NOALLOC_SCRATCH_BYTES);
+    });
     if (rc == GMR_OK && (o.flags & GMR_RUN_TIMED) && h->e.barrier.take_failure())
         return dev_fail(GMR_ERR_BARRIER, "this persistent launch's grid barrier timed out: the Y it wrote is invalid");
     return rc;

@@ -97,6 +97,14 @@ struct Engine {
     int bits_tw = 0;                 // terminal words of ws_bits: ceil(U / 64)
     unsigned long long* ensure_bits();
 
+    // no-allocation runs (gmr_engine_run_noalloc): while set, the workspace is
+    // the caller-side scratch array and every path that would allocate throws
+    bool no_alloc = false;
+    int device = 0;                  // the device the engine was uploaded to
+    void forbid_alloc(const char* what) const;
+    void run_noalloc(int semiring, const float* X, int B, void* Y, const gmr_run_opts& o, hipStream_t st,
+                     void* scratch, std::size_t scratch_bytes);
+
     // device facts and timing
     int num_cus = 0, wall_khz = 0;
     BarrierState barrier;

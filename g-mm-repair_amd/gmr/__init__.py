@@ -336,6 +336,22 @@ class Engine:
                                          C.byref(o), C.c_void_p(stream)))
         del keep
 
+    def run_noalloc_torch(self, semiring, X, Y, **kw):
+        """gmr_engine_run_noalloc on torch device tensors: X float32 (cols, B)
+        or (cols,), Y preallocated (rows, B) or (rows,), float32 for (+,x) and
+        int32 otherwise. The workspace is the library's static scratch array."""
+        import torch
+        if isinstance(semiring, str):
+            semiring = SEMIRINGS[semiring]
+        assert X.is_cuda and X.dtype == torch.float32 and X.is_contiguous() and Y.is_contiguous()
+        B = 1 if X.dim() == 1 else X.shape[1]
+        o, keep = self._opts(**kw)
+        stream = torch.cuda.current_stream(X.device).cuda_stream
+        _check(_lib.dev().gmr_engine_run_noalloc(self._h, semiring, C.c_void_p(X.data_ptr()), B,
+                                                 C.c_void_p(Y.data_ptr()), C.byref(o), C.c_void_p(stream)))
+        del keep
+        return Y
+
     def time_levels(self, semiring, X, Y, reps=3, **kw):
         """Device times (ms): [0] init and terminal emits, [k] level k's node
         update, [L+1] the last emits or the row gather. X, Y torch tensors."""

@@ -109,6 +109,7 @@ def dev():
     _sig(L, "gmr_engine_from_schedule", C.c_int, [vp, c_float_p, C.c_int, vpp])
     _sig(L, "gmr_engine_info", C.c_int, [vp, c_ll_p])
     _sig(L, "gmr_engine_run", C.c_int, [vp, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(RunOpts), C.c_void_p])
+    _sig(L, "gmr_engine_run_noalloc", C.c_int, [vp, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(RunOpts), C.c_void_p])
     _sig(L, "gmr_engine_last_kernel_ms", C.c_int, [vp, c_double_p])
     _sig(L, "gmr_engine_check", C.c_int, [vp, c_int_p])
     _sig(L, "gmr_engine_set_barrier_timeout", C.c_int, [vp, C.c_longlong])

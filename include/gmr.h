@@ -116,6 +116,25 @@ int gmr_engine_info(const gmr_engine* e, long long info[16]);
  * returns after enqueueing. opts may be NULL (levels variant, spot emit). */
 int gmr_engine_run(gmr_engine* e, int semiring, const void* d_X, int B, void* d_Y,
                    const gmr_run_opts* opts, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. As gmr_engine_run, but the sweep's workspace (terminal
+ * values and the rule-value array or frontier buffers: about (U + rules) x 4
+ * bytes per vector of a tile in the Resident layout) comes from a static device
+ * array of GMR_NOALLOC_SCRATCH_BYTES per GPU, and the call allocates nothing
+ * (no hipMalloc, hipFree, graph instantiation or table upload). A run whose
+ * workspace exceeds that size is refused with GMR_ERR_INVALID (use a narrower
+ * b_tile or gmr_engine_run), and so is any run that would have to build
+ * something first: the graph variant, level_kind tables, and the one-vector
+ * gathers' lists (XCD-partitioned, LDS ranges, bit mode) for options not yet
+ * used with gmr_engine_run on this engine -- run it once with the same options
+ * at set-up, or use GMR_RUN_SINGLE_LIST / GMR_RUN_ROW_SEGMENTS. Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (several
+ * streams or threads, any engines) race; calls in stream order are safe,
+ * since each sweep writes all of its workspace before reading it. */
+#define GMR_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int gmr_engine_run_noalloc(gmr_engine* e, int semiring, const void* d_X, int B, void* d_Y,
+                           const gmr_run_opts* opts, void* stream);
 /* Resident layout: order in which the row gather visits its segments.
  * 0 = row order (default); 1 = segments sharing their highest rule together
  * (cache locality for the value rows they read). Results are unchanged. */
