@@ -36,6 +36,13 @@ def log(*a):
     print(f"[bench {time.time() - T0:6.1f}s]", *a, file=sys.stderr, flush=True)
 
 
+def rel_err_cols(y, y_star, den):
+    """max |y - y*| / sum_j |M_ij x_j| (exact match required where the sum is 0)."""
+    import numpy as np
+    d = np.abs(np.asarray(y, np.float64) - np.asarray(y_star, np.float64))
+    return float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1), np.where(d == 0, 0, np.inf))))
+
+
 def dag_bytes(info, B):
     """Algorithmic bytes of one sweep over B vectors, counted on the grammar DAG
     (schedule-independent): per rule 2 child ids + per vector 2 reads, 1 write;
@@ -43,6 +50,64 @@ def dag_bytes(info, B):
     (value, column) + per vector an x gather and a T write; y initialised."""
     R, E, Et, U, rows = info["num_raw_nt"], info["emit_total"], info["num_term_emit"], info["U"], info["rows"]
     return 8 * R + 8 * (E + Et) + 8 * (U - 1) + 4 * B * (3 * R + 3 * (E + Et) + 2 * (U - 1) + rows)
+
+
+def amended_bytes(info, B):
+    """The DAG count with the row gather's emission: per emission one 4-byte
+    read per vector (no read-modify-write of y), and y written once; rules and
+    terminals as in dag_bytes. Counted on the grammar form that runs."""
+    R, E, Et, U, rows = info["num_raw_nt"], info["emit_total"], info["num_term_emit"], info["U"], info["rows"]
+    return 8 * R + 8 * (E + Et) + 8 * (U - 1) + 4 * B * (3 * R + (E + Et) + 2 * (U - 1) + rows)
+
+
+def cpu_cores():
+    """(threads this process may run on, physical cores from lscpu or None)."""
+    avail = len(os.sched_getaffinity(0))
+    phys = None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = {k.strip(): v.strip() for k, v in (ln.split(":", 1) for ln in out.splitlines() if ":" in ln)}
+        phys = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except Exception:
+        pass
+    return avail, phys
+
+
+def reference_driver_leg(g, rows, cols, x):
+    """The reference's own driver (gpu_engine_test.cu, compiled unchanged against
+    this engine's drop-in headers into oracle/_ref) on the same grammar and x:
+    its OpenMP CPU sweep, its 1-thread sweep and its GPU-vs-CPU verification in
+    one run. None if the binary was not built."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "gpu_engine_test")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        base = os.path.join(d, "g")
+        g.write(base)
+        xf = os.path.join(d, "x.bin")
+        x.astype("float32").tofile(xf)
+        env = dict(os.environ, SEMIRING="plustimes", XVEC=xf, GMR_ENERGY_SECONDS="0")
+        t = time.time()
+        r = subprocess.run([exe, base, str(rows), str(cols), "1"], capture_output=True, text=True, env=env,
+                           timeout=600)
+    out = r.stdout
+
+    def num(tag, field=0):
+        if tag not in out:
+            return None
+        return float(out.split(tag)[1].split()[field].rstrip(","))
+    return {"exit_code": r.returncode, "wall_s": round(time.time() - t, 1),
+            "cpu_openmp_ms": num("CPU reference (OpenMP):"),
+            "cpu_openmp_threads": int(out.split("ms/vector (")[1].split()[0]) if "ms/vector (" in out else None,
+            "cpu_sequential_ms": num("CPU reference (sequential):"),
+            "gpu_vs_cpu_max_rel": (float(out.split("Verification: ")[1].split("max_rel_diff=")[1].split(",")[0])
+                                   if "Verification: " in out else None),
+            "verdict": "SUCCESS" if "SUCCESS: GPU and CPU reference results match" in out else "FAILURE",
+            "note": "the reference's gpu_engine_test.cu, unchanged, linked with this engine's drop-in; "
+                    "metric |gpu - cpu| / (1 + |cpu|), pass < 1e-4"}
 
 
 def layered_bytes(info_ref, B):
@@ -108,6 +173,9 @@ def main():
     ap.add_argument("--variant", default=None, help="force a variant (levels|graph|persistent)")
     ap.add_argument("--emit", default=None, help="force an emission strategy (spot|gather)")
     ap.add_argument("--b-tile", type=int, default=None)
+    ap.add_argument("--schedule", default=None,
+                    help="force a schedule given as a JSON object of run keywords (autotune candidate form), e.g. "
+                         "'{\"variant\": 1, \"emit\": 1, \"flags\": 16, \"lds_values\": 32768}'")
     ap.add_argument("--profile", action="store_true",
                     help="only the timed batched region (for rocprofv3); no plain sweep, CPU baseline or check")
     ap.add_argument("--calibrate", action="store_true",
@@ -186,10 +254,12 @@ def main():
         X, Y = X[:, 0].contiguous(), Y[:, 0].contiguous()
 
     # ---- schedule choice (grammar form, variant, emission, tile, launch shape)
-    if args.no_tune or args.variant:
+    if args.no_tune or args.variant or args.schedule:
         form = untuned_form(args, forms, B)
         best = dict(variant=gmr.VARIANTS[args.variant or "persistent"], emit=gmr.EMITS[args.emit or "gather"],
                     b_tile=args.b_tile or 0, block=256)
+        if args.schedule:
+            best.update(json.loads(args.schedule))
         tuned = []
     else:
         form, best, tuned = tune_forms(forms, X, Y, autotune.candidates(B, gmr.LAYOUTS[args.layout]), reps=2,
@@ -258,8 +328,9 @@ def main():
     # phase, on the stream the kernels run on, over as many sweeps as were timed
     gather_ms = None
     if run_kw.get("emit") == gmr.EMIT_GATHER:
-        lv = eng.time_levels(0, X, Y, reps=args.steps, emit=gmr.EMIT_GATHER, block=run_kw.get("block", 0),
-                             flags=run_kw.get("flags", 0), xcd_passes=run_kw.get("xcd_passes", 0) or 0)
+        # the same options as the timed steps (time_levels runs the levels variant)
+        lv = eng.time_levels(0, X, Y, reps=args.steps,
+                             **{k: v for k, v in run_kw.items() if k not in ("variant", "grid_cap", "b_tile")})
         gather_ms = float(lv[info["max_depth"] + 1])
         log(f"phases (ms, mean of {args.steps}): init {lv[0]:.4f}, node updates {float(lv[1:-1].sum()):.4f}, "
             f"k_gather {gather_ms:.4f}")
@@ -276,10 +347,8 @@ def main():
     Xs = X.reshape(args.cols, -1)[:, cols_chk].contiguous().cpu().numpy()
     Ys, den = O.brute_force_batched(0, args.rows, args.cols, seq, V, Xs)
     Yg = Y.reshape(args.rows, -1)[:, cols_chk].cpu().numpy().astype(np.float64)
-    d = np.abs(Yg - Ys)
-    err = float(np.max(np.where(den > 0, d / np.where(den > 0, den, 1), np.where(d == 0, 0, np.inf))))
+    err = rel_err_cols(Yg, Ys, den)
     log(f"fp32 err (4 columns vs FP64 oracle): {err:.3e}")
-    del seq
 
     # ---- the plain per-level sweep (reference layout, B = 1, spot atomics)
     eng_ref = s_ref.engine()
@@ -313,11 +382,16 @@ def main():
     # device time at B = 64); the whole sweep is reported beside it
     workload = f"{args.workload}{args.rows}x{args.cols}_B{B}"
     sweep_traffic = read_traffic(workload, "sweep")
-    sweep = {"ms": ms_step, "bytes_dag": dag_bytes(info, B), "bytes_reference_layered": layered_bytes(info_ref, B),
+    amended = amended_bytes(info, B)
+    sweep = {"ms": ms_step, "bytes": amended, "achieved": round(amended / (ms_step * 1e-3) / 1e9, 1),
+             "frac": round(amended / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+             "bytes_dag_literal": dag_bytes(info, B), "bytes_reference_layered": layered_bytes(info_ref, B),
              "traffic": sweep_traffic,
-             "note": "DAG-model bytes (of the grammar form run) charge every emission a y read-modify-write that "
-                     "row gather does not do; the reference-layered count is that of the reference's own schedule "
-                     "of the grammar as built"}
+             "note": "bytes: the DAG count of the grammar form run with the row gather's emission (one 4-byte "
+                     "read per emission per vector, y written once), over the whole step's time; "
+                     "bytes_dag_literal charges every emission a y read-modify-write that the row gather does not "
+                     "do, so it overcounts (it would exceed the chip's bandwidth); the reference-layered count is "
+                     "that of the reference's own schedule of the grammar as built"}
     if sweep_traffic:
         sweep["achieved_measured"] = round(sweep_traffic / (kern_ms * 1e-3) / 1e9, 1)
         sweep["frac_measured"] = round(sweep_traffic / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
@@ -333,11 +407,15 @@ def main():
     # ---- CPU baseline: the reference's OpenMP FP32 sweep, bounded sample
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # every core this process may run on, within the OpenMP thread count the
+        # machine sets for it (a GPU box gives each GPU a share of its host)
+        avail, phys = cpu_cores()
+        omp_env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = min(avail, omp_env) if omp_env > 0 else avail
         sa = s_ref.arrays()
         xs = X.reshape(args.cols, -1)[:, 0].cpu().numpy()
         O.set_threads(threads)
-        O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)  # warm-up
+        y_cpu = O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)  # warm-up; its y is checked below
         times, t0 = [], time.time()
         while not times or time.time() - t0 < args.cpu_seconds:
             t1 = time.time()
@@ -350,7 +428,16 @@ def main():
             t1 = time.time()
             O.cpu_layered_f32(0, sa, args.rows, args.cols, V, xs)
             one.append(time.time() - t1)
-        cpu = {"value": nnz / cpu_s, "unit": "nnz*vectors/s", "cores": threads, "kind": "port",
+        # the same run's GPU column 0 against this CPU y, and both against the oracle
+        y0 = Yg[:, 0]  # the timed run's column 0, copied before any later leg reused Y
+        cpu_vs_gpu = rel_err_cols(y0, np.asarray(y_cpu, np.float64), den[:, 0])
+        cpu_err = rel_err_cols(np.asarray(y_cpu, np.float64), Ys[:, 0], den[:, 0])
+        log(f"cpu baseline y vs GPU y (column 0): {cpu_vs_gpu:.3e}; vs FP64 oracle {cpu_err:.3e}")
+        ref_drv = reference_driver_leg(g0, args.rows, args.cols, xs)
+        log("reference driver", ref_drv)
+        cpu = {"value": nnz / cpu_s, "unit": "nnz*vectors/s", "cores": threads, "cores_available": avail,
+               "cores_physical": phys, "kind": "port", "cpu_vs_gpu_err": cpu_vs_gpu, "cpu_err": cpu_err,
+               "reference_driver": ref_drv,
                "sample": f"{len(times)} single-vector sweeps of the same grammar (reference layout, FP32, "
                          f"OpenMP {threads} threads): mean {cpu_s * 1e3:.1f} ms, best {min(times) * 1e3:.1f} ms, "
                          f"median {float(np.median(times)) * 1e3:.1f} ms; schedule build excluded",
