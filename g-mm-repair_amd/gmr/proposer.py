@@ -171,6 +171,52 @@ def heuristic(table, B):
     return sweep_cost(table, B, B, DEFAULT_PLAN, kinds), B, DEFAULT_PLAN, kinds
 
 
+# Constant schedules: one plan, one node-update kernel on every level and one
+# tile rule (0: tile = B; T: tile = min(B, T)) for every grammar and B -- the
+# best of these is the method without the network to beat.
+CONST_TILES = (0, 16, 32, 64, 128)
+
+
+def constant_rules():
+    return [(plan, kind, T) for plan in range(len(PLANS)) for kind in KINDS for T in CONST_TILES]
+
+
+def constant_cost(table, B, rule):
+    """Measured cost (ms) of a constant rule for B vectors, or None when the
+    table has no timing of the tile it asks for. A kernel that does not apply
+    to a level (LDS staging for tiles wider than one vector) falls back to the
+    group kernel there, as the engine does."""
+    plan, kind, T = rule
+    Bt = B if T == 0 else min(B, T)
+    if Bt not in table:
+        return None
+    ev = table[Bt]["eval"]
+    kinds = [kind if np.isfinite(ev[k, kind]) else EVAL_GROUP for k in range(ev.shape[0])]
+    return sweep_cost(table, B, Bt, plan, kinds)
+
+
+def best_constant(problems):
+    """The constant rule with the lowest summed cost over problems [(table, B)]
+    among the rules every problem has timings for: (rule, total ms)."""
+    best = None
+    for rule in constant_rules():
+        tot = 0.0
+        for table, B in problems:
+            c = constant_cost(table, B, rule)
+            if c is None:
+                tot = None
+                break
+            tot += c
+        if tot is not None and (best is None or tot < best[1]):
+            best = (rule, tot)
+    return best
+
+
+def describe_rule(rule):
+    plan, kind, T = rule
+    return f"{PLAN_NAMES[plan]}/{['group', 'thread', 'wave', 'lds'][kind]}/tile={'B' if T == 0 else f'min(B,{T})'}"
+
+
 class Mlp:
     """The network on the GPU (gmr_mlp_*); inputs are standardised features."""
 
@@ -240,6 +286,10 @@ class Proposer:
         self.sd = np.ones(N_FEAT, np.float32)
         self.tmu = 0.0
         self.tsd = 1.0
+        # The constant schedule the network has to beat (the best one on the
+        # training timings) and the predicted gain it needs to be followed.
+        self.baseline = None
+        self.margin = 0.0
 
     def fit(self, X, T, M, epochs=3000, lr=3e-3):
         self.mu = X.mean(0).astype(np.float32)
@@ -255,14 +305,45 @@ class Proposer:
         o = self.net.forward((X - self.mu) / self.sd)
         return np.exp(o * self.tsd + self.tmu) / 1e3
 
-    def choose(self, arr, B, lpop=None, tiles=None):
-        """(Bt, plan, kinds per level, predicted ms) for B vectors."""
+    def _predicted(self, arr, Bt, lpop, cache):
+        """Predicted times [L + 1, N_OUT] for tile Bt (cached per tile: they do
+        not depend on B)."""
+        if cache is not None and Bt in cache:
+            return cache[Bt]
+        F, g = features(arr, Bt, lpop)
+        P = self.predict(np.vstack([F, g[None, :]]))
+        if cache is not None:
+            cache[Bt] = P
+        return P
+
+    def _predict_rule(self, arr, B, rule, lpop, cache=None):
+        """(Bt, plan, kinds, predicted ms) of a constant rule."""
+        plan, kind, T = rule
+        Bt = B if T == 0 else min(B, T)
+        P = self._predicted(arr, Bt, lpop, cache)
+        k = EVAL_GROUP if (kind == EVAL_LDS and Bt != 1) else kind
+        delta = float((P[:-1, k] - P[:-1, EVAL_GROUP]).sum())
+        return Bt, plan, [k] * (P.shape[0] - 1), math.ceil(B / Bt) * (float(P[-1, 4 + plan]) + delta)
+
+    def choose(self, arr, B, lpop=None, tiles=None, use_baseline=True, cache=None):
+        """(Bt, plan, kinds per level, predicted ms) for B vectors. With a
+        baseline rule set, the network's choice is taken only when it is
+        predicted to beat the baseline's predicted cost by the margin;
+        otherwise the baseline rule is the choice. `cache` (a dict) keeps the
+        predictions of one grammar across calls."""
         if lpop is None:
             lpop = grammar_stats(arr)
+        best = self._choose_net(arr, B, lpop, tiles, cache)
+        if use_baseline and self.baseline is not None:
+            base = self._predict_rule(arr, B, self.baseline, lpop, cache)
+            if not best[3] < (1.0 - self.margin) * base[3]:
+                return base
+        return best
+
+    def _choose_net(self, arr, B, lpop, tiles, cache=None):
         best = None
         for Bt in (tiles or tile_candidates(B)):
-            F, g = features(arr, Bt, lpop)
-            P = self.predict(np.vstack([F, g[None, :]]))
+            P = self._predicted(arr, Bt, lpop, cache)
             ev = P[:-1, :4].copy()
             if Bt != 1:
                 ev[:, EVAL_LDS] = np.inf
@@ -275,13 +356,40 @@ class Proposer:
         return best
 
     def state(self):
-        return {"params": self.net.get_params(), "mu": self.mu, "sd": self.sd,
-                "tmu": np.float32(self.tmu), "tsd": np.float32(self.tsd)}
+        st = {"params": self.net.get_params(), "mu": self.mu, "sd": self.sd,
+              "tmu": np.float32(self.tmu), "tsd": np.float32(self.tsd)}
+        if self.baseline is not None:
+            st["baseline"] = np.array(self.baseline, np.int32)
+            st["margin"] = np.float32(self.margin)
+        return st
 
     def load_state(self, st):
         self.net.set_params(st["params"])
         self.mu, self.sd = st["mu"].astype(np.float32), st["sd"].astype(np.float32)
         self.tmu, self.tsd = float(st["tmu"]), float(st["tsd"])
+        if "baseline" in st:
+            self.baseline = tuple(int(v) for v in st["baseline"])
+            self.margin = float(st["margin"])
+        else:
+            self.baseline, self.margin = None, 0.0
+
+    def fit_margin(self, grammars, Bs, margins=(0.0, 0.05, 0.1, 0.2, 0.3, 0.5), log=None):
+        """Pick the margin with the lowest summed measured cost over training
+        grammars [(arr, table)] at every B in Bs (tiles limited to those the
+        table timed). Returns {margin: total ms}."""
+        totals = {m: 0.0 for m in margins}
+        for i, (arr, table) in enumerate(grammars):
+            lpop = grammar_stats(arr)
+            cache = {}
+            for m in margins:
+                self.margin = m
+                for B in Bs:
+                    c = self.choose(arr, B, lpop=lpop, tiles=[t for t in table if t <= B], cache=cache)
+                    totals[m] += sweep_cost(table, B, c[0], c[1], c[2])
+            if log:
+                log(f"margin fit: grammar {i + 1} of {len(grammars)}")
+        self.margin = min(totals, key=totals.get)
+        return totals
 
     def save(self, path):
         np.savez(path, **self.state())

@@ -96,6 +96,10 @@ def main(argv=None):
                     help="also train on haplotype/genotype matrices generated with weak-compression parameters")
     ap.add_argument("--weights", default=None,
                     help="score these trained weights (.npz) on the held-out set instead of training")
+    ap.add_argument("--save-weights", default=None,
+                    help="write the proposer (network, baseline rule and margin) to this .npz after fitting")
+    ap.add_argument("--constant-from-training", action="store_true",
+                    help="with --weights: also time the training set, to pick the best constant schedule on it")
     ap.add_argument("--inline-top-from", type=int, default=0,
                     help="held-out grammars with B >= this run on Grammar.inline_top(2) (0: never)")
     args = ap.parse_args(argv)
@@ -107,13 +111,18 @@ def main(argv=None):
     os.makedirs(args.out, exist_ok=True)
     train_B = [1, 3, 4, 16, 29, 64, 100, 256] if not args.quick else [1, 4, 16, 37, 64]
     Xs, Ts, Ms = [], [], []
+    train_problems = []  # (timing table, B): where the best constant schedule is chosen
+    train_arrs = []      # (schedule arrays, timing table): where the decision margin is fitted
     t_collect = time.time()
-    for name, m in ([] if args.weights else training_set(args.quick, args.graphs, args.vary)):
+    timed = not args.weights or args.constant_from_training
+    for name, m in (training_set(args.quick, args.graphs, args.vary) if timed else []):
         g = m.repair()
         s = g.schedule(m.rows, m.cols, "resident")
         arr = s.arrays()
         eng = s.engine()
-        _, (X, T, M) = P.collect(eng, arr, train_B, reps=args.reps)
+        table, (X, T, M) = P.collect(eng, arr, train_B, reps=args.reps)
+        train_problems += [(table, B) for B in train_B]
+        train_arrs.append((arr, table))
         Xs.append(X)
         Ts.append(T)
         Ms.append(M)
@@ -132,10 +141,23 @@ def main(argv=None):
         losses = prop.fit(X, T, M, epochs=args.epochs)
         t_train = time.time() - t_train
         log(f"trained on {len(X)} samples: loss {losses[0]:.4f} -> {losses[-1]:.4f} in {t_train:.1f}s")
+    # The method without the network, chosen on the training timings, becomes the
+    # proposer's fallback: the network's choice is followed only when it is
+    # predicted to beat that constant schedule by a margin fitted on the same timings.
+    best_train = P.best_constant(train_problems) if train_problems else None
+    margins = None
+    if best_train is not None:
+        prop.baseline = best_train[0]
+        margins = prop.fit_margin(train_arrs, train_B, log=log)
+        log(f"baseline {P.describe_rule(prop.baseline)}, margin {prop.margin} (training totals {margins})")
+    if not args.weights:
         prop.save(os.path.join(args.out, "proposer.npz"))
+    if args.save_weights:
+        prop.save(args.save_weights)
+    del train_arrs
 
     # ---- held-out evaluation
-    rows_out, engines, Xg, opts = [], [], [], []
+    rows_out, engines, Xg, opts, held_problems = [], [], [], [], []
     t_tune_total = 0.0
     for name, m, B in heldout_set(args.heldout if not args.quick else 6, 20261203,
                                   args.nnz_lo if not args.quick else 2e4, args.nnz_hi if not args.quick else 3e5):
@@ -152,24 +174,53 @@ def main(argv=None):
         t_tune = time.time() - t0
         t_tune_total += t_tune
         t0 = time.time()
-        choice = prop.choose(arr, B, tiles=tiles)
+        cache = {}
+        choice = prop.choose(arr, B, tiles=tiles, cache=cache)
         t_choose = time.time() - t0
+        net = prop.choose(arr, B, tiles=tiles, use_baseline=False, cache=cache)
         ex = P.exhaustive(table, B)
         he = P.heuristic(table, B)
         mine = P.sweep_cost(table, B, choice[0], choice[1], choice[2])
+        net_ms = P.sweep_cost(table, B, net[0], net[1], net[2])
         r = dict(name=name, nnz=m.nnz(), B=B, rules=arr["info"]["num_raw_nt"], depth=arr["info"]["max_depth"],
                  inlined=inlined,
                  exhaustive_ms=ex[0], exhaustive=dict(tile=ex[1], plan=P.PLAN_NAMES[ex[2]]),
                  proposer_ms=mine, proposer=dict(tile=choice[0], plan=P.PLAN_NAMES[choice[1]]),
                  plain_ms=he[0], regret=mine / ex[0] - 1.0, gain_over_plain=he[0] / mine,
+                 network_only_ms=net_ms, network_only_regret=net_ms / ex[0] - 1.0,
+                 followed_network=bool(choice[:3] == net[:3] and prop.baseline is not None),
                  tune_s=t_tune, choose_s=t_choose)
         rows_out.append(r)
+        held_problems.append((table, B))
         log(f"{name:12s} nnz={r['nnz']:>10d} B={B:3d} regret={r['regret'] * 100:6.2f}% "
             f"gain={r['gain_over_plain']:5.2f}x  {r['proposer']} vs {r['exhaustive']}")
         engines.append(eng)
         gen = torch.Generator(device="cuda").manual_seed(len(engines))
         Xg.append(torch.randn(m.cols, B, device="cuda", generator=gen))
         opts.append({"emit": P.PLANS[choice[1]][1], "level_kind": np.array([0] + list(choice[2]), np.int32)})
+    # The method without the network: one constant schedule for every grammar
+    # and B, the best one on the training set (and, stricter, the best one on
+    # the held-out set itself, chosen in hindsight).
+    constants = {}
+    best_hind = P.best_constant(held_problems)
+    for tag, bc in (("training", best_train), ("hindsight", best_hind)):
+        if bc is None:
+            continue
+        costs = [P.constant_cost(t, B, bc[0]) for t, B in held_problems]
+        if any(c is None for c in costs):
+            continue
+        for r, c in zip(rows_out, costs):
+            r[f"constant_{tag}_ms"] = c
+        creg = np.array([c / r["exhaustive_ms"] - 1.0 for r, c in zip(rows_out, costs)])
+        cgain = np.array([c / r["proposer_ms"] for r, c in zip(rows_out, costs)])
+        constants[tag] = dict(rule=P.describe_rule(bc[0]), regret_mean=float(creg.mean()),
+                              regret_median=float(np.median(creg)), regret_max=float(creg.max()),
+                              regret_of_totals=sum(costs) / sum(r["exhaustive_ms"] for r in rows_out) - 1.0,
+                              proposer_gain_geomean=float(np.exp(np.log(cgain).mean())),
+                              proposer_gain_of_totals=sum(costs) / sum(r["proposer_ms"] for r in rows_out),
+                              proposer_wins=int((cgain > 1.0 + 1e-9).sum()),
+                              proposer_losses=int((cgain < 1.0 - 1e-9).sum()))
+        log(f"best constant schedule ({tag}): {constants[tag]}")
     reg = np.array([r["regret"] for r in rows_out])
     gain = np.array([r["gain_over_plain"] for r in rows_out])
     tot_mine = sum(r["proposer_ms"] for r in rows_out)
@@ -202,7 +253,16 @@ def main(argv=None):
                      regret_max=float(reg.max()), regret_of_totals=tot_mine / tot_ex - 1.0,
                      gain_over_plain_geomean=float(np.exp(np.log(gain).mean())),
                      gain_over_plain_of_totals=tot_plain / tot_mine,
-                     tune_s_total=t_tune_total, choose_s_total=sum(r["choose_s"] for r in rows_out)),
+                     tune_s_total=t_tune_total, choose_s_total=sum(r["choose_s"] for r in rows_out),
+                     best_constant=constants,
+                     decision=dict(baseline=P.describe_rule(prop.baseline) if prop.baseline else None,
+                                   margin=prop.margin, margin_training_totals=margins,
+                                   followed_network=int(sum(r["followed_network"] for r in rows_out))),
+                     network_only=dict(
+                         regret_mean=float(np.mean([r["network_only_regret"] for r in rows_out])),
+                         regret_median=float(np.median([r["network_only_regret"] for r in rows_out])),
+                         regret_max=float(np.max([r["network_only_regret"] for r in rows_out])),
+                         regret_of_totals=sum(r["network_only_ms"] for r in rows_out) / tot_ex - 1.0)),
         fused_ms=fused, fused_nnz_vec_per_s={k: nnz_vec / (v * 1e-3) for k, v in fused.items()},
         per_grammar=rows_out)
     with open(os.path.join(args.out, "proposer_report.json"), "w") as f:

@@ -68,3 +68,33 @@ def test_run_options_from_a_choice():
     Bt, variant, opts = P.run_options((32, 5, [1, 0, 2], 0.1))
     assert Bt == 32 and variant == P.PLANS[5][0] and opts["emit"] == P.PLANS[5][1]
     assert opts["level_kind"].tolist() == [0, 1, 0, 2] and opts["b_tile"] == 32
+
+
+def _syn_table(tiles, L=3, plan_ms=None, eval_ms=None):
+    """A synthetic timing table: per tile, [L, 4] level times and 6 plan times."""
+    t = {}
+    for Bt in tiles:
+        ev = np.array(eval_ms if eval_ms is not None else [[1.0, 2.0, 3.0, 0.5]] * L, dtype=float)
+        if Bt != 1:
+            ev[:, P.EVAL_LDS] = np.nan
+        t[Bt] = {"eval": ev, "plan": np.array(plan_ms if plan_ms is not None else [6, 5, 4, 3, 2, 1.0]) * Bt}
+    return t
+
+
+def test_constant_rules_cost_and_best():
+    tab = _syn_table([1, 16, 40])
+    # plan 5 (persistent/gather), group kernel, tile = B: 1 pass of the B-wide plan time
+    assert P.constant_cost(tab, 40, (5, P.EVAL_GROUP, 0)) == 40.0
+    # tile min(B, 16) = 16: three passes of the 16-wide sweep
+    assert P.constant_cost(tab, 40, (5, P.EVAL_GROUP, 16)) == 3 * 16.0
+    # a tile the table has no timing for: not applicable
+    assert P.constant_cost(tab, 40, (5, P.EVAL_GROUP, 32)) is None
+    # the LDS kernel only exists for one vector; elsewhere the group kernel stands in
+    assert P.constant_cost(tab, 40, (5, P.EVAL_LDS, 0)) == P.constant_cost(tab, 40, (5, P.EVAL_GROUP, 0))
+    assert P.constant_cost(tab, 1, (5, P.EVAL_LDS, 0)) == 1.0 + 3 * (0.5 - 1.0)
+    # the best rule over several problems only uses rules every problem has timings for
+    rule, tot = P.best_constant([(tab, 40), (tab, 1)])
+    assert rule[0] == 5 and rule[1] == P.EVAL_LDS and rule[2] in (0, 16, 64, 128)
+    assert abs(tot - (40.0 + (1.0 - 1.5))) < 1e-12
+    assert P.describe_rule((5, P.EVAL_LDS, 16)) == "persistent/gather/lds/tile=min(B,16)"
+    assert len(P.constant_rules()) == len(P.PLANS) * 4 * len(P.CONST_TILES)
