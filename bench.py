@@ -559,13 +559,14 @@ def vendor_leg(args, m, X, Y, eng, run_kw, B, nnz, value):
 
 def grammar_forms(args, g, s_res, eng, log):
     """The grammar forms to run: as built and/or with its top rules written
-    out in C (Grammar.inline_top), each with its Resident schedule and engine."""
+    out in C (Grammar.inline_top), each with its Resident schedule and engine.
+    'auto' tries thresholds 2 (best for many vectors) and 8 (fewer values for
+    the one-vector LDS-range gather: fewer ranges, fewer row pieces)."""
     forms = []
     it = str(args.inline_top)
     if it in ("auto", "0"):
         forms.append({"label": "as built", "max_emits": 0, "grammar": g, "schedule": s_res, "engine": eng})
-    if it != "0":
-        n = 2 if it == "auto" else int(it)
+    for n in ((2, 8) if it == "auto" else (() if it == "0" else (int(it),))):
         t = time.time()
         g2 = g.inline_top(n)
         s2 = g2.schedule(args.rows, args.cols, args.layout)

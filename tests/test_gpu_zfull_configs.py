@@ -2,8 +2,8 @@
 brute force from the uncompressed matrix, computed on the host cores):
 
   * 500,000 x 10,000 genotype matrix (nnz ~5.1e8), as built and with its top
-    rules inlined, at B = 1, 4, 16, 64, 256 (the batched products checked on
-    four sampled columns);
+    rules inlined (thresholds 2 and 8), at B = 1, 4, 16, 64, 256 (the batched
+    products checked on four sampled columns);
   * R-MAT graph with 2^23 vertices from 2^27 edge draws: Boolean one-step
     reachability from a 1% frontier (int32 values and bit mode) and min/plus
     relaxation with weights 1..100 from 1,024 sources (INT32_MAX/2 elsewhere);
@@ -69,6 +69,7 @@ def cfg2():
     seq, V = m.arrays()
     forms = [("as built", g.schedule(500000, 10000, "resident").engine())]
     forms.append(("inlined", g.inline_top(2).schedule(500000, 10000, "resident").engine()))
+    forms.append(("inlined <= 8", g.inline_top(8).schedule(500000, 10000, "resident").engine()))
     _say(f"config 2: nnz {m.nnz()}, {g.info()['nrules']} rules, built in {time.time() - t:.1f}s")
     return seq, V, forms
 
@@ -82,6 +83,8 @@ def test_config2_single_vector(cfg2):
              dict(emit=gmr.EMIT_GATHER, xcd_passes=4, chunks=4),
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES),                  # LDS ranges
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=16384, chunks=2),
+             dict(variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=40960,
+                  chunks=4),
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_SINGLE_LIST),
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_ROW_SEGMENTS),
              dict(emit=gmr.EMIT_SPOT, eval_kind=gmr.EVAL_LDS),
