@@ -82,7 +82,7 @@ Engine::~Engine() {
         if (p) (void)hipFree(p);
     auto free_lists = [](const XParts& x) {
         for (void* p : {static_cast<void*>(x.d_sym), static_cast<void*>(x.d_bits), static_cast<void*>(x.d_ord),
-                        static_cast<void*>(x.d_row), static_cast<void*>(x.d_poff)})
+                        static_cast<void*>(x.d_row), static_cast<void*>(x.d_poff), static_cast<void*>(x.d_cbase)})
             if (p) (void)hipFree(p);
     };
     for (auto* m : {&xparts, &bparts, &lparts})
@@ -440,9 +440,39 @@ const Engine::XParts& Engine::xparts_for(int passes) {
 
 namespace {
 
+// Packs every entry of range lists (already indices inside their range, -1
+// for padding) as (row - first row of its chunk) << 16 | index (0xFFFF for
+// padding), with the chunks' first rows in cbase. Fails (false, L unchanged)
+// when a range is 65,535 keys or more or a chunk spans 65,536 rows or more.
+bool pack_entries(PartLists& L, long span, std::vector<int>& cbase) {
+    if (span >= 0xFFFF) return false;
+    const long nch = static_cast<long>(L.bits.size());
+    std::vector<int> packed(L.sym.size());
+    cbase.assign(static_cast<std::size_t>(nch), 0);
+    for (long c = 0; c < nch; ++c) {
+        int base = 0;
+        for (int lane = 0; lane < 64; ++lane) {
+            const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+            // the row of this entry: the last row started at or before it (padding
+            // belongs to the list's last piece)
+            const int row = L.row[static_cast<std::size_t>(L.ord[c] - 1 + __builtin_popcountll(L.bits[c] & upto))];
+            if (lane == 0) base = cbase[c] = row;
+            const long d = static_cast<long>(row) - base;
+            if (d < 0 || d > 0xFFFF) return false;
+            const int idx = L.sym[static_cast<std::size_t>(c * 64 + lane)];
+            const unsigned e = (static_cast<unsigned>(d) << 16) | (idx < 0 ? 0xFFFFu : static_cast<unsigned>(idx));
+            packed[static_cast<std::size_t>(c * 64 + lane)] = static_cast<int>(e);
+        }
+    }
+    L.sym.swap(packed);
+    return true;
+}
+
 // Lists by fixed key ranges of `span` keys (one list per range, each entry
-// stored as its key's index inside the range; padding -1), uploaded.
-Engine::XParts range_lists(Engine& en, long nkey, long span, const std::function<long(int)>& key) {
+// stored as its key's index inside the range; padding -1), uploaded. With
+// `pack`, entries are packed with their row (pack_entries) where possible.
+Engine::XParts range_lists(Engine& en, long nkey, long span, const std::function<long(int)>& key,
+                           bool pack = false) {
     const long nsym = en.crow_host[0];
     std::vector<int> crow(static_cast<std::size_t>(en.rows) + 1), csym(static_cast<std::size_t>(nsym));
     GMR_HIP(hipDeviceSynchronize());
@@ -459,11 +489,17 @@ Engine::XParts range_lists(Engine& en, long nkey, long span, const std::function
             L.sym[q] = c == -1 ? -1 : static_cast<int>(key(c) - static_cast<long>(p) * span);
         }
     Engine::XParts x;
+    std::vector<int> cbase;
+    x.packed = pack && pack_entries(L, span, cbase);
     x.poff = L.poff;
     x.d_sym = dev_upload(L.sym.data(), L.sym.size(), &en.struct_bytes);
     x.d_bits = dev_upload(L.bits.data(), L.bits.size(), &en.struct_bytes);
-    x.d_ord = dev_upload(L.ord.data(), L.ord.size(), &en.struct_bytes);
-    x.d_row = dev_upload(L.row.data(), L.row.size(), &en.struct_bytes);
+    if (x.packed) {
+        x.d_cbase = dev_upload(cbase.data(), cbase.size(), &en.struct_bytes);
+    } else {
+        x.d_ord = dev_upload(L.ord.data(), L.ord.size(), &en.struct_bytes);
+        x.d_row = dev_upload(L.row.data(), L.row.size(), &en.struct_bytes);
+    }
     x.d_poff = dev_upload(x.poff.data(), x.poff.size(), &en.struct_bytes);
     return x;
 }
@@ -494,7 +530,7 @@ const Engine::XParts& Engine::lds_lists(int span) {
     if (!resident) throw std::invalid_argument("the LDS-range gather needs the Resident layout");
     const long u = U;
     auto key = [u](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : u + c; };
-    return lparts.emplace(span, range_lists(*this, u + NL, span, key)).first->second;
+    return lparts.emplace(span, range_lists(*this, u + NL, span, key, true)).first->second;
 }
 
 void Engine::ensure_workspace(int Bt, int vbytes) {
@@ -670,12 +706,16 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         const std::size_t lds = static_cast<std::size_t>(a.lspan) * sizeof(typename S::T);
         const unsigned g = lds_gather_grid(en, nch, lds);
         if (nch > 0) {
-            if (a.chunks == 2)
-                hipLaunchKernelGGL((dev::k_gather_lds<S, 2>), dim3(g), dim3(dev::kLdsGatherBlock), lds, st, a);
-            else if (a.chunks == 8)
-                hipLaunchKernelGGL((dev::k_gather_lds<S, 8>), dim3(g), dim3(dev::kLdsGatherBlock), lds, st, a);
-            else
-                hipLaunchKernelGGL((dev::k_gather_lds<S, 4>), dim3(g), dim3(dev::kLdsGatherBlock), lds, st, a);
+            const dim3 gr(g), bl(dev::kLdsGatherBlock);
+            if (a.packed) {
+                if (a.chunks == 2) hipLaunchKernelGGL((dev::k_gather_lds<S, 2, true>), gr, bl, lds, st, a);
+                else if (a.chunks == 8) hipLaunchKernelGGL((dev::k_gather_lds<S, 8, true>), gr, bl, lds, st, a);
+                else hipLaunchKernelGGL((dev::k_gather_lds<S, 4, true>), gr, bl, lds, st, a);
+            } else {
+                if (a.chunks == 2) hipLaunchKernelGGL((dev::k_gather_lds<S, 2, false>), gr, bl, lds, st, a);
+                else if (a.chunks == 8) hipLaunchKernelGGL((dev::k_gather_lds<S, 8, false>), gr, bl, lds, st, a);
+                else hipLaunchKernelGGL((dev::k_gather_lds<S, 4, false>), gr, bl, lds, st, a);
+            }
         }
     } else if (a.gather) {
         // (the B-wide row gather takes two segments per lane group at once)
@@ -876,6 +916,8 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
                 "workgroup (" + std::to_string(en.max_lds) + " bytes)");
         const Engine::XParts& x = en.lds_lists(span);
         a.lspan = span;
+        a.packed = x.packed ? 1 : 0;
+        a.cbase = x.d_cbase;
         a.nkey = static_cast<long>(en.U) + en.NL;
         a.xsym = x.d_sym;
         a.xbits = x.d_bits;

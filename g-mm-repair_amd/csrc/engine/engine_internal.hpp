@@ -75,6 +75,11 @@ struct Engine {
         int* d_row = nullptr;
         int* d_poff = nullptr;
         std::vector<int> poff;       // 8 x passes + 1 chunk offsets
+        // LDS-range gather with packed entries: entry = (row - d_cbase[chunk]) << 16 |
+        // index inside the range (0xFFFF: padding), so a piece's row needs no lookup;
+        // d_ord / d_row are then not used
+        bool packed = false;
+        int* d_cbase = nullptr;      // per chunk: row of its first entry
     };
     std::map<int, XParts> xparts;
     std::map<int, XParts> bparts;    // Boolean bit mode: lists by LDS-sized bit range, per range size

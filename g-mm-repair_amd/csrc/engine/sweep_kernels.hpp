@@ -139,6 +139,10 @@ struct SweepArgs {
     int lds_window;          // EV_LDS left-child window of the per-level launches (values)
     int bit_range;           // bit mode: bits per LDS range
     int chunks;              // 64-symbol chunks per wave in flight (chunked single-vector gathers)
+    // LDS-range gather, packed entries: entry = (row - cbase[chunk]) << 16 | index
+    // inside the range (0xFFFF: padding); xord / xrow are then not used
+    int packed;
+    const int* cbase;
 };
 
 // Per-level node-update kernels (one is chosen for each level):
@@ -1250,7 +1254,10 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
 // reaches the L1/L2 caches; the cost moves to one atomic per row piece.
 constexpr int kLdsGatherBlock = 1024;
 
-template <class S, int K>
+// PACKED: every entry carries its row as an offset from its chunk's first row
+// (the high 16 bits; the index inside the range is the low 16 bits), so the
+// row of a piece comes from the entry that ends it, with no dependent load.
+template <class S, int K, bool PACKED>
 __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
     using T = typename S::T;
     T* lds = reinterpret_cast<T*>(dyn_lds);
@@ -1289,7 +1296,7 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
                 const long cj = cc + j < c1 ? cc + j : c1 - 1;
                 l[j] = __builtin_nontemporal_load(loc + cj * 64 + lane);
                 fb[j] = a.xbits[cj];
-                ord[j] = a.xord[cj];
+                ord[j] = PACKED ? a.cbase[cj] : a.xord[cj];
             }
         };
         load_chunks(c);
@@ -1298,10 +1305,13 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
             bool end[K];
             int ri[K];
 #pragma unroll
-            for (int j = 0; j < K; ++j) v[j] = lds[l[j] < static_cast<unsigned>(span) ? l[j] : 0u];
+            for (int j = 0; j < K; ++j) {
+                const unsigned li = PACKED ? (l[j] & 0xFFFFu) : l[j];
+                v[j] = lds[li < static_cast<unsigned>(span) ? li : 0u];
+            }
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                if (l[j] >= static_cast<unsigned>(span)) v[j] = S::zero();  // padding
+                if ((PACKED ? (l[j] & 0xFFFFu) : l[j]) >= static_cast<unsigned>(span)) v[j] = S::zero();  // padding
                 const unsigned long long bits = fb[j];
 #pragma unroll
                 for (int d = 1; d < 64; d <<= 1) {
@@ -1314,7 +1324,7 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
             }
             int row[K];
 #pragma unroll
-            for (int j = 0; j < K; ++j) row[j] = a.xrow[ri[j]];
+            for (int j = 0; j < K; ++j) row[j] = PACKED ? ord[j] + static_cast<int>(l[j] >> 16) : a.xrow[ri[j]];
             const long cn = c + step;
             load_chunks(cn);
 #pragma unroll

@@ -94,6 +94,25 @@ def test_lds_range_gather(hap, geno, sr):
             check(sr, y, y_star, den, f"{name} lds span=40960")
 
 
+def test_lds_range_gather_unpacked_rows():
+    """Entries carry their row as a 16-bit offset from their chunk's first row;
+    a chunk spanning 65,536 rows or more (here: three non-zeros in 150,000 rows)
+    keeps the row table instead. Both forms against the oracle."""
+    rows, cols = 150000, 8
+    M = np.zeros((rows, cols))
+    M[0, 0], M[100000, 1], M[149999, 2], M[149999, 0] = 2.0, 3.0, 5.0, 7.0
+    m = gmr.Matrix.from_dense(M)
+    g = m.repair()
+    seq, V = m.arrays()
+    eng = g.schedule(rows, cols, "resident").engine()
+    for sr in (0, 1, 2):
+        x = x_for(sr, cols, 990 + sr)
+        y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
+        for ch in (2, 4):
+            y = run(eng, sr, x, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=4096, chunks=ch)
+            check(sr, y, y_star, den, f"sparse rows lds chunks={ch}")
+
+
 def test_level_chunk_options(hap, geno):
     """EV_LDS windows, XCD-partitioned gather chunks, bit-mode ranges and chunks."""
     for name, (eng, seq, V), rows, cols in (("hap", hap, 4096, 2048), ("geno", geno, 6000, 10000)):
