@@ -87,7 +87,7 @@ Engine::~Engine() {
     };
     for (auto* m : {&xparts, &bparts, &lparts})
         for (auto& kv : *m) free_lists(kv.second);
-    void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_emits, d_term, d_crow, d_csym, d_gseg, d_zrows, d_fbits, d_ford,
+    void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_tref, d_emits, d_term, d_crow, d_csym, d_gseg, d_zrows, d_fbits, d_ford,
                     d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_kinds,
                     static_cast<void*>(ws_bits)};
     for (void* p : ptrs)
@@ -295,7 +295,22 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
         gather_lanes = 1;
         while (gather_lanes < 64 && gather_lanes * 4 < mean) gather_lanes <<= 1;
     }
+    // Terminals that rules have as children: the only terminal values a sweep
+    // needs outside the row gather (the LDS-range gather computes the others
+    // as it fills its ranges).
+    std::vector<int> tref;
+    {
+        std::vector<char> used(static_cast<std::size_t>(std::max(U, 1)), 0);
+        for (const int2& k : kids) {
+            if (k.x < 0) used[static_cast<std::size_t>(-k.x - 1)] = 1;
+            if (k.y < 0) used[static_cast<std::size_t>(-k.y - 1)] = 1;
+        }
+        for (int q = 1; q < U; ++q)
+            if (used[q]) tref.push_back(q);
+    }
+    n_tref = static_cast<int>(tref.size());
     struct_bytes = 0;
+    d_tref = dev_upload(tref.data(), tref.size(), &struct_bytes);
     d_kids = dev_upload(kids.data(), kids.size(), &struct_bytes);
     d_coeff = has_runlen ? dev_upload(coeff.data(), coeff.size(), &struct_bytes) : nullptr;
     d_tcol = dev_upload(tcol.data(), tcol.size(), &struct_bytes);
@@ -918,6 +933,10 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
         a.lspan = span;
         a.packed = x.packed ? 1 : 0;
         a.cbase = x.d_cbase;
+        // the range fills compute the terminal values; the init computes only
+        // those the rules read
+        a.tref = en.d_tref;
+        a.n_tref = en.n_tref;
         a.nkey = static_cast<long>(en.U) + en.NL;
         a.xsym = x.d_sym;
         a.xbits = x.d_bits;

@@ -52,6 +52,8 @@ struct Engine {
     float* d_coeff = nullptr;
     int* d_tcol = nullptr;
     float* d_tval = nullptr;
+    int* d_tref = nullptr;           // terminals some rule has as a child (increasing)
+    int n_tref = 0;
     int2* d_emits = nullptr;
     int2* d_term = nullptr;
     int* d_crow = nullptr;

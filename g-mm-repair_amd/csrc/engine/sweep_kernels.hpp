@@ -139,6 +139,10 @@ struct SweepArgs {
     int lds_window;          // EV_LDS left-child window of the per-level launches (values)
     int bit_range;           // bit mode: bits per LDS range
     int chunks;              // 64-symbol chunks per wave in flight (chunked single-vector gathers)
+    // LDS-range gather (one vector): the init phase computes only the terminal
+    // values rules read (tref, n_tref of them); the range fills compute the rest
+    const int* tref;
+    int n_tref;
     // LDS-range gather, packed entries: entry = (row - cbase[chunk]) << 16 | index
     // inside the range (0xFFFF: padding); xord / xrow are then not used
     int packed;
@@ -247,9 +251,12 @@ __device__ void ph_init_range(const SweepArgs& a, long i0, long i1, long tid, lo
     using T = typename S::T;
     T* Tt = tp<S>(a.Tv);
     T* Y = tp<S>(a.Y);
-    const long nT = static_cast<long>(a.U) * a.Bs;
+    const long nT = a.lspan ? static_cast<long>(a.n_tref) : static_cast<long>(a.U) * a.Bs;
     for (long i = i0 + tid; i < i1; i += nth) {
-        if (i < nT) {
+        if (a.lspan && i < nT) {  // one vector: only the terminals rules read
+            const int p = a.tref[i];
+            Tt[p] = S::leaf(a.tval[p], a.X[static_cast<long>(a.tcol[p]) * a.B]);
+        } else if (i < nT) {
             int p = static_cast<int>(i / a.Bs);
             int b = static_cast<int>(i - static_cast<long>(p) * a.Bs);
             Tt[i] = (p == 0 || b >= a.Bt) ? S::zero()
@@ -268,7 +275,8 @@ __device__ void ph_init_range(const SweepArgs& a, long i0, long i1, long tid, lo
 // the split rows for the row-segment gather.
 __host__ __device__ __forceinline__ long init_items(const SweepArgs& a) {
     const long ny = zero_list(a) ? (a.gsplit ? static_cast<long>(a.n_zrows) : 0L) : static_cast<long>(a.rows);
-    return static_cast<long>(a.U) * a.Bs + ny * a.Bt;
+    const long nt = a.lspan ? static_cast<long>(a.n_tref) : static_cast<long>(a.U) * a.Bs;
+    return nt + ny * a.Bt;
 }
 
 // Work items of the gather phase: 64-symbol chunks (flat) or row segments.
@@ -1246,8 +1254,9 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
 // One vector, 4-byte values staged in LDS by range (GMR_RUN_LDS_RANGES): the
 // same structure as the bit-mode gather above with a value per entry. List p
 // holds the symbols whose value key lies in [p * lspan, (p + 1) * lspan); a
-// workgroup copies that range of the terminal and rule values into LDS
-// (dynamic, lspan values) once per list its share of chunks touches, looks
+// workgroup fills LDS (dynamic, lspan values) with that range once per list
+// its share of chunks touches -- terminal values computed from V and x on the
+// spot, rule values copied from the resident array -- looks
 // every symbol's value up there, reduces each 64-symbol chunk per row piece
 // with the segmented shuffle scan, and combines every row piece atomically
 // into the zeroed y (a row's symbols sit in many lists). No value lookup
@@ -1265,7 +1274,6 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int nwave = blockDim.x >> 6;
-    const T* Tt = tp<S>(a.Tv);
     const T* W = tp<S>(a.W0);
     T* Y = tp<S>(a.Y);
     const unsigned* loc = reinterpret_cast<const unsigned*>(a.xsym);  // key index inside the list's range
@@ -1280,7 +1288,10 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
         const long k0 = static_cast<long>(p) * span;
         for (int i = threadIdx.x; i < span; i += blockDim.x) {
             const long k = k0 + i;
-            lds[i] = k < a.U ? Tt[k] : (k < a.nkey ? W[k - a.U] : S::zero());
+            // terminal values are computed here (the init phase computed only
+            // those rules read); terminal 0 is the semiring zero
+            lds[i] = k < a.U ? (k == 0 ? S::zero() : S::leaf(a.tval[k], a.X[static_cast<long>(a.tcol[k]) * a.B]))
+                             : (k < a.nkey ? W[k - a.U] : S::zero());
         }
         __syncthreads();
         // As in the bit-mode gather: unconditional, clamped loads, and the next
