@@ -788,9 +788,8 @@ void run_tile(Engine& en, int semiring, SweepArgs a, const gmr_run_opts& o, hipS
         // level, gather form and pass count.
         auto key = std::make_tuple(semiring * 1000003 + kinds_hash(en, o), reinterpret_cast<uintptr_t>(a.X),
                                    reinterpret_cast<uintptr_t>(a.Y), a.B, a.Bt,
-                                   a.gather | (o.flags << 1) | (o.xcd_passes << 8) | (o.chunks << 12) |
-                                       (o.lds_values > 0 ? (__builtin_ctz(o.lds_values) << 16) : 0),
-                                   block,
+                                   a.gather | (o.flags << 1) | (o.xcd_passes << 8) | (o.chunks << 12),
+                                   o.lds_values, block,
                                    reinterpret_cast<uintptr_t>(st));
         auto it = en.graphs.find(key);
         if (it == en.graphs.end()) {
@@ -1504,7 +1503,7 @@ void run_left_t(Engine& en, int semiring, const float* X, int B, void* Y, const 
         a.xparts = (a.Bt == 1 && !(o.flags & GMR_RUN_SINGLE_LIST)) ? 1 : 0;
         if (o.variant == GMR_VARIANT_GRAPH) {
             auto key = std::make_tuple(semiring, reinterpret_cast<uintptr_t>(a.X), reinterpret_cast<uintptr_t>(a.Y), a.B,
-                                       a.Bt, a.xparts, block, reinterpret_cast<uintptr_t>(st));
+                                       a.Bt, a.xparts, 0, block, reinterpret_cast<uintptr_t>(st));
             auto it = en.left_graphs.find(key);
             if (it == en.left_graphs.end()) {
                 if (!en.capture_stream) GMR_HIP(hipStreamCreateWithFlags(&en.capture_stream, hipStreamNonBlocking));
@@ -1700,6 +1699,7 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
             key = mix(key, static_cast<std::size_t>(B[g]) * 4 + static_cast<std::size_t>(o[g].emit));
             key = mix(key, static_cast<std::size_t>(kinds_hash(*bt.eng[g], o[g])));
             key = mix(key, static_cast<std::size_t>(o[g].flags) * 16 + static_cast<std::size_t>(o[g].xcd_passes));
+            key = mix(key, static_cast<std::size_t>(o[g].lds_values) * 16 + static_cast<std::size_t>(o[g].chunks));
             // an engine's workspace may have moved since the graph was captured
             key = mix(key, reinterpret_cast<uintptr_t>(args[g].Tv));
             key = mix(key, reinterpret_cast<uintptr_t>(args[g].W0));

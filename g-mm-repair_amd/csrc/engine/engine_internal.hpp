@@ -127,7 +127,7 @@ struct Engine {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
     };
-    using GraphKey = std::tuple<int, uintptr_t, uintptr_t, int, int, int, int, uintptr_t>;
+    using GraphKey = std::tuple<int, uintptr_t, uintptr_t, int, int, int, int, int, uintptr_t>;
     std::map<GraphKey, Graph> graphs;
 
     // left product (built on first use, Resident layout only)

@@ -1156,6 +1156,28 @@ __device__ void ph_bits_gather(const SweepArgs& a, long tid, long nth) {
     }
 }
 
+// The list that holds chunk c of range lists (xpoff: first chunk of every
+// list, non-decreasing, c < xpoff[nbl]): the number of lists that end at or
+// before c, counted by the whole workgroup at once -- a walk from list 0 would
+// be one dependent load per list. `cnt` is a word of LDS, free until the
+// caller's next barrier.
+__device__ __forceinline__ int list_of_chunk(const SweepArgs& a, long c, unsigned* cnt) {
+    if (threadIdx.x == 0) *cnt = 0u;
+    __syncthreads();
+    unsigned mine = 0u;
+    for (int i = threadIdx.x; i < a.nbl; i += blockDim.x) mine += a.xpoff[i + 1] <= c ? 1u : 0u;
+    if (mine) atomicAdd(cnt, mine);
+    __syncthreads();
+    const int p = static_cast<int>(*cnt);
+    __syncthreads();  // every lane has read the word before LDS is refilled
+    return p;
+}
+
+// Loads issued per lane before any is used in the LDS range fills below: a
+// fill then waits for one round trip (two for terminal leaves, whose column
+// is loaded first) per kFillBatch values a lane writes, not per value.
+constexpr int kFillBatch = 8;
+
 // Gather from LDS: list p holds the symbols whose bits lie in range p
 // ([p * a.bit_range, (p + 1) * a.bit_range), 64 KiB by default). Workgroup w
 // takes the w-th equal share of all lists' chunks; for each list its share
@@ -1180,16 +1202,29 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
     const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
     const long N = a.xpoff[a.nbl];
     const long g0 = N * blockIdx.x / gridDim.x, g1 = N * (blockIdx.x + 1) / gridDim.x;
-    int p = 0;
+    int p = g0 < g1 ? list_of_chunk(a, g0, lds) : 0;
     for (long c0 = g0; c0 < g1;) {
         while (a.xpoff[p + 1] <= c0) ++p;  // ends below nbl, since c0 < N
         const long c1 = a.xpoff[p + 1] < g1 ? static_cast<long>(a.xpoff[p + 1]) : g1;
         __syncthreads();  // the previous range's lookups are done
         const long w0 = static_cast<long>(p) * (range / 64);
-        for (int i = threadIdx.x; i < range / 64; i += blockDim.x) {
-            const unsigned long long v = w0 + i < a.nvw ? a.vbits[w0 + i] : 0ull;
-            lds[2 * i] = static_cast<unsigned>(v);
-            lds[2 * i + 1] = static_cast<unsigned>(v >> 32);
+        const int nw = range / 64;
+        for (int i0 = threadIdx.x; i0 < nw; i0 += kFillBatch * blockDim.x) {
+            unsigned long long v[kFillBatch];
+#pragma unroll
+            for (int u = 0; u < kFillBatch; ++u) {
+                const long w = w0 + i0 + static_cast<long>(u) * blockDim.x;
+                v[u] = a.vbits[w < a.nvw ? w : a.nvw - 1];  // clamped: unconditional
+            }
+#pragma unroll
+            for (int u = 0; u < kFillBatch; ++u) {
+                const int i = i0 + u * static_cast<int>(blockDim.x);
+                if (i < nw) {
+                    const unsigned long long x = w0 + i < a.nvw ? v[u] : 0ull;
+                    lds[2 * i] = static_cast<unsigned>(x);
+                    lds[2 * i + 1] = static_cast<unsigned>(x >> 32);
+                }
+            }
         }
         __syncthreads();
         // Every load is unconditional (chunk indices clamped into the segment,
@@ -1280,18 +1315,38 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
     const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
     const long N = a.xpoff[a.nbl];
     const long g0 = N * blockIdx.x / gridDim.x, g1 = N * (blockIdx.x + 1) / gridDim.x;
-    int p = 0;
+    int p = g0 < g1 ? list_of_chunk(a, g0, reinterpret_cast<unsigned*>(dyn_lds)) : 0;
+    const long U = a.U, nw = a.nkey - a.U;  // terminal keys; rule values held in W
     for (long c0 = g0; c0 < g1;) {
         while (a.xpoff[p + 1] <= c0) ++p;  // ends below nbl, since c0 < N
         const long c1 = a.xpoff[p + 1] < g1 ? static_cast<long>(a.xpoff[p + 1]) : g1;
         __syncthreads();  // the previous range's lookups are done
         const long k0 = static_cast<long>(p) * span;
-        for (int i = threadIdx.x; i < span; i += blockDim.x) {
-            const long k = k0 + i;
-            // terminal values are computed here (the init phase computed only
-            // those rules read); terminal 0 is the semiring zero
-            lds[i] = k < a.U ? (k == 0 ? S::zero() : S::leaf(a.tval[k], a.X[static_cast<long>(a.tcol[k]) * a.B]))
-                             : (k < a.nkey ? W[k - a.U] : S::zero());
+        // Terminal values are computed here (the init phase computed only those
+        // rules read; terminal 0 is the semiring zero), rule values copied from
+        // W. Every load is clamped into its array, so none is conditional.
+        for (int i0 = threadIdx.x; i0 < span; i0 += kFillBatch * blockDim.x) {
+            float tv[kFillBatch];
+            int tc[kFillBatch];
+            T wv[kFillBatch];
+#pragma unroll
+            for (int u = 0; u < kFillBatch; ++u) {
+                const long k = k0 + i0 + static_cast<long>(u) * blockDim.x;
+                const long kt = k < U ? k : U - 1;
+                tv[u] = a.tval[kt];
+                tc[u] = a.tcol[kt];
+                const long kw = k - U;
+                wv[u] = nw > 0 ? W[kw < 0 ? 0 : (kw < nw ? kw : nw - 1)] : S::zero();
+            }
+            float xv[kFillBatch];
+#pragma unroll
+            for (int u = 0; u < kFillBatch; ++u) xv[u] = a.X[static_cast<long>(tc[u]) * a.B];
+#pragma unroll
+            for (int u = 0; u < kFillBatch; ++u) {
+                const int i = i0 + u * static_cast<int>(blockDim.x);
+                const long k = k0 + i;
+                if (i < span) lds[i] = k < U ? (k == 0 ? S::zero() : S::leaf(tv[u], xv[u])) : (k < a.nkey ? wv[u] : S::zero());
+            }
         }
         __syncthreads();
         // As in the bit-mode gather: unconditional, clamped loads, and the next

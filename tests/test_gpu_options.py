@@ -97,7 +97,8 @@ def test_lds_range_gather(hap, geno, sr):
 def test_lds_range_gather_unpacked_rows():
     """Entries carry their row as a 16-bit offset from their chunk's first row;
     a chunk spanning 65,536 rows or more (here: three non-zeros in 150,000 rows)
-    keeps the row table instead. Both forms against the oracle."""
+    keeps the row table instead. Both forms against the oracle, and the
+    bit-mode gather over the same lists (one chunk spanning every row)."""
     rows, cols = 150000, 8
     M = np.zeros((rows, cols))
     M[0, 0], M[100000, 1], M[149999, 2], M[149999, 0] = 2.0, 3.0, 5.0, 7.0
@@ -111,6 +112,44 @@ def test_lds_range_gather_unpacked_rows():
         for ch in (2, 4):
             y = run(eng, sr, x, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=4096, chunks=ch)
             check(sr, y, y_star, den, f"sparse rows lds chunks={ch}")
+    xb = x_for(1, cols, 993)
+    yb_star, _ = O.brute_force(1, rows, cols, seq, V, xb)
+    for rng in (131072, 1048576):
+        y = run(eng, 1, xb, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS, lds_values=rng, chunks=2)
+        check(1, y, yb_star, None, f"sparse rows bits range={rng}")
+
+
+def test_graph_cache_keys_on_every_level_chunk_option():
+    """A captured graph bakes in the range size, the chunks per wave and the
+    EV_LDS window: runs on the same buffers that differ only there (8,192 and
+    40,960 values share their lowest set bit) each capture a graph of their
+    own and replay it."""
+    m = gmr.Matrix.haplotype(20261204, 4096, 2048)
+    seq, V = m.arrays()
+    eng = m.repair().schedule(4096, 2048, "resident").engine()  # no graphs captured yet
+    x = x_for(0, 2048, 77)
+    y_star, den = O.brute_force(0, 4096, 2048, seq, V, x)
+    xt = torch.from_numpy(x).cuda()
+    Y = torch.empty(4096, device="cuda")
+    opts = [dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=8192, chunks=2),
+            dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=40960, chunks=2),
+            dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=8192, chunks=4),
+            dict(eval_kind=gmr.EVAL_LDS, lds_values=2048),
+            dict(eval_kind=gmr.EVAL_LDS, lds_values=4096)]
+    assert eng.debug()["graphs"] == 0
+    ran = 0
+    for rep in range(2):
+        for kw in opts:
+            Y.fill_(12345.0)
+            try:
+                eng.run_torch(0, xt, Y, variant=gmr.VARIANT_GRAPH, **kw)
+            except gmr.GmrError as e:  # 160 KiB ranges where a workgroup may not hold that much LDS
+                assert kw.get("lds_values") == 40960 and "fit the LDS" in str(e), str(e)
+                continue
+            torch.cuda.synchronize()
+            ran += rep == 0
+            check(0, Y.cpu().numpy(), y_star, den, f"graph {kw} replay {rep}")
+        assert eng.debug()["graphs"] == ran, "one captured graph per distinct option set"
 
 
 def test_level_chunk_options(hap, geno):

@@ -87,6 +87,37 @@ def test_readme_example_on_gpu():
         np.testing.assert_allclose(y, f["expect_y"], rtol=1e-6)
 
 
+def test_integration_matrix_on_gpu():
+    """The reference's CI matrix (tests/run_integration_tests.py:110-128: 50 x 50,
+    rows drawn from 5 patterns over {0, 1, 2}), compressed by this package's
+    RePair, through every layout, launch variant and emission, the single-vector
+    gathers (row segments, LDS ranges, bit mode) and batches, for the three
+    semirings, against the oracle."""
+    f = fixture("integration_50x50")
+    rows, cols = f["rows"], f["cols"]
+    m = gmr.Matrix.from_dense(np.array(f["M"]))
+    g = m.repair()
+    seq, V = m.arrays()
+    for sr in (0, 1, 2):
+        x = x_for(sr, cols, 4200 + sr)
+        y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
+        X = x_for(sr, cols, 4210 + sr, B=5)
+        Y_star, Den = O.brute_force_batched(sr, rows, cols, seq, V, X)
+        for lay in ("refcompat", "wave64", "resident"):
+            eng = g.schedule(rows, cols, lay).engine()
+            for var, em in (RES_CASES if lay == "resident" else CASES):
+                _say("ci50", sr, lay, var, em)
+                kw = dict(variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
+                check(sr, run_engine(eng, sr, x, **kw), y_star, den, f"ci50 {lay}/{var}/{em}")
+                if lay != "refcompat":
+                    check(sr, run_engine(eng, sr, X, **kw), Y_star, Den, f"ci50 B=5 {lay}/{var}/{em}")
+        eng = g.schedule(rows, cols, "resident").engine()
+        for kw in (dict(flags=gmr.RUN_ROW_SEGMENTS), dict(flags=gmr.RUN_LDS_RANGES, lds_values=4096)):
+            check(sr, run_engine(eng, sr, x, emit=gmr.EMIT_GATHER, **kw), y_star, den, f"ci50 {kw}")
+        if sr == 1:
+            check(sr, run_engine(eng, sr, x, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS), y_star, den, "ci50 bits")
+
+
 @pytest.mark.parametrize("sr", [0, 1, 2])
 @pytest.mark.parametrize("lay", ["refcompat", "wave64", "resident"])
 def test_haplotype_single_vector(hap, sr, lay):
