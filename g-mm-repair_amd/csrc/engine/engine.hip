@@ -884,7 +884,7 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
         if (!ok) throw std::invalid_argument(lds_gather ? "chunks must be 0, 2, 4 or 8 for the LDS-range gathers"
                                                         : "chunks must be 0, 1, 2 or 4");
     }
-    a.chunks = o.chunks > 0 ? o.chunks : (lds_gather ? 4 : dev::kPartChunks);
+    a.chunks = o.chunks > 0 ? o.chunks : ((o.flags & GMR_RUN_BITS) ? 8 : lds_gather ? 4 : dev::kPartChunks);
     // EV_LDS window of the per-level launches (lds_values, unless it sizes an LDS-range gather)
     a.lds_window = dev::kLdsWindow;
     if (!lds_gather && o.lds_values > 0) {

@@ -1173,6 +1173,13 @@ __device__ __forceinline__ int list_of_chunk(const SweepArgs& a, long c, unsigne
     return p;
 }
 
+// Lane j's value of v, as a wave-uniform value.
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int j) {
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), j));
+    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), j));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
 // Loads issued per lane before any is used in the LDS range fills below: a
 // fill then waits for one round trip (two for terminal leaves, whose column
 // is loaded first) per kFillBatch values a lane writes, not per value.
@@ -1188,9 +1195,13 @@ constexpr int kFillBatch = 8;
 constexpr int kBitRangeBits = 1 << GMR_BIT_RANGE_LOG2;
 constexpr int kBitGatherBlock = GMR_BIT_BLOCK;
 constexpr int kBitWgPerCu = GMR_BIT_WG_PER_CU;
+// Waves per SIMD of kBitWgPerCu workgroups per CU (two of 1,024 lanes: eight,
+// which leaves 64 VGPRs a lane). The kernel is held to that register count, so
+// eight chunks per wave in flight still run that many workgroups per CU.
+constexpr int kBitWavesPerSimd = kBitWgPerCu * kBitGatherBlock / 64 / 4;
 
 template <int K>
-__global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a) {
+__global__ __launch_bounds__(kBitGatherBlock, kBitWavesPerSimd) void k_bits_gather_lds(SweepArgs a) {
     unsigned* lds = reinterpret_cast<unsigned*>(dyn_lds);
     const int range = a.bit_range;
     const int lane = threadIdx.x & 63;
@@ -1236,25 +1247,36 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
         const long step = static_cast<long>(nwave) * K;
         long c = c0 + static_cast<long>(wave) * K;
         unsigned l[K];
-        unsigned long long fb[K];
-        int ord[K];
-        auto load_chunks = [&](long cc, unsigned* lo, unsigned long long* fo, int* oo) {
+        // lane j < K holds chunk c + j's row-start word and row order (one load
+        // for the K chunks; read out per chunk where they are used)
+        unsigned long long fw;
+        int ow;
+        auto load_chunks = [&](long cc) {
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 const long cj = cc + j < c1 ? cc + j : c1 - 1;
 #if GMR_BIT_NT
-                lo[j] = __builtin_nontemporal_load(loc + cj * 64 + lane);
+                l[j] = __builtin_nontemporal_load(loc + cj * 64 + lane);
 #else
-                lo[j] = loc[cj * 64 + lane];
+                l[j] = loc[cj * 64 + lane];
 #endif
-                fo[j] = a.xbits[cj];
-                oo[j] = a.xord[cj];
             }
+            const long cl = cc + (lane < K ? lane : 0);
+            const long cw = cl < c1 ? cl : c1 - 1;
+            fw = a.xbits[cw];
+            ow = a.xord[cw];
         };
-        load_chunks(c, l, fb, ord);  // (past the segment: its last chunk again, unused)
+        load_chunks(c);  // (past the segment: its last chunk again, unused)
         while (c < c1) {
             bool hit[K];
             int ri[K];
+            unsigned long long fb[K];
+            int ord[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                fb[j] = readlane64(fw, j);
+                ord[j] = __builtin_amdgcn_readlane(ow, j);
+            }
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 const bool in_range = l[j] < static_cast<unsigned>(range);
@@ -1271,7 +1293,7 @@ __global__ __launch_bounds__(kBitGatherBlock) void k_bits_gather_lds(SweepArgs a
 #pragma unroll
             for (int j = 0; j < K; ++j) row[j] = a.xrow[ri[j]];
             const long cn = c + step;
-            load_chunks(cn, l, fb, ord);  // unconditional: a branch here would make the stores wait for it
+            load_chunks(cn);  // unconditional: a branch here would make the stores wait for it
             // The row ids are used here, after the prefetch, so the compiler cannot
             // sink their loads into the stores' branches (where the wait for them
             // would also wait for the prefetch). An empty statement: no code.
@@ -1354,22 +1376,33 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
         const long step = static_cast<long>(nwave) * K;
         long c = c0 + static_cast<long>(wave) * K;
         unsigned l[K];
-        unsigned long long fb[K];
-        int ord[K];
+        // lane j < K holds chunk c + j's row-start word and first row (or row
+        // order): one load for the K chunks, read out per chunk where used
+        unsigned long long fw;
+        int ow;
         auto load_chunks = [&](long cc) {
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 const long cj = cc + j < c1 ? cc + j : c1 - 1;
                 l[j] = __builtin_nontemporal_load(loc + cj * 64 + lane);
-                fb[j] = a.xbits[cj];
-                ord[j] = PACKED ? a.cbase[cj] : a.xord[cj];
             }
+            const long cl = cc + (lane < K ? lane : 0);
+            const long cw = cl < c1 ? cl : c1 - 1;
+            fw = a.xbits[cw];
+            ow = PACKED ? a.cbase[cw] : a.xord[cw];
         };
         load_chunks(c);
         while (c < c1) {
             T v[K];
             bool end[K];
             int ri[K];
+            unsigned long long fb[K];
+            int ord[K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                fb[j] = readlane64(fw, j);
+                ord[j] = __builtin_amdgcn_readlane(ow, j);
+            }
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 const unsigned li = PACKED ? (l[j] & 0xFFFFu) : l[j];

@@ -73,7 +73,8 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
                                         out.append(dict(base, lds_values=span, chunks=ch))
                             elif fl == RUN_BITS:
                                 for rng in (262144, 524288, 1048576):
-                                    out.append(dict(base, lds_values=rng))
+                                    for ch in (4, 8):
+                                        out.append(dict(base, lds_values=rng, chunks=ch))
                             else:
                                 out.append(base)
     if B == 1:

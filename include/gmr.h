@@ -90,8 +90,8 @@ typedef struct gmr_run_opts {
      *             131072..1048576); otherwise the left-child window of GMR_EVAL_LDS levels in the
      *             levels and graph variants (default 4096, 1024..16384)
      * chunks      64-symbol chunks a wave of a chunked single-vector gather has in flight:
-     *             the XCD-partitioned gather 1, 2 or 4 (default 2), GMR_RUN_LDS_RANGES and
-     *             GMR_RUN_BITS 2, 4 or 8 (default 4) */
+     *             the XCD-partitioned gather 1, 2 or 4 (default 2), GMR_RUN_LDS_RANGES 2, 4
+     *             or 8 (default 4), GMR_RUN_BITS 2, 4 or 8 (default 8) */
     int lds_values;
     int chunks;
 } gmr_run_opts;

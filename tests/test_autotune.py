@@ -99,8 +99,9 @@ def test_bit_mode_candidates_only_for_boolean_single_vectors():
     bits = {x for x in d if "bits" in x}
     for v in ("levels", "graph"):
         for r in (262144, 524288, 1048576):
-            assert f"{v}/gather/tile=B/blk=256/bits(range={r})" in bits
-    assert len(bits) == 6
+            for ch in (4, 8):
+                assert f"{v}/gather/tile=B/blk=256/bits(range={r})/chunks={ch}" in bits
+    assert len(bits) == 12
     for B, sr, lay in ((1, gmr.PLUS_TIMES, gmr.RESIDENT), (1, gmr.TROPICAL, gmr.RESIDENT), (4, gmr.BOOLEAN, gmr.RESIDENT),
                        (1, gmr.BOOLEAN, gmr.WAVE64), (1, None, gmr.RESIDENT)):
         assert not any(c.get("flags", 0) & gmr.RUN_BITS for c in autotune.candidates(B, lay, semiring=sr))
