@@ -1210,7 +1210,6 @@ __global__ __launch_bounds__(kBitGatherBlock, kBitWavesPerSimd) void k_bits_gath
     const int nwave = blockDim.x >> 6;
     int* Y = static_cast<int*>(a.Y);
     const unsigned* loc = reinterpret_cast<const unsigned*>(a.xsym);  // bit index inside the list's range
-    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
     const long N = a.xpoff[a.nbl];
     const long g0 = N * blockIdx.x / gridDim.x, g1 = N * (blockIdx.x + 1) / gridDim.x;
     int p = g0 < g1 ? list_of_chunk(a, g0, lds) : 0;
@@ -1283,11 +1282,22 @@ __global__ __launch_bounds__(kBitGatherBlock, kBitWavesPerSimd) void k_bits_gath
                 const unsigned li = in_range ? l[j] : 0u;
                 const bool b = (((lds[li >> 5] >> (li & 31)) & 1u) & (in_range ? 1u : 0u)) != 0u;
                 const unsigned long long m = __ballot(b);
-                const unsigned long long ends = (fb[j] >> 1) | (1ull << 63);  // lane l ends a row piece
-                const unsigned long long st = fb[j] & upto;
-                const int first = st ? 63 - __clzll(st) : 0;  // the piece's first lane
-                hit[j] = c + j < c1 && ((ends >> lane) & 1ull) != 0ull && ((m & upto) >> first) != 0ull;
-                ri[j] = hit[j] ? ord[j] - 1 + __popcll(st) : 0;
+                // Which row pieces hold a 1, for the whole wave in a few scalar
+                // operations: add m to x = ~starts | m, so a carry starts at
+                // every 1 and runs up to the next piece start that is not a 1;
+                // the carry into lane l then says whether l's piece holds a 1
+                // below l. A piece ending at l holds one if m has l, or if l is
+                // not a start and that carry reaches it.
+                const unsigned long long S = fb[j];
+                const unsigned long long x = ~S | m;
+                const unsigned long long cin = (x + m) ^ x ^ m;
+                const unsigned long long ends = (S >> 1) | (1ull << 63);  // lane l ends a row piece
+                const unsigned long long H = c + j < c1 ? (m | (cin & ~S)) & ends : 0ull;
+                hit[j] = ((H >> lane) & 1ull) != 0ull;
+                // the piece's row: its order is that of the last start at or below l
+                const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                    static_cast<unsigned>(S >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(S), 0u)));
+                ri[j] = hit[j] ? ord[j] - 1 + below + static_cast<int>((S >> lane) & 1ull) : 0;
             }
             int row[K];
 #pragma unroll
