@@ -241,6 +241,35 @@ void Engine::upload(int rows_, int cols_, int U_, int num_rules, const LayeredRu
         tcol[q] = c;
         tval[q] = V[v];
     }
+    // Renumber the compact terminals in column order (terminal 0 stays first;
+    // a column's terminals keep their order): consecutive terminals then read
+    // neighbouring x entries, so the LDS-range fills and the per-terminal leaf
+    // computations read x in runs instead of scattered. Only the engine's own
+    // arrays hold compact terminal numbers, and all of them are renumbered here.
+    {
+        std::vector<int> coff(static_cast<std::size_t>(cols) + 1, 0), pos(static_cast<std::size_t>(U), 0);
+        for (int q = 1; q < U; ++q) ++coff[static_cast<std::size_t>(tcol[q]) + 1];
+        coff[0] = 1;
+        for (int c = 0; c < cols; ++c) coff[static_cast<std::size_t>(c) + 1] += coff[static_cast<std::size_t>(c)];
+        bool moved = false;
+        for (int q = 1; q < U; ++q) {
+            pos[q] = coff[static_cast<std::size_t>(tcol[q])]++;
+            moved |= pos[q] != q;
+        }
+        if (moved) {
+            std::vector<int> tc2(static_cast<std::size_t>(U), 0);
+            std::vector<float> tv2(static_cast<std::size_t>(U), 0.0f);
+            for (int q = 1; q < U; ++q) {
+                tc2[static_cast<std::size_t>(pos[q])] = tcol[q];
+                tv2[static_cast<std::size_t>(pos[q])] = tval[q];
+            }
+            tcol.swap(tc2);
+            tval.swap(tv2);
+            auto rn = [&](int c) { return c < 0 ? -pos[static_cast<std::size_t>(-c - 1)] - 1 : c; };
+            for (int2& k : kids) k = make_int2(rn(k.x), rn(k.y));
+            for (int2& e : te) e.x = pos[static_cast<std::size_t>(e.x)];
+        }
+    }
 
     // Resident layout: the row-ordered top sequence for gather emission.
     std::vector<int> crow, csym;

@@ -1183,7 +1183,7 @@ __device__ __forceinline__ unsigned long long readlane64(unsigned long long v, i
 // Loads issued per lane before any is used in the LDS range fills below: a
 // fill then waits for one round trip (two for terminal leaves, whose column
 // is loaded first) per kFillBatch values a lane writes, not per value.
-constexpr int kFillBatch = 8;
+constexpr int kFillBatch = 16;
 
 // Gather from LDS: list p holds the symbols whose bits lie in range p
 // ([p * a.bit_range, (p + 1) * a.bit_range), 64 KiB by default). Workgroup w
@@ -1330,6 +1330,41 @@ __global__ __launch_bounds__(kBitGatherBlock, kBitWavesPerSimd) void k_bits_gath
 // reaches the L1/L2 caches; the cost moves to one atomic per row piece.
 constexpr int kLdsGatherBlock = 1024;
 
+// Fills lds[0, span) with the values of keys k0 .. k0 + span - 1 of a
+// one-vector LDS range: terminal leaves computed from their value and x
+// (terminal 0 is the semiring zero), rule values copied from W, zero past the
+// last key. Every load is clamped into its array, so none is conditional, and
+// a lane issues kFillBatch of each before using any.
+template <class S>
+__device__ __forceinline__ void fill_range(const SweepArgs& a, typename S::T* lds, long k0, int span) {
+    using T = typename S::T;
+    const T* W = tp<S>(a.W0);
+    const long U = a.U, nw = a.nkey - a.U;  // terminal keys; rule values held in W
+    for (int i0 = threadIdx.x; i0 < span; i0 += kFillBatch * blockDim.x) {
+        float tv[kFillBatch];
+        int tc[kFillBatch];
+        T wv[kFillBatch];
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) {
+            const long k = k0 + i0 + static_cast<long>(u) * blockDim.x;
+            const long kt = k < U ? k : U - 1;
+            tv[u] = a.tval[kt];
+            tc[u] = a.tcol[kt];
+            const long kw = k - U;
+            wv[u] = nw > 0 ? W[kw < 0 ? 0 : (kw < nw ? kw : nw - 1)] : S::zero();
+        }
+        float xv[kFillBatch];
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) xv[u] = a.X[static_cast<long>(tc[u]) * a.B];
+#pragma unroll
+        for (int u = 0; u < kFillBatch; ++u) {
+            const int i = i0 + u * static_cast<int>(blockDim.x);
+            const long k = k0 + i;
+            if (i < span) lds[i] = k < U ? (k == 0 ? S::zero() : S::leaf(tv[u], xv[u])) : (k < a.nkey ? wv[u] : S::zero());
+        }
+    }
+}
+
 // PACKED: every entry carries its row as an offset from its chunk's first row
 // (the high 16 bits; the index inside the range is the low 16 bits), so the
 // row of a piece comes from the entry that ends it, with no dependent load.
@@ -1341,45 +1376,18 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int nwave = blockDim.x >> 6;
-    const T* W = tp<S>(a.W0);
     T* Y = tp<S>(a.Y);
     const unsigned* loc = reinterpret_cast<const unsigned*>(a.xsym);  // key index inside the list's range
     const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
     const long N = a.xpoff[a.nbl];
     const long g0 = N * blockIdx.x / gridDim.x, g1 = N * (blockIdx.x + 1) / gridDim.x;
     int p = g0 < g1 ? list_of_chunk(a, g0, reinterpret_cast<unsigned*>(dyn_lds)) : 0;
-    const long U = a.U, nw = a.nkey - a.U;  // terminal keys; rule values held in W
     for (long c0 = g0; c0 < g1;) {
         while (a.xpoff[p + 1] <= c0) ++p;  // ends below nbl, since c0 < N
         const long c1 = a.xpoff[p + 1] < g1 ? static_cast<long>(a.xpoff[p + 1]) : g1;
         __syncthreads();  // the previous range's lookups are done
         const long k0 = static_cast<long>(p) * span;
-        // Terminal values are computed here (the init phase computed only those
-        // rules read; terminal 0 is the semiring zero), rule values copied from
-        // W. Every load is clamped into its array, so none is conditional.
-        for (int i0 = threadIdx.x; i0 < span; i0 += kFillBatch * blockDim.x) {
-            float tv[kFillBatch];
-            int tc[kFillBatch];
-            T wv[kFillBatch];
-#pragma unroll
-            for (int u = 0; u < kFillBatch; ++u) {
-                const long k = k0 + i0 + static_cast<long>(u) * blockDim.x;
-                const long kt = k < U ? k : U - 1;
-                tv[u] = a.tval[kt];
-                tc[u] = a.tcol[kt];
-                const long kw = k - U;
-                wv[u] = nw > 0 ? W[kw < 0 ? 0 : (kw < nw ? kw : nw - 1)] : S::zero();
-            }
-            float xv[kFillBatch];
-#pragma unroll
-            for (int u = 0; u < kFillBatch; ++u) xv[u] = a.X[static_cast<long>(tc[u]) * a.B];
-#pragma unroll
-            for (int u = 0; u < kFillBatch; ++u) {
-                const int i = i0 + u * static_cast<int>(blockDim.x);
-                const long k = k0 + i;
-                if (i < span) lds[i] = k < U ? (k == 0 ? S::zero() : S::leaf(tv[u], xv[u])) : (k < a.nkey ? wv[u] : S::zero());
-            }
-        }
+        fill_range<S>(a, lds, k0, span);  // (the init phase computed only the terminal values rules read)
         __syncthreads();
         // As in the bit-mode gather: unconditional, clamped loads, and the next
         // iteration's chunk loads issued before this one's emits.
@@ -1422,14 +1430,21 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
             for (int j = 0; j < K; ++j) {
                 if ((PACKED ? (l[j] & 0xFFFFu) : l[j]) >= static_cast<unsigned>(span)) v[j] = S::zero();  // padding
                 const unsigned long long bits = fb[j];
+                // Segmented scan: at step d a lane at distance >= d from its
+                // piece's first lane takes the partial of the lane d below it.
+                // (Every step runs, unrolled, so the K chunks' steps interleave.)
+                const unsigned long long st = bits & upto;
+                const int dist = lane - (st ? 63 - __clzll(st) : 0);
 #pragma unroll
                 for (int d = 1; d < 64; d <<= 1) {
                     const T t = __shfl_up(v[j], d, 64);
-                    if (lane >= d && ((bits >> (lane - d + 1)) & ((1ull << d) - 1ull)) == 0ull) v[j] = S::oplus(v[j], t);
+                    if (dist >= d) v[j] = S::oplus(v[j], t);
                 }
                 const bool ends = lane == 63 || ((bits >> (lane + 1)) & 1ull) != 0ull;
                 end[j] = c + j < c1 && ends && !S::skip(v[j]);
-                ri[j] = end[j] ? ord[j] - 1 + __popcll(bits & upto) : 0;
+                const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                    static_cast<unsigned>(bits >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bits), 0u)));
+                ri[j] = end[j] ? ord[j] - 1 + below + static_cast<int>((bits >> lane) & 1ull) : 0;
             }
             int row[K];
 #pragma unroll

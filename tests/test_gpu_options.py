@@ -119,6 +119,31 @@ def test_lds_range_gather_unpacked_rows():
         check(1, y, yb_star, None, f"sparse rows bits range={rng}")
 
 
+def test_lds_range_fills_over_renumbered_terminals():
+    """The engine numbers terminals in column order and the range fills compute
+    their values from that numbering: terminals spread thinly over many columns
+    (one non-zero every 150 of 300,000 columns), more than 65,536 distinct
+    values, and a small dense matrix, against the oracle."""
+    rng = np.random.default_rng(77)
+    cases = []
+    M = np.zeros((6, 300000))
+    M[rng.integers(0, 6, 2000), np.arange(2000) * 150] = rng.integers(1, 4, 2000)
+    cases.append(("sparse columns", M))
+    cases.append(("many values", rng.uniform(1.0, 2.0, (8, 9000)).round(7)))
+    cases.append(("dense", rng.integers(0, 3, (500, 3000)).astype(np.float64)))
+    for name, M in cases:
+        m = gmr.Matrix.from_dense(M)
+        seq, V = m.arrays()
+        rows, cols = M.shape
+        eng = m.repair().schedule(rows, cols, "resident").engine()
+        for sr in (0, 1, 2):
+            x = x_for(sr, cols, 1200 + sr)
+            y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
+            _say(name, sr, len(V), "values")
+            y = run(eng, sr, x, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=4096, chunks=4)
+            check(sr, y, y_star, den, f"{name} lds ranges")
+
+
 def test_graph_cache_keys_on_every_level_chunk_option():
     """A captured graph bakes in the range size, the chunks per wave and the
     EV_LDS window: runs on the same buffers that differ only there (8,192 and
