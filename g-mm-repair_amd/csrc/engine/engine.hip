@@ -89,7 +89,7 @@ Engine::~Engine() {
         for (auto& kv : *m) free_lists(kv.second);
     void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_tref, d_emits, d_term, d_crow, d_csym, d_gseg, d_zrows, d_fbits, d_ford,
                     d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_kinds,
-                    static_cast<void*>(ws_bits)};
+                    static_cast<void*>(ws_bits), d_row_new, ws_yrow};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     barrier.release();
@@ -512,16 +512,54 @@ bool pack_entries(PartLists& L, long span, std::vector<int>& cbase) {
     return true;
 }
 
+// A row order that puts similar rows next to each other: rows by their
+// top-sequence length, then by the least hash (splitmix64 finaliser) of their
+// symbols, then by row. Returns new number -> matrix row.
+std::vector<int> similar_row_order(const std::vector<int>& crow, const std::vector<int>& csym, int rows) {
+    std::vector<unsigned long long> mh(static_cast<std::size_t>(rows), ~0ull);
+    for (int r = 0; r < rows; ++r)
+        for (int q = crow[r]; q < crow[r + 1]; ++q) {
+            unsigned long long z = static_cast<unsigned long long>(static_cast<unsigned>(csym[q])) + 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            if (z < mh[r]) mh[r] = z;
+        }
+    std::vector<int> order(static_cast<std::size_t>(rows));
+    for (int r = 0; r < rows; ++r) order[r] = r;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        const int la = crow[a + 1] - crow[a], lb = crow[b + 1] - crow[b];
+        if (la != lb) return la < lb;
+        if (mh[a] != mh[b]) return mh[a] < mh[b];
+        return a < b;
+    });
+    return order;
+}
+
 // Lists by fixed key ranges of `span` keys (one list per range, each entry
 // stored as its key's index inside the range; padding -1), uploaded. With
 // `pack`, entries are packed with their row (pack_entries) where possible.
+// With `row_order` (new number -> matrix row) the lists number rows by it.
 Engine::XParts range_lists(Engine& en, long nkey, long span, const std::function<long(int)>& key,
-                           bool pack = false) {
+                           bool pack = false, const std::vector<int>* row_order = nullptr) {
     const long nsym = en.crow_host[0];
     std::vector<int> crow(static_cast<std::size_t>(en.rows) + 1), csym(static_cast<std::size_t>(nsym));
     GMR_HIP(hipDeviceSynchronize());
     GMR_HIP(hipMemcpy(crow.data(), en.d_crow, sizeof(int) * crow.size(), hipMemcpyDeviceToHost));
     if (nsym) GMR_HIP(hipMemcpy(csym.data(), en.d_csym, sizeof(int) * nsym, hipMemcpyDeviceToHost));
+    if (row_order) {
+        // the same rows, renumbered: new row j holds matrix row (*row_order)[j]
+        std::vector<int> crow2(crow.size()), csym2(csym.size());
+        crow2[0] = 0;
+        for (int j = 0; j < en.rows; ++j) {
+            const int r = (*row_order)[static_cast<std::size_t>(j)];
+            const int n = crow[r + 1] - crow[r];
+            std::copy(csym.begin() + crow[r], csym.begin() + crow[r] + n, csym2.begin() + crow2[j]);
+            crow2[j + 1] = crow2[j] + n;
+        }
+        crow.swap(crow2);
+        csym.swap(csym2);
+    }
     const int P = static_cast<int>(std::max(1L, (nkey + span - 1) / span));
     PartLists L;
     build_part_lists(0, en.rows, crow, csym, key, nkey, P, -1, L, span);
@@ -566,15 +604,31 @@ const Engine::XParts& Engine::bit_lists(int range_bits) {
 }
 
 // Lists of the LDS-range gather of one vector: value keys (terminal t -> t,
-// rule slot s -> U + s) in ranges of `span` values.
-const Engine::XParts& Engine::lds_lists(int span) {
-    auto it = lparts.find(span);
+// rule slot s -> U + s) in ranges of `span` values. With `row_order` the lists
+// number rows by similar_row_order, and the first such call also uploads the
+// matrix row -> new number map and allocates the row-ordered copy of y.
+const Engine::XParts& Engine::lds_lists(int span, bool row_order) {
+    const int mk = 2 * span + (row_order ? 1 : 0);
+    auto it = lparts.find(mk);
     if (it != lparts.end()) return it->second;
     forbid_alloc("the LDS-range gather's lists for this range size");
     if (!resident) throw std::invalid_argument("the LDS-range gather needs the Resident layout");
     const long u = U;
     auto key = [u](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : u + c; };
-    return lparts.emplace(span, range_lists(*this, u + NL, span, key, true)).first->second;
+    if (row_order && !d_row_new) {
+        const long nsym = crow_host[0];
+        std::vector<int> crow(static_cast<std::size_t>(rows) + 1), csym(static_cast<std::size_t>(nsym));
+        GMR_HIP(hipDeviceSynchronize());
+        GMR_HIP(hipMemcpy(crow.data(), d_crow, sizeof(int) * crow.size(), hipMemcpyDeviceToHost));
+        if (nsym) GMR_HIP(hipMemcpy(csym.data(), d_csym, sizeof(int) * nsym, hipMemcpyDeviceToHost));
+        row_order_host = similar_row_order(crow, csym, rows);
+        std::vector<int> inv(static_cast<std::size_t>(rows));
+        for (int j = 0; j < rows; ++j) inv[static_cast<std::size_t>(row_order_host[j])] = j;
+        d_row_new = dev_upload(inv.data(), inv.size(), &struct_bytes);
+        GMR_HIP(hipMalloc(&ws_yrow, sizeof(float) * std::max(rows, 1)));
+    }
+    return lparts.emplace(mk, range_lists(*this, u + NL, span, key, true, row_order ? &row_order_host : nullptr))
+        .first->second;
 }
 
 void Engine::ensure_workspace(int Bt, int vbytes) {
@@ -746,7 +800,7 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         }
     } else if (a.gather && a.lspan) {
         // one LDS range per list; workgroups take equal shares of all chunks
-        const long nch = en.lparts.at(a.lspan).poff.back();
+        const long nch = en.lparts.at(2 * a.lspan + (a.ynew ? 1 : 0)).poff.back();
         const std::size_t lds = static_cast<std::size_t>(a.lspan) * sizeof(typename S::T);
         const unsigned g = lds_gather_grid(en, nch, lds);
         if (nch > 0) {
@@ -761,6 +815,8 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
                 else hipLaunchKernelGGL((dev::k_gather_lds<S, 4, false>), gr, bl, lds, st, a);
             }
         }
+        // rows in the lists' order: y into the caller's Y in matrix row order
+        if (a.ynew) hipLaunchKernelGGL(dev::k_rows_to_matrix_order<S>, dim3(grid_for(a.rows, block)), dim3(block), 0, st, a);
     } else if (a.gather) {
         // (the B-wide row gather takes two segments per lane group at once)
         const long threads = a.flat ? static_cast<long>(en.n_fchunk) * 64
@@ -923,6 +979,8 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
             throw std::invalid_argument("lds_values (the EV_LDS window) must be in [1024, 16384]");
         a.lds_window = o.lds_values;
     }
+    if ((o.flags & GMR_RUN_ROW_ORDER) && !(o.flags & GMR_RUN_LDS_RANGES))
+        throw std::invalid_argument("GMR_RUN_ROW_ORDER orders the rows of GMR_RUN_LDS_RANGES lists; set both");
     if (lds_gather && (!gather || !en.resident || B != 1 || o.variant == GMR_VARIANT_PERSISTENT))
         throw std::invalid_argument(
             "GMR_RUN_BITS and GMR_RUN_LDS_RANGES need one vector, gather emission (Resident layout) and the levels or "
@@ -957,7 +1015,14 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
             throw std::invalid_argument(
                 "lds_values (values per range of GMR_RUN_LDS_RANGES) must be at least 4096 and fit the LDS of a "
                 "workgroup (" + std::to_string(en.max_lds) + " bytes)");
-        const Engine::XParts& x = en.lds_lists(span);
+        const bool ordered = (o.flags & GMR_RUN_ROW_ORDER) != 0;
+        const Engine::XParts& x = en.lds_lists(span, ordered);
+        if (ordered) {
+            // pieces combine in the row-ordered copy, which the init zeroes
+            a.ynew = en.d_row_new;
+            a.Yout = a.Y;
+            a.Y = en.ws_yrow;
+        }
         a.lspan = span;
         a.packed = x.packed ? 1 : 0;
         a.cbase = x.d_cbase;

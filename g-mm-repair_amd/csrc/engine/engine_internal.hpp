@@ -86,8 +86,15 @@ struct Engine {
     std::map<int, XParts> xparts;
     std::map<int, XParts> bparts;    // Boolean bit mode: lists by LDS-sized bit range, per range size
     const XParts& bit_lists(int range_bits);
-    std::map<int, XParts> lparts;    // one vector, GMR_RUN_LDS_RANGES: lists by value range, per range size
-    const XParts& lds_lists(int span);
+    // one vector, GMR_RUN_LDS_RANGES: lists by value range, keyed by 2 x range size + 1 when their
+    // rows follow row_new (GMR_RUN_ROW_ORDER)
+    std::map<int, XParts> lparts;
+    const XParts& lds_lists(int span, bool row_order = false);
+    // GMR_RUN_ROW_ORDER: rows numbered by (top-sequence length, min-hash of the row's symbols);
+    // d_row_new[r] is the number of matrix row r, ws_yrow the row-ordered copy of y (rows values)
+    int* d_row_new = nullptr;
+    void* ws_yrow = nullptr;
+    std::vector<int> row_order_host;  // new number -> matrix row (built with the first ordered lists)
     int max_lds = 65536;             // LDS bytes a workgroup may use
     const XParts& xparts_for(int passes);
     int auto_xcd_passes(int Bs = 1, int Bt = 1) const;  // pass count of the XCD-partitioned gather

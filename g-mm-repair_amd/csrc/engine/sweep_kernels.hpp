@@ -147,6 +147,11 @@ struct SweepArgs {
     // inside the range (0xFFFF: padding); xord / xrow are then not used
     int packed;
     const int* cbase;
+    // LDS-range gather, rows renumbered (GMR_RUN_ROW_ORDER): the lists' rows are
+    // new numbers, pieces combine into Y (then the row-ordered copy) and a last
+    // pass writes Yout[r] = Y[ynew[r]]; nullptr: rows in matrix order
+    const int* ynew;
+    void* Yout;
 };
 
 // Per-level node-update kernels (one is chosen for each level):
@@ -1460,6 +1465,16 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
         }
         c0 = c1;
     }
+}
+
+// Y of the lists' row order into the caller's Y in matrix row order (one vector).
+template <class S>
+__global__ void k_rows_to_matrix_order(SweepArgs a) {
+    using T = typename S::T;
+    const T* Yr = tp<S>(a.Y);
+    T* Y = tp<S>(a.Yout);
+    const long n = a.rows, nth = static_cast<long>(gridDim.x) * blockDim.x;
+    for (long r = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; r < n; r += nth) Y[r] = Yr[a.ynew[r]];
 }
 
 __global__ void k_bits_init(SweepArgs a) { ph_bits_init(a, xcd_tid(), static_cast<long>(gridDim.x) * blockDim.x); }

@@ -11,8 +11,8 @@ import hashlib
 import json
 import os
 
-from . import (BOOLEAN, EMIT_GATHER, EMIT_SPOT, EVAL_LDS, RESIDENT, RUN_BITS, RUN_LDS_RANGES, RUN_ROW_SEGMENTS,
-               RUN_SINGLE_LIST, VARIANT_GRAPH, VARIANT_LEVELS, VARIANT_PERSISTENT, GmrError)
+from . import (BOOLEAN, EMIT_GATHER, EMIT_SPOT, EVAL_LDS, RESIDENT, RUN_BITS, RUN_LDS_RANGES, RUN_ROW_ORDER,
+               RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, VARIANT_GRAPH, VARIANT_LEVELS, VARIANT_PERSISTENT, GmrError)
 
 VARIANT_NAMES = {VARIANT_LEVELS: "levels", VARIANT_GRAPH: "graph", VARIANT_PERSISTENT: "persistent"}
 EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
@@ -28,7 +28,8 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
     vector adds the bit mode to the levels and graph variants), and the
     level-chunk sizes: for one vector, the chunks per wave of the
     XCD-partitioned gather, the LDS-range gather (levels and graph variants)
-    at several range sizes and chunk counts, the bit mode at several range
+    at several range sizes and chunk counts, with the rows in matrix order
+    or renumbered so similar rows sit together, the bit mode at several range
     sizes, and the LDS-staged node update (GMR_EVAL_LDS) at several window
     sizes. Persistent shapes are (block, grid cap); a cap of 0 means the
     engine's default of 16 waves per CU."""
@@ -68,9 +69,11 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
                                             c["chunks"] = ch
                                         out.append(c)
                             elif fl == RUN_LDS_RANGES:
+                                # x rows in matrix order or renumbered (GMR_RUN_ROW_ORDER)
                                 for span in (16384, 32768, 40960):
                                     for ch in (2, 4):
-                                        out.append(dict(base, lds_values=span, chunks=ch))
+                                        for ro in (0, RUN_ROW_ORDER):
+                                            out.append(dict(base, flags=fl | ro, lds_values=span, chunks=ch))
                             elif fl == RUN_BITS:
                                 for rng in (262144, 524288, 1048576):
                                     for ch in (4, 8):
@@ -106,6 +109,7 @@ def describe(c):
             + ("/single-list" if fl & RUN_SINGLE_LIST else "")
             + (f"/bits(range={lds or 'default'})" if fl & RUN_BITS else "")
             + (f"/lds-ranges(values={lds or 'default'})" if fl & RUN_LDS_RANGES else "")
+            + ("/row-order" if fl & RUN_ROW_ORDER else "")
             + (f"/xcd-passes={c['xcd_passes'] or 'auto'}" if "xcd_passes" in c else "")
             + (f"/chunks={c['chunks']}" if c.get("chunks") else "")
             + (f"/eval=lds(window={lds or 'default'})" if c.get("eval_kind") == EVAL_LDS else ""))

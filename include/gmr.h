@@ -68,7 +68,13 @@ enum gmr_run_flags {
      * regrouped by value into ranges of lds_values values, one list per range; a workgroup
      * copies a range of the value array into LDS once and looks its symbols' values up
      * there, and every row piece combines atomically into a zeroed y. */
-    GMR_RUN_LDS_RANGES = 16
+    GMR_RUN_LDS_RANGES = 16,
+    /* With GMR_RUN_LDS_RANGES: the lists number the rows so that rows of equal top-sequence
+     * length and similar symbols sit next to each other (length, then the least hash of the
+     * row's symbols), so the pieces a wave combines land on nearby words of a row-ordered
+     * copy of y that the engine holds (rows x 4 bytes, allocated with the first such lists);
+     * a last pass writes that copy into y in the matrix's row order. */
+    GMR_RUN_ROW_ORDER = 32
 };
 
 typedef struct gmr_run_opts {

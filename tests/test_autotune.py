@@ -109,7 +109,8 @@ def test_bit_mode_candidates_only_for_boolean_single_vectors():
 
 def test_candidates_search_the_level_chunk_sizes():
     """One vector: the XCD-partitioned gather's chunks per wave, the LDS-range
-    gather's range sizes and chunks (levels and graph, every semiring), and
+    gather's range sizes and chunks with and without the row order (levels and
+    graph, every semiring), and
     the LDS-staged node update's windows; several vectors: none of these."""
     for sr in (gmr.PLUS_TIMES, gmr.BOOLEAN, gmr.TROPICAL):
         d = {autotune.describe(c) for c in autotune.candidates(1, gmr.RESIDENT, semiring=sr)}
@@ -117,6 +118,7 @@ def test_candidates_search_the_level_chunk_sizes():
             for span in (16384, 32768, 40960):
                 for ch in (2, 4):
                     assert f"{v}/gather/tile=B/blk=256/lds-ranges(values={span})/chunks={ch}" in d
+                    assert f"{v}/gather/tile=B/blk=256/lds-ranges(values={span})/row-order/chunks={ch}" in d
             for win in (2048, 4096, 16384):
                 assert f"{v}/spot/tile=B/blk=256/eval=lds(window={win})" in d
         for ch in (2, 4):

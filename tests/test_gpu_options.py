@@ -94,6 +94,40 @@ def test_lds_range_gather(hap, geno, sr):
             check(sr, y, y_star, den, f"{name} lds span=40960")
 
 
+@pytest.mark.parametrize("sr", [0, 1, 2])
+def test_lds_range_gather_row_order(hap, geno, sr):
+    """GMR_RUN_ROW_ORDER: the LDS-range lists with their rows renumbered (rows
+    of equal length and similar symbols together), the pieces combined in the
+    engine's row-ordered copy of y and written back in matrix row order. The
+    ordered and the matrix-order lists live side by side on one engine."""
+    for name, (eng, seq, V), rows, cols in (("hap", hap, 4096, 2048), ("geno", geno, 6000, 10000)):
+        x = x_for(sr, cols, 310 + sr)
+        y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
+        for span in (4096, 16384, 32768):
+            for ch in (2, 4):
+                for var in (gmr.VARIANT_LEVELS, gmr.VARIANT_GRAPH):
+                    for fl in (gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER, gmr.RUN_LDS_RANGES):
+                        _say(name, "lds ranges", sr, span, ch, var, fl)
+                        y = run(eng, sr, x, variant=var, emit=gmr.EMIT_GATHER, flags=fl, lds_values=span, chunks=ch)
+                        check(sr, y, y_star, den, f"{name} lds span={span} chunks={ch} var={var} flags={fl}")
+
+
+def test_lds_range_gather_row_order_empty_rows():
+    """Rows with no non-zero get the semiring zero through the row-ordered copy
+    too (every third row of a dense 0/1 matrix emptied)."""
+    rng = np.random.default_rng(91)
+    M = (rng.random((3000, 700)) < 0.05).astype(np.float32)
+    M[::3] = 0.0
+    m = gmr.Matrix.from_dense(M)
+    seq, V = m.arrays()
+    eng = m.repair().schedule(3000, 700, "resident").engine()
+    for sr in (0, 1, 2):
+        x = x_for(sr, 700, 320 + sr)
+        y_star, den = O.brute_force(sr, 3000, 700, seq, V, x)
+        y = run(eng, sr, x, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER, lds_values=4096)
+        check(sr, y, y_star, den, f"empty rows sr={sr}")
+
+
 def test_lds_range_gather_unpacked_rows():
     """Entries carry their row as a 16-bit offset from their chunk's first row;
     a chunk spanning 65,536 rows or more (here: three non-zeros in 150,000 rows)
@@ -159,6 +193,7 @@ def test_graph_cache_keys_on_every_level_chunk_option():
     opts = [dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=8192, chunks=2),
             dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=40960, chunks=2),
             dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=8192, chunks=4),
+            dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER, lds_values=8192, chunks=4),
             dict(eval_kind=gmr.EVAL_LDS, lds_values=2048),
             dict(eval_kind=gmr.EVAL_LDS, lds_values=4096)]
     assert eng.debug()["graphs"] == 0
@@ -211,6 +246,7 @@ def test_option_errors(hap):
            dict(emit=gmr.EMIT_SPOT, flags=gmr.RUN_LDS_RANGES),                              # needs gather
            dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, variant=gmr.VARIANT_PERSISTENT),
            dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_BITS),
+           dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_ROW_ORDER),                             # needs LDS ranges
            dict(emit=gmr.EMIT_GATHER, chunks=8),                                            # XCD gather: 1, 2, 4
            dict(eval_kind=gmr.EVAL_LDS, lds_values=65536)]                                  # window too large
     for kw in bad:

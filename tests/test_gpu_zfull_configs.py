@@ -85,6 +85,8 @@ def test_config2_single_vector(cfg2):
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=16384, chunks=2),
              dict(variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=40960,
                   chunks=4),
+             dict(variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER,
+                  lds_values=40960, chunks=4),
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_SINGLE_LIST),
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_ROW_SEGMENTS),
              dict(emit=gmr.EMIT_SPOT, eval_kind=gmr.EVAL_LDS),
@@ -151,6 +153,7 @@ def test_config4a_boolean_reachability():
                    dict(variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS, lds_values=262144),
                    dict(emit=gmr.EMIT_GATHER),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES),
+                   dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER),
                    dict(emit=gmr.EMIT_SPOT),
                    dict(variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_GATHER)):
             y = eng.run_torch(1, xt, **kw)
@@ -171,6 +174,7 @@ def test_config4b_minplus_relaxation():
     for label, eng in forms:
         for kw in (dict(emit=gmr.EMIT_GATHER),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES),
+                   dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_SINGLE_LIST),
                    dict(emit=gmr.EMIT_SPOT),
                    dict(variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_SPOT)):

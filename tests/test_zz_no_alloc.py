@@ -95,7 +95,8 @@ def test_noalloc_one_vector_gathers_after_setup(hap):
     xt = torch.from_numpy(x).cuda()
     Y = torch.empty(4096, device="cuda")
     for kw in (dict(emit=gmr.EMIT_GATHER),  # XCD-partitioned lists
-               dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=16384)):
+               dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=16384),
+               dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER, lds_values=16384)):
         with pytest.raises(gmr.GmrError) as e:
             eng.run_noalloc_torch(0, xt, Y, **kw)
         assert e.value.code == gmr.ERR_INVALID and "allocated" in str(e.value)
