@@ -89,7 +89,7 @@ Engine::~Engine() {
         for (auto& kv : *m) free_lists(kv.second);
     void* ptrs[] = {d_kids, d_coeff, d_tcol, d_tval, d_tref, d_emits, d_term, d_crow, d_csym, d_gseg, d_zrows, d_fbits, d_ford,
                     d_frow, d_level_off, d_emit_off, ws_T, ws_W0, ws_W1, d_kinds,
-                    static_cast<void*>(ws_bits), d_row_new, ws_yrow};
+                    static_cast<void*>(ws_bits), d_row_new, ws_yrow, ws_tail};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     barrier.release();
@@ -721,6 +721,20 @@ inline unsigned lds_gather_grid(const Engine& en, long nch, std::size_t lds) {
     return static_cast<unsigned>(std::max(1L, std::min<long>(per_cu * en.num_cus, (nch + 63) / 64)));
 }
 
+static_assert(Engine::kTailStageBytes == 2 * dev::kTailRules * 4, "the fused tail's staging buffer");
+
+// First level of the fused tail (GMR_RUN_FUSED_TAIL): the lowest k whose
+// levels k .. L hold at most kTailRules rules, if they are two levels or more;
+// L + 1 (no tail) otherwise or without the flag.
+inline int tail_start(const Engine& en, const SweepArgs& a, const gmr_run_opts& o) {
+    if (!(o.flags & GMR_RUN_FUSED_TAIL) || !a.resident || a.Bs != 1 || a.bits ||
+        en.max_lds < dev::kTailLdsBytes)
+        return en.L + 1;
+    int k = en.L + 1;
+    while (k > 1 && en.level_off[en.L] - en.level_off[k - 2] <= dev::kTailRules) --k;
+    return en.L - k + 1 >= 2 ? k : en.L + 1;
+}
+
 template <class S>
 void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, int block, hipStream_t st,
                    hipEvent_t* level_ev = nullptr) {
@@ -766,11 +780,20 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         long work = (a.Bt == 1 && a.B == 1) ? en.Et : ((static_cast<long>(en.Et) + dev::kEmitChunk - 1) / dev::kEmitChunk) * G;
         hipLaunchKernelGGL(dev::k_emit<S>, dim3(grid_for(work, block)), dim3(block), 0, st, a, 0);
     }
+    const int k_tail = tail_start(en, a, o);
     for (int k = 1; k <= en.L; ++k) {
         const int w = en.level_off[k] - en.level_off[k - 1];
         const int kind = host_kind(o, k);
         if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * k - 1], st));
-        if (w > 0) {
+        if (k == k_tail) {
+            // levels k .. L at once (their time is level k's)
+            auto* stage = static_cast<unsigned*>(a.tail_stage);
+            const long n = en.level_off[en.L] - en.level_off[k - 1];
+            hipLaunchKernelGGL(dev::k_tail_stage<S>, dim3(grid_for(n, block)), dim3(block), 0, st, a, k, stage);
+            hipLaunchKernelGGL(dev::k_eval_tail<S>, dim3(1), dim3(dev::kTailBlock),
+                               static_cast<std::size_t>(dev::kTailLdsBytes), st, a, k,
+                               static_cast<const unsigned*>(stage));
+        } else if (w > 0 && k < k_tail) {
             const bool lds = kind == dev::EV_LDS && a.Bt == 1;
             hipLaunchKernelGGL(dev::k_eval<S>, dim3(eval_grid(a, kind, w, block)), dim3(block),
                                lds ? static_cast<std::size_t>(a.lds_window) * sizeof(typename S::T) : 0, st, a, k, kind);
@@ -979,6 +1002,16 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
             throw std::invalid_argument("lds_values (the EV_LDS window) must be in [1024, 16384]");
         a.lds_window = o.lds_values;
     }
+    if ((o.flags & GMR_RUN_FUSED_TAIL) &&
+        (!en.resident || B != 1 || o.variant == GMR_VARIANT_PERSISTENT || (o.flags & GMR_RUN_BITS)))
+        throw std::invalid_argument(
+            "GMR_RUN_FUSED_TAIL needs one vector, the Resident layout and the levels or graph variant (not the bit "
+            "mode)");
+    if ((o.flags & GMR_RUN_FUSED_TAIL) && !en.ws_tail) {
+        en.forbid_alloc("the fused tail's staging buffer");
+        GMR_HIP(hipMalloc(&en.ws_tail, Engine::kTailStageBytes));
+    }
+    a.tail_stage = en.ws_tail;
     if ((o.flags & GMR_RUN_ROW_ORDER) && !(o.flags & GMR_RUN_LDS_RANGES))
         throw std::invalid_argument("GMR_RUN_ROW_ORDER orders the rows of GMR_RUN_LDS_RANGES lists; set both");
     if (lds_gather && (!gather || !en.resident || B != 1 || o.variant == GMR_VARIANT_PERSISTENT))
@@ -1120,18 +1153,20 @@ void Engine::run_noalloc(int semiring, const float* X, int B, void* Y, const gmr
     const std::size_t tb = up(static_cast<std::size_t>(U) * Bs * 4);
     const std::size_t wn = resident ? static_cast<std::size_t>(NL) : static_cast<std::size_t>(max_width);
     const std::size_t wb = up(std::max<std::size_t>(wn, 1) * Bs * 4);
-    const std::size_t need = tb + wb * (resident ? 1 : 2);
+    const std::size_t sb = (o.flags & GMR_RUN_FUSED_TAIL) ? kTailStageBytes : 0;
+    const std::size_t need = tb + wb * (resident ? 1 : 2) + sb;
     if (need > scratch_bytes)
         throw std::invalid_argument("no-allocation run: the sweep needs " + std::to_string(need) +
                                     " bytes of workspace (vector tile of " + std::to_string(Bt) +
                                     "), more than the static scratch array's " + std::to_string(scratch_bytes) +
                                     "; use a narrower b_tile or gmr_engine_run");
-    void* const saved[3] = {ws_T, ws_W0, ws_W1};
+    void* const saved[4] = {ws_T, ws_W0, ws_W1, ws_tail};
     const std::size_t saved_b[2] = {ws_T_bytes, ws_W_bytes};
     char* base = static_cast<char*>(scratch);
     ws_T = base;
     ws_W0 = base + tb;
     ws_W1 = resident ? nullptr : base + tb + wb;
+    if (sb) ws_tail = base + tb + wb * (resident ? 1 : 2);
     ws_T_bytes = tb;
     ws_W_bytes = wb;
     no_alloc = true;
@@ -1140,6 +1175,7 @@ void Engine::run_noalloc(int semiring, const float* X, int B, void* Y, const gmr
         ws_T = saved[0];
         ws_W0 = saved[1];
         ws_W1 = saved[2];
+        ws_tail = saved[3];
         ws_T_bytes = saved_b[0];
         ws_W_bytes = saved_b[1];
     };

@@ -95,6 +95,10 @@ struct Engine {
     int* d_row_new = nullptr;
     void* ws_yrow = nullptr;
     std::vector<int> row_order_host;  // new number -> matrix row (built with the first ordered lists)
+    // GMR_RUN_FUSED_TAIL: staged child values of the fused top levels (2 x 8,192
+    // values), allocated with the first such run
+    void* ws_tail = nullptr;
+    static constexpr std::size_t kTailStageBytes = 2 * 8192 * 4;
     int max_lds = 65536;             // LDS bytes a workgroup may use
     const XParts& xparts_for(int passes);
     int auto_xcd_passes(int Bs = 1, int Bt = 1) const;  // pass count of the XCD-partitioned gather

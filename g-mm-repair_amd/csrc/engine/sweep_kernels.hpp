@@ -152,6 +152,9 @@ struct SweepArgs {
     // pass writes Yout[r] = Y[ynew[r]]; nullptr: rows in matrix order
     const int* ynew;
     void* Yout;
+    // GMR_RUN_FUSED_TAIL: 2 x kTailRules values staging the tail's children that
+    // lie below it
+    void* tail_stage;
 };
 
 // Per-level node-update kernels (one is chosen for each level):
@@ -1003,6 +1006,88 @@ __global__ void k_eval(SweepArgs a, int k, int kind) {
     long nth = static_cast<long>(gridDim.x) * blockDim.x;
     ph_eval_kind<S>(a, kind, a.level_off[k - 1], a.level_off[k], level_src<S>(a, k), level_dst<S>(a, k), tid,
                     nth, reinterpret_cast<typename S::T*>(dyn_lds), a.lds_window);
+}
+
+// The top levels of a one-vector sweep in one workgroup (Resident layout,
+// GMR_RUN_FUSED_TAIL): levels k0 .. L, at most kTailRules rules together, each
+// a launch of its own otherwise. One workgroup alone would fetch the tail's
+// children from below the tail slowly (one CU's L1 looks up about one line
+// per clock, and these lookups are random), so a staging launch over many
+// workgroups first copies them into stage[] in the tail's rule order; the
+// tail workgroup then reads kids and staged words coalesced into LDS, runs the
+// levels out of LDS with a workgroup barrier between them and writes the
+// tail's values to the resident array. Two launches replace one per level.
+constexpr int kTailBlock = 1024;
+constexpr int kTailRules = 8192;
+constexpr int kTailLdsBytes = kTailRules * 20;  // per rule: its value, kids and two staged words
+
+// The 32-bit word of a value or a coefficient in the staging buffer.
+__device__ __forceinline__ unsigned tail_word(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ unsigned tail_word(int v) { return static_cast<unsigned>(v); }
+template <class T> __device__ __forceinline__ T tail_value(unsigned w);
+template <> __device__ __forceinline__ float tail_value<float>(unsigned w) { return __uint_as_float(w); }
+template <> __device__ __forceinline__ int tail_value<int>(unsigned w) { return static_cast<int>(w); }
+
+// Two words per tail rule i, at 2(i - base): the left one holds the left
+// child's value with the rule's coefficient applied when that child lies below
+// the tail (terminal or lower rule), else the coefficient itself; the right one
+// the right child's value when it lies below the tail (else unused).
+template <class S>
+__global__ void k_tail_stage(SweepArgs a, int k0, unsigned* stage) {
+    using T = typename S::T;
+    const T* Tt = tp<S>(a.Tv);
+    const T* W = tp<S>(a.W0);
+    const int base = a.level_off[k0 - 1], end = a.level_off[a.L];
+    for (long i = base + static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < end;
+         i += static_cast<long>(gridDim.x) * blockDim.x) {
+        const int2 c = a.kids[i];
+        const float cf = a.coeff ? a.coeff[i] : 1.0f;
+        const T l = *(c.x < 0 ? Tt + (-c.x - 1) : W + (c.x < base ? c.x : 0));
+        const T r = *(c.y < 0 ? Tt + (-c.y - 1) : W + (c.y < base ? c.y : 0));
+        stage[2 * (i - base)] = c.x >= base ? __float_as_uint(cf) : tail_word(S::apply(l, cf));
+        stage[2 * (i - base) + 1] = tail_word(r);
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(kTailBlock) void k_eval_tail(SweepArgs a, int k0, const unsigned* stage) {
+    using T = typename S::T;
+    const int base = a.level_off[k0 - 1], end = a.level_off[a.L], n = end - base;
+    T* tv = reinterpret_cast<T*>(dyn_lds);                        // n values
+    int4* meta = reinterpret_cast<int4*>(dyn_lds + 4 * kTailRules);  // n x (left, right, left word, right word)
+    T* W = tp<S>(a.W0);
+    // kids and staged words into LDS: eight rules per lane in flight
+    for (int t0 = 0; t0 < n; t0 += 8 * kTailBlock) {
+        int2 c[8];
+        uint2 w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int t = t0 + u * kTailBlock + static_cast<int>(threadIdx.x);
+            const int tc = t < n ? t : n - 1;
+            c[u] = a.kids[base + tc];
+            w[u] = *reinterpret_cast<const uint2*>(stage + 2 * tc);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int t = t0 + u * kTailBlock + static_cast<int>(threadIdx.x);
+            if (t < n) meta[t] = make_int4(c[u].x, c[u].y, static_cast<int>(w[u].x), static_cast<int>(w[u].y));
+        }
+    }
+    __syncthreads();
+    for (int k = k0; k <= a.L; ++k) {
+        const int lo = a.level_off[k - 1], hi = a.level_off[k];
+        for (int i = lo + static_cast<int>(threadIdx.x); i < hi; i += kTailBlock) {
+            const int4 m = meta[i - base];
+            const T l = tv[m.x >= base ? m.x - base : 0];
+            const T r = tv[m.y >= base ? m.y - base : 0];
+            const unsigned lw = static_cast<unsigned>(m.z), rw = static_cast<unsigned>(m.w);
+            const T lv = m.x >= base ? S::apply(l, __uint_as_float(lw)) : tail_value<T>(lw);
+            const T rv = m.y >= base ? r : tail_value<T>(rw);
+            tv[i - base] = S::oplus(lv, rv);
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < n; t += kTailBlock) W[base + t] = tv[t];
 }
 
 // k = 0: terminal emissions; k >= 1: level k's emissions.

@@ -74,7 +74,14 @@ enum gmr_run_flags {
      * row's symbols), so the pieces a wave combines land on nearby words of a row-ordered
      * copy of y that the engine holds (rows x 4 bytes, allocated with the first such lists);
      * a last pass writes that copy into y in the matrix's row order. */
-    GMR_RUN_ROW_ORDER = 32
+    GMR_RUN_ROW_ORDER = 32,
+    /* One vector, Resident layout, levels or graph variant: the top levels, from the lowest
+     * one whose rules number at most 8,192 together with every level above it (at least two
+     * levels), run in one workgroup that keeps their values in LDS (160 KiB), after a launch
+     * that stages the values of their children below them: two launches instead of one per
+     * level. Without such levels, or where a workgroup may not hold that much LDS, the sweep
+     * runs as without the flag. */
+    GMR_RUN_FUSED_TAIL = 64
 };
 
 typedef struct gmr_run_opts {

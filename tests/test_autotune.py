@@ -123,7 +123,11 @@ def test_candidates_search_the_level_chunk_sizes():
                 assert f"{v}/spot/tile=B/blk=256/eval=lds(window={win})" in d
         for ch in (2, 4):
             assert f"levels/gather/tile=B/blk=256/xcd-passes=2/chunks={ch}" in d
+            assert f"graph/gather/tile=B/blk=256/lds-ranges(values=40960)/row-order/fused-tail/chunks={ch}" in d
+        assert "levels/spot/tile=B/blk=256/fused-tail" in d
+        assert not any("fused-tail" in x for x in d if x.startswith("persistent") or "/bits(" in x)
         assert not any("lds-ranges" in x for x in d if x.startswith("persistent"))
     for B in (4, 64):
         d = {autotune.describe(c) for c in autotune.candidates(B, gmr.RESIDENT)}
-        assert not any("lds" in x or "chunks" in x for x in d)
+        assert not any("lds" in x or "chunks" in x or "fused-tail" in x for x in d)
+    assert not any("fused-tail" in autotune.describe(c) for c in autotune.candidates(1, gmr.WAVE64))

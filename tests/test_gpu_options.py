@@ -128,6 +128,49 @@ def test_lds_range_gather_row_order_empty_rows():
         check(sr, y, y_star, den, f"empty rows sr={sr}")
 
 
+@pytest.mark.parametrize("sr", [0, 1, 2])
+def test_fused_tail(hap, geno, sr):
+    """GMR_RUN_FUSED_TAIL: the top levels with at most 16,384 rules together in
+    one workgroup, with spot and gather emission, the levels and graph variants
+    and the one-vector gathers; on a grammar that small every level is in it."""
+    small = gmr.Matrix.haplotype(20261204, 512, 256, founders=16, seg_mean=32.0)
+    sseq, sV = small.arrays()
+    seng = small.repair().schedule(512, 256, "resident").engine()
+    cases = (("small", (seng, sseq, sV), 512, 256), ("hap", hap, 4096, 2048), ("geno", geno, 6000, 10000))
+    for name, (eng, seq, V), rows, cols in cases:
+        x = x_for(sr, cols, 330 + sr)
+        y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
+        for var in (gmr.VARIANT_LEVELS, gmr.VARIANT_GRAPH):
+            for kw in (dict(emit=gmr.EMIT_SPOT), dict(emit=gmr.EMIT_GATHER),
+                       dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_SINGLE_LIST),
+                       dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=8192),
+                       dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER, lds_values=8192),
+                       dict(emit=gmr.EMIT_SPOT, eval_kind=gmr.EVAL_LDS)):
+                kw = dict(kw, flags=kw.get("flags", 0) | gmr.RUN_FUSED_TAIL)
+                _say(name, "fused tail", sr, var, kw)
+                y = run(eng, sr, x, variant=var, **kw)
+                check(sr, y, y_star, den, f"{name} fused tail var={var} {kw}")
+
+
+def test_fused_tail_refusals(hap):
+    eng, seq, V = hap
+    x = torch.zeros(2048, device="cuda")
+    for kw in (dict(variant=gmr.VARIANT_PERSISTENT, flags=gmr.RUN_FUSED_TAIL),
+               dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_FUSED_TAIL | gmr.RUN_BITS)):
+        with pytest.raises(gmr.GmrError) as e:
+            eng.run_torch(1, x, **kw)
+        assert e.value.code == gmr.ERR_INVALID, kw
+    with pytest.raises(gmr.GmrError) as e:
+        eng.run_torch(0, torch.zeros(2048, 4, device="cuda"), flags=gmr.RUN_FUSED_TAIL)  # one vector only
+    assert e.value.code == gmr.ERR_INVALID
+    m = gmr.Matrix.haplotype(20261204, 512, 256, founders=16, seg_mean=32.0)
+    weng = m.repair().schedule(512, 256, "wave64").engine()
+    with pytest.raises(gmr.GmrError) as e:
+        weng.run_torch(0, torch.zeros(256, device="cuda"), flags=gmr.RUN_FUSED_TAIL)  # Resident layout only
+    assert e.value.code == gmr.ERR_INVALID
+    torch.cuda.synchronize()
+
+
 def test_lds_range_gather_unpacked_rows():
     """Entries carry their row as a 16-bit offset from their chunk's first row;
     a chunk spanning 65,536 rows or more (here: three non-zeros in 150,000 rows)

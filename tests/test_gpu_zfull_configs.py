@@ -87,6 +87,9 @@ def test_config2_single_vector(cfg2):
                   chunks=4),
              dict(variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER,
                   lds_values=40960, chunks=4),
+             dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER | gmr.RUN_FUSED_TAIL,
+                  lds_values=40960, chunks=8),
+             dict(emit=gmr.EMIT_SPOT, flags=gmr.RUN_FUSED_TAIL),
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_SINGLE_LIST),
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_ROW_SEGMENTS),
              dict(emit=gmr.EMIT_SPOT, eval_kind=gmr.EVAL_LDS),
@@ -174,7 +177,7 @@ def test_config4b_minplus_relaxation():
     for label, eng in forms:
         for kw in (dict(emit=gmr.EMIT_GATHER),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES),
-                   dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER),
+                   dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER | gmr.RUN_FUSED_TAIL),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_SINGLE_LIST),
                    dict(emit=gmr.EMIT_SPOT),
                    dict(variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_SPOT)):
