@@ -368,7 +368,7 @@ def main():
     plain_val = nnz / (plain_ms * 1e-3)
     log(f"plain per-level sweep B=1: {plain_ms:.3f} ms, {plain_val / 1e9:.2f} G nnz/s")
     # best single-vector schedule, for reference
-    form1, best1, _ = tune_forms(forms, x1, y1, autotune.candidates(1, gmr.RESIDENT), reps=3)
+    form1, best1, _ = tune_forms(forms, x1, y1, autotune.candidates(1, gmr.RESIDENT), reps=3, log=log)
     one_ms = autotune.time_candidate(form1["engine"], 0, x1, y1, best1, reps=args.steps)
     del eng_ref
 
@@ -630,7 +630,7 @@ def run_sweep_B(args, m, forms):
         Y = torch.empty(args.rows, B, device="cuda")
         if B == 1:
             X, Y = X[:, 0].contiguous(), Y[:, 0].contiguous()
-        form, best, tuned = tune_forms(forms, X, Y, autotune.candidates(B, gmr.RESIDENT), reps=2)
+        form, best, tuned = tune_forms(forms, X, Y, autotune.candidates(B, gmr.RESIDENT), reps=2, log=log)
         eng = form["engine"]
         ms = _time_steps(eng, 0, X, Y, best, args.steps, args.warmup)
         if best["variant"] == gmr.VARIANT_PERSISTENT:
@@ -801,7 +801,7 @@ def run_rmat(args):
                 best["xcd_passes"] = 0
         else:
             form, best, tuned = tune_forms(forms, xt, y, autotune.candidates(1, gmr.RESIDENT, semiring=sr), reps=2,
-                                           semiring=sr)
+                                           semiring=sr, log=log)
         ms = _time_steps(form["engine"], sr, xt, y, best, args.steps, args.warmup)
         # phases of the chosen schedule (levels variant, HIP events around every launch)
         # and the row gather's rate on its algorithmic bytes: 4 per top-sequence symbol

@@ -237,6 +237,39 @@ struct Op4 {
     }
 };
 
+// Segmented inclusive scan of a wave by DPP: lane l ends up with the oplus of
+// lanes [l - dist, l], where dist is its distance to the first lane of its
+// piece (0 at a piece's first lane). Four row_shr steps scan each row of 16
+// lanes; row_bcast:15 then carries lane 15's (47's) partial into rows 1 (3),
+// and row_bcast:31 lane 31's into rows 2 and 3, each taken only by lanes whose
+// piece reaches back that far. Every lane of the wave must be active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xF, false);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+template <class S>
+__device__ __forceinline__ typename S::T seg_scan(typename S::T v, int dist, int lane) {
+    using T = typename S::T;
+    const int r = lane & 15;
+    T t = dpp_move<0x111, 0xF>(v);  // row_shr:1
+    if (dist >= 1 && r >= 1) v = S::oplus(v, t);
+    t = dpp_move<0x112, 0xF>(v);  // row_shr:2
+    if (dist >= 2 && r >= 2) v = S::oplus(v, t);
+    t = dpp_move<0x114, 0xF>(v);  // row_shr:4
+    if (dist >= 4 && r >= 4) v = S::oplus(v, t);
+    t = dpp_move<0x118, 0xF>(v);  // row_shr:8
+    if (dist >= 8 && r >= 8) v = S::oplus(v, t);
+    t = dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    if ((lane & 16) && dist > r) v = S::oplus(v, t);
+    t = dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    if (lane >= 32 && dist > lane - 32) v = S::oplus(v, t);
+    return v;
+}
+
 // Lanes per item for rows of Bs values: one lane per value (vec = 0) or per
 // four values (vec = 1), capped at the wave width.
 __host__ __device__ __forceinline__ int lanes_for(int Bs, int vec) {
@@ -1520,16 +1553,11 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
             for (int j = 0; j < K; ++j) {
                 if ((PACKED ? (l[j] & 0xFFFFu) : l[j]) >= static_cast<unsigned>(span)) v[j] = S::zero();  // padding
                 const unsigned long long bits = fb[j];
-                // Segmented scan: at step d a lane at distance >= d from its
-                // piece's first lane takes the partial of the lane d below it.
-                // (Every step runs, unrolled, so the K chunks' steps interleave.)
+                // Segmented scan over the lanes of each row piece (DPP; the K
+                // chunks' steps interleave).
                 const unsigned long long st = bits & upto;
                 const int dist = lane - (st ? 63 - __clzll(st) : 0);
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const T t = __shfl_up(v[j], d, 64);
-                    if (dist >= d) v[j] = S::oplus(v[j], t);
-                }
+                v[j] = seg_scan<S>(v[j], dist, lane);
                 const bool ends = lane == 63 || ((bits >> (lane + 1)) & 1ull) != 0ull;
                 end[j] = c + j < c1 && ends && !S::skip(v[j]);
                 const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
