@@ -397,9 +397,13 @@ struct PartLists {
     std::vector<unsigned long long> bits;
 };
 
+// With max_rows > 0, a row piece that would start max_rows or more rows after
+// the row of its chunk's first entry starts a new chunk instead (the rest of
+// the chunk is padding, which belongs to the piece before it), so every chunk
+// spans fewer than max_rows rows.
 template <class Off, class Key>
 void build_part_lists(long r0, long r1, const Off& off, const std::vector<int>& sym, Key key, long nkey, int P, int pad,
-                      PartLists& out, long span = 0) {
+                      PartLists& out, long span = 0, long max_rows = 0) {
     const long e0 = off[r0], e1 = off[r1], nent = e1 - e0;
     std::vector<long> cut(P + 1);
     cut[0] = 0;
@@ -420,15 +424,18 @@ void build_part_lists(long r0, long r1, const Off& off, const std::vector<int>& 
     };
     std::vector<std::vector<int>> lsym(P), lrow(P);
     std::vector<std::vector<long>> lstart(P);  // positions (in the list) that start a row
-    std::vector<long> seen(P, -1);
+    std::vector<long> seen(P, -1), row0(P, 0);  // row0: row of the current chunk's first entry
     for (long r = r0; r < r1; ++r) {
         for (long q = off[r]; q < off[r + 1]; ++q) {
             const int p = part(sym[q]);
             if (seen[p] != r) {
                 seen[p] = r;
+                if (max_rows > 0 && (lsym[p].size() & 63) != 0 && r - row0[p] >= max_rows)
+                    lsym[p].resize((lsym[p].size() + 63) & ~static_cast<std::size_t>(63), pad);
                 lstart[p].push_back(static_cast<long>(lsym[p].size()));
                 lrow[p].push_back(static_cast<int>(r));
             }
+            if ((lsym[p].size() & 63) == 0) row0[p] = r;
             lsym[p].push_back(sym[q]);
         }
     }
@@ -562,7 +569,8 @@ Engine::XParts range_lists(Engine& en, long nkey, long span, const std::function
     }
     const int P = static_cast<int>(std::max(1L, (nkey + span - 1) / span));
     PartLists L;
-    build_part_lists(0, en.rows, crow, csym, key, nkey, P, -1, L, span);
+    // packed entries hold a 16-bit row offset: chunks then span fewer than 65,536 rows
+    build_part_lists(0, en.rows, crow, csym, key, nkey, P, -1, L, span, pack ? 65536 : 0);
     // Entries become indices inside their list's range; padding (-1) becomes an
     // index past every range (it reads the semiring zero).
     for (int p = 0; p < P; ++p)

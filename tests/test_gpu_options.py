@@ -172,10 +172,11 @@ def test_fused_tail_refusals(hap):
 
 
 def test_lds_range_gather_unpacked_rows():
-    """Entries carry their row as a 16-bit offset from their chunk's first row;
-    a chunk spanning 65,536 rows or more (here: three non-zeros in 150,000 rows)
-    keeps the row table instead. Both forms against the oracle, and the
-    bit-mode gather over the same lists (one chunk spanning every row)."""
+    """Entries carry their row as a 16-bit offset from their chunk's first row,
+    so a row piece that starts 65,536 rows or more after its chunk's first row
+    starts a new chunk (here: three non-zeros in 150,000 rows, with and without
+    the row order), against the oracle; and the bit-mode gather over the same
+    rows (one chunk spanning every row)."""
     rows, cols = 150000, 8
     M = np.zeros((rows, cols))
     M[0, 0], M[100000, 1], M[149999, 2], M[149999, 0] = 2.0, 3.0, 5.0, 7.0
@@ -187,8 +188,9 @@ def test_lds_range_gather_unpacked_rows():
         x = x_for(sr, cols, 990 + sr)
         y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
         for ch in (2, 4):
-            y = run(eng, sr, x, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=4096, chunks=ch)
-            check(sr, y, y_star, den, f"sparse rows lds chunks={ch}")
+            for fl in (gmr.RUN_LDS_RANGES, gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER):
+                y = run(eng, sr, x, emit=gmr.EMIT_GATHER, flags=fl, lds_values=4096, chunks=ch)
+                check(sr, y, y_star, den, f"sparse rows lds chunks={ch} flags={fl}")
     xb = x_for(1, cols, 993)
     yb_star, _ = O.brute_force(1, rows, cols, seq, V, xb)
     for rng in (131072, 1048576):
