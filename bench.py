@@ -370,6 +370,25 @@ def main():
     # best single-vector schedule, for reference
     form1, best1, _ = tune_forms(forms, x1, y1, autotune.candidates(1, gmr.RESIDENT), reps=3, log=log)
     one_ms = autotune.time_candidate(form1["engine"], 0, x1, y1, best1, reps=args.steps)
+    # its row gather against HBM peak: algorithmic bytes 4 per top-sequence symbol
+    # read + 4 per y row written (+ 4 per value an LDS range fill reads), over the
+    # gather launch's own HIP-event time (levels variant, same options)
+    one_roof = None
+    if best1.get("emit") == gmr.EMIT_GATHER:
+        kw1 = {k: v for k, v in autotune.apply(form1["engine"], best1).items() if k not in ("variant", "grid_cap")}
+        ph1 = form1["engine"].time_levels(0, x1, y1, reps=args.steps, **kw1)
+        fi1 = form1["engine"].info()
+        lds1 = bool(best1.get("flags", 0) & gmr.RUN_LDS_RANGES)
+        gb1 = (4 * (fi1["emit_total"] + fi1["num_term_emit"]) + 4 * args.rows
+               + (4 * (fi1["U"] + fi1["num_layered"]) if lds1 else 0))
+        ach1 = gb1 / (float(ph1[-1]) * 1e-3) / 1e9
+        one_roof = {"kernel": "k_gather_lds" if lds1 else "k_gather", "bound": "hbm", "achieved": round(ach1, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach1 / HBM_PEAK_GBS, 4),
+                    "launch_ms": round(float(ph1[-1]), 4), "bytes_per_launch": int(gb1),
+                    "traffic": read_traffic(f"{args.workload}{args.rows}x{args.cols}_B1"
+                                            + (f"_inline{form1['max_emits']}" if form1["max_emits"] else ""),
+                                            "k_gather_lds" if lds1 else "k_gather")}
+        log("best single vector", autotune.describe(best1), "gather", one_roof)
     del eng_ref
 
     # ---- vendor baseline (rocSPARSE CSR product of the uncompressed matrix at
@@ -473,7 +492,7 @@ def main():
         "pcie_inclusive": {"ms": pcie_ms, "value": nnz * B / (pcie_ms * 1e-3), "unit": "nnz*vectors/s",
                            "note": "X copied in from and Y copied out to pinned host memory inside every step"},
         "best_single_vector": {"value": nnz / (one_ms * 1e-3), "ms": one_ms, "schedule": autotune.describe(best1),
-                               "grammar_form": form1["label"]},
+                               "grammar_form": form1["label"], "roofline": one_roof},
         "fp32_err": err,
         "vendor_csr": vendor_line,
         "energy": energy,
@@ -790,12 +809,15 @@ def run_rmat(args):
             x[rng.choice(n, 1024, replace=False)] = 0.0
         xt = torch.from_numpy(x).cuda()
         y = torch.empty(n, dtype=torch.int32, device="cuda")
-        if args.no_tune:
+        if args.no_tune or args.schedule:
             # Boolean: values as bits (the LDS-range gather); min/plus: the XCD-partitioned gather
+            # (or the schedule given)
             form, tuned = untuned_form(args, forms, 1), []
             best = dict(variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER, b_tile=0, block=256, grid_cap=0,
                         gather_order=0)
-            if sr == gmr.BOOLEAN:
+            if args.schedule:
+                best.update(json.loads(args.schedule))
+            elif sr == gmr.BOOLEAN:
                 best["flags"] = gmr.RUN_BITS
             else:
                 best["xcd_passes"] = 0
@@ -821,7 +843,8 @@ def run_rmat(args):
         kernel = "k_bits_gather_lds" if bits else "k_gather"
         roof = {"kernel": kernel, "bound": "hbm", "achieved": phases["gather_GBps"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(phases["gather_GBps"] / HBM_PEAK_GBS, 4),
-                "traffic": read_traffic(f"rmat{scale}_{tag}", kernel) if form["label"] == "as built" else None}
+                "traffic": read_traffic(f"rmat{scale}_{tag}" + (f"_inline{form['max_emits']}" if form["max_emits"] else ""),
+                                        kernel)}
         seq, V = m.arrays()
         y_star, _ = O.brute_force(sr, n, n, seq, V, x)
         exact = bool(np.array_equal(y.cpu().numpy().astype(np.int64), y_star.astype(np.int64)))

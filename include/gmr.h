@@ -133,7 +133,8 @@ int gmr_engine_run(gmr_engine* e, int semiring, const void* d_X, int B, void* d_
  * memory: code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc. As gmr_engine_run, but the sweep's workspace (terminal
  * values and the rule-value array or frontier buffers: about (U + rules) x 4
- * bytes per vector of a tile in the Resident layout) comes from a static device
+ * bytes per vector of a tile in the Resident layout, plus 64 KiB of staging with
+ * GMR_RUN_FUSED_TAIL) comes from a static device
  * array of GMR_NOALLOC_SCRATCH_BYTES per GPU, and the call allocates nothing
  * (no hipMalloc, hipFree, graph instantiation or table upload). A run whose
  * workspace exceeds that size is refused with GMR_ERR_INVALID (use a narrower

@@ -104,7 +104,7 @@ def test_noalloc_one_vector_gathers_after_setup(hap):
         same(0, y, y_ref, den, f"one vector {kw}")
         same(0, y, y_star, den, f"one vector {kw} vs oracle")
     # lists built at upload: usable at once
-    for fl in (gmr.RUN_SINGLE_LIST, gmr.RUN_ROW_SEGMENTS):
+    for fl in (gmr.RUN_SINGLE_LIST, gmr.RUN_ROW_SEGMENTS, gmr.RUN_SINGLE_LIST | gmr.RUN_FUSED_TAIL):
         y, y_ref = pair(eng, 0, x, emit=gmr.EMIT_GATHER, flags=fl)
         same(0, y, y_star, den, f"one vector flags={fl}")
 
