@@ -1458,11 +1458,32 @@ constexpr int kLdsGatherBlock = 1024;
 // (terminal 0 is the semiring zero), rule values copied from W, zero past the
 // last key. Every load is clamped into its array, so none is conditional, and
 // a lane issues kFillBatch of each before using any.
+// A range of rule values only (no terminal key) is filled kRuleFill values per
+// lane at a time: its loads do not depend on other loads, so one round trip
+// fills a 40,960-value range of a 1,024-lane workgroup.
+constexpr int kRuleFill = 40;
+
 template <class S>
 __device__ __forceinline__ void fill_range(const SweepArgs& a, typename S::T* lds, long k0, int span) {
     using T = typename S::T;
     const T* W = tp<S>(a.W0);
     const long U = a.U, nw = a.nkey - a.U;  // terminal keys; rule values held in W
+    if (k0 >= U && nw > 0) {
+        for (int i0 = threadIdx.x; i0 < span; i0 += kRuleFill * blockDim.x) {
+            T wv[kRuleFill];
+#pragma unroll
+            for (int u = 0; u < kRuleFill; ++u) {
+                const long kw = k0 + i0 + static_cast<long>(u) * blockDim.x - U;
+                wv[u] = W[kw < nw ? kw : nw - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < kRuleFill; ++u) {
+                const int i = i0 + u * static_cast<int>(blockDim.x);
+                if (i < span) lds[i] = k0 + i < a.nkey ? wv[u] : S::zero();
+            }
+        }
+        return;
+    }
     for (int i0 = threadIdx.x; i0 < span; i0 += kFillBatch * blockDim.x) {
         float tv[kFillBatch];
         int tc[kFillBatch];
