@@ -1828,6 +1828,8 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
         if (variant == GMR_VARIANT_PERSISTENT) args[g].xparts = 0;
         if (args[g].bits || args[g].lspan)
             throw std::invalid_argument("GMR_RUN_BITS and GMR_RUN_LDS_RANGES run only in the levels variant of a batch");
+        if (variant == GMR_VARIANT_PERSISTENT && (o[g].flags & GMR_RUN_FUSED_TAIL))
+            throw std::invalid_argument("GMR_RUN_FUSED_TAIL runs in the levels and graph variants of a batch");
     }
     if (variant == GMR_VARIANT_GRAPH) {
         std::size_t key = mix(static_cast<std::size_t>(semiring), static_cast<std::size_t>(block));

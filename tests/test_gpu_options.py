@@ -171,6 +171,29 @@ def test_fused_tail_refusals(hap):
     torch.cuda.synchronize()
 
 
+def test_fused_tail_in_a_batch(hap, geno):
+    """One-vector grammars with GMR_RUN_FUSED_TAIL in a batch, levels variant
+    and graph variant (a branch per grammar), against the oracle; the
+    persistent batch, whose tiles run every level themselves, refuses it."""
+    cases = ((hap, 4096, 2048), (geno, 6000, 10000))
+    Xs, stars = [], []
+    for i, ((eng, seq, V), rows, cols) in enumerate(cases):
+        x = x_for(0, cols, 340 + i)
+        stars.append(O.brute_force(0, rows, cols, seq, V, x))
+        Xs.append(torch.from_numpy(x).reshape(cols, 1).cuda())
+    batch = gmr.Batch([c[0][0] for c in cases])
+    opts = [{"emit": gmr.EMIT_GATHER, "flags": gmr.RUN_FUSED_TAIL}, {"emit": gmr.EMIT_SPOT, "flags": gmr.RUN_FUSED_TAIL}]
+    for var in ("levels", "graph"):
+        Ys = batch.run_torch(0, Xs, variant=gmr.VARIANTS[var], opts=opts)
+        torch.cuda.synchronize()
+        for i, (Y, (ys, den)) in enumerate(zip(Ys, stars)):
+            check(0, Y.cpu().numpy()[:, 0], ys, den, f"batch {var} grammar {i} fused tail")
+    with pytest.raises(gmr.GmrError) as e:
+        batch.run_torch(0, Xs, variant=gmr.VARIANT_PERSISTENT, opts=opts)
+    assert e.value.code == gmr.ERR_INVALID
+    torch.cuda.synchronize()
+
+
 def test_lds_range_gather_unpacked_rows():
     """Entries carry their row as a 16-bit offset from their chunk's first row,
     so a row piece that starts 65,536 rows or more after its chunk's first row
