@@ -789,11 +789,22 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         hipLaunchKernelGGL(dev::k_emit<S>, dim3(grid_for(work, block)), dim3(block), 0, st, a, 0);
     }
     const int k_tail = tail_start(en, a, o);
+    const bool paired = (o.flags & GMR_RUN_PAIRED_LEVELS) && a.resident && a.Bs == 1 && !a.bits;
+    int pair_hi = 0;  // a level below this one was computed with the pair launch before it
     for (int k = 1; k <= en.L; ++k) {
         const int w = en.level_off[k] - en.level_off[k - 1];
         const int kind = host_kind(o, k);
         if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * k - 1], st));
-        if (k == k_tail) {
+        if (paired && k + 1 < k_tail && k > pair_hi &&
+            en.level_off[k + 1] - en.level_off[k - 1] <= dev::kPairMaxRules) {
+            // levels k and k + 1 in one launch (their time is level k's)
+            const long n2 = en.level_off[k + 1] - en.level_off[k - 1];
+            hipLaunchKernelGGL(dev::k_eval_pair<S>, dim3(grid_for((n2 + dev::kPairRules - 1) / dev::kPairRules, block)),
+                               dim3(block), 0, st, a, k);
+            pair_hi = k + 1;
+        } else if (k <= pair_hi) {
+            // computed by the pair launch of level k - 1
+        } else if (k == k_tail) {
             // levels k .. L at once (their time is level k's)
             auto* stage = static_cast<unsigned*>(a.tail_stage);
             const long n = en.level_off[en.L] - en.level_off[k - 1];
@@ -1010,11 +1021,11 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
             throw std::invalid_argument("lds_values (the EV_LDS window) must be in [1024, 16384]");
         a.lds_window = o.lds_values;
     }
-    if ((o.flags & GMR_RUN_FUSED_TAIL) &&
+    if ((o.flags & (GMR_RUN_FUSED_TAIL | GMR_RUN_PAIRED_LEVELS)) &&
         (!en.resident || B != 1 || o.variant == GMR_VARIANT_PERSISTENT || (o.flags & GMR_RUN_BITS)))
         throw std::invalid_argument(
-            "GMR_RUN_FUSED_TAIL needs one vector, the Resident layout and the levels or graph variant (not the bit "
-            "mode)");
+            "GMR_RUN_FUSED_TAIL and GMR_RUN_PAIRED_LEVELS need one vector, the Resident layout and the levels or graph "
+            "variant (not the bit mode)");
     if ((o.flags & GMR_RUN_FUSED_TAIL) && !en.ws_tail) {
         en.forbid_alloc("the fused tail's staging buffer");
         GMR_HIP(hipMalloc(&en.ws_tail, Engine::kTailStageBytes));
@@ -1828,8 +1839,9 @@ void batch_run_t(Batch& bt, int semiring, const float* const* X, const int* B, v
         if (variant == GMR_VARIANT_PERSISTENT) args[g].xparts = 0;
         if (args[g].bits || args[g].lspan)
             throw std::invalid_argument("GMR_RUN_BITS and GMR_RUN_LDS_RANGES run only in the levels variant of a batch");
-        if (variant == GMR_VARIANT_PERSISTENT && (o[g].flags & GMR_RUN_FUSED_TAIL))
-            throw std::invalid_argument("GMR_RUN_FUSED_TAIL runs in the levels and graph variants of a batch");
+        if (variant == GMR_VARIANT_PERSISTENT && (o[g].flags & (GMR_RUN_FUSED_TAIL | GMR_RUN_PAIRED_LEVELS)))
+            throw std::invalid_argument(
+                "GMR_RUN_FUSED_TAIL and GMR_RUN_PAIRED_LEVELS run in the levels and graph variants of a batch");
     }
     if (variant == GMR_VARIANT_GRAPH) {
         std::size_t key = mix(static_cast<std::size_t>(semiring), static_cast<std::size_t>(block));

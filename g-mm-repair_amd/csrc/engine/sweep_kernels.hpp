@@ -1041,6 +1041,74 @@ __global__ void k_eval(SweepArgs a, int k, int kind) {
                     nth, reinterpret_cast<typename S::T*>(dyn_lds), a.lds_window);
 }
 
+// Two levels of a one-vector sweep in one launch (Resident layout,
+// GMR_RUN_PAIRED_LEVELS): the rules of levels k and k + 1 together. A level
+// k + 1 rule computes each child that lies in level k from that child's own
+// kids, whose values earlier launches wrote, so it need not wait for level k's
+// stores: three rounds of loads (kids; the children's kids and values; the
+// grandchildren's values) in one launch instead of two rounds in each of two.
+// U rules per lane; every load is unconditional, and a load whose value is not
+// used reads terminal 0 (the semiring zero, written by the init phase).
+constexpr int kPairRules = 4;
+// Pairs hold at most this many rules: larger levels are bound by their loads, not by the
+// launch, and their recomputed children would only add loads.
+constexpr int kPairMaxRules = 1 << 18;
+
+template <class S>
+__global__ void k_eval_pair(SweepArgs a, int k) {
+    using T = typename S::T;
+    constexpr int U = kPairRules;
+    const T* Tt = tp<S>(a.Tv);
+    T* W = tp<S>(a.W0);
+    const int lo = a.level_off[k - 1], mid = a.level_off[k], hi = a.level_off[k + 1];
+    const long tid = xcd_tid(), nth = static_cast<long>(gridDim.x) * blockDim.x;
+    auto at = [&](int c) -> const T* { return c >= 0 ? W + c : Tt + (-c - 1); };
+    for (long i0 = lo + tid; i0 < hi; i0 += U * nth) {
+        int2 kc[U];
+        float cf[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long r = i0 + u * nth;
+            const long rc = r < hi ? r : hi - 1;
+            kc[u] = a.kids[rc];
+            cf[u] = a.coeff ? a.coeff[rc] : 1.0f;
+        }
+        int2 gl[U], gr[U];
+        float fl[U], fr[U];
+        T vl[U], vr[U];
+        bool il[U], ir[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool up = i0 + u * nth >= mid;  // a level k + 1 rule
+            il[u] = up && kc[u].x >= lo && kc[u].x < mid;
+            ir[u] = up && kc[u].y >= lo && kc[u].y < mid;
+            gl[u] = a.kids[il[u] ? kc[u].x : 0];
+            gr[u] = a.kids[ir[u] ? kc[u].y : 0];
+            fl[u] = a.coeff ? a.coeff[il[u] ? kc[u].x : 0] : 1.0f;
+            fr[u] = a.coeff ? a.coeff[ir[u] ? kc[u].y : 0] : 1.0f;
+            vl[u] = *(il[u] ? Tt : at(kc[u].x));
+            vr[u] = *(ir[u] ? Tt : at(kc[u].y));
+        }
+        T al[U], bl[U], ar[U], br[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            al[u] = *(il[u] ? at(gl[u].x) : Tt);
+            bl[u] = *(il[u] ? at(gl[u].y) : Tt);
+            ar[u] = *(ir[u] ? at(gr[u].x) : Tt);
+            br[u] = *(ir[u] ? at(gr[u].y) : Tt);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long r = i0 + u * nth;
+            if (r < hi) {
+                const T l = il[u] ? S::oplus(S::apply(al[u], fl[u]), bl[u]) : vl[u];
+                const T rv = ir[u] ? S::oplus(S::apply(ar[u], fr[u]), br[u]) : vr[u];
+                W[r] = S::oplus(S::apply(l, cf[u]), rv);
+            }
+        }
+    }
+}
+
 // The top levels of a one-vector sweep in one workgroup (Resident layout,
 // GMR_RUN_FUSED_TAIL): levels k0 .. L, at most kTailRules rules together, each
 // a launch of its own otherwise. One workgroup alone would fetch the tail's

@@ -35,6 +35,7 @@ EMITS = {"spot": EMIT_SPOT, "gather": EMIT_GATHER}
 EVAL_GROUP, EVAL_THREAD, EVAL_WAVE, EVAL_LDS = 0, 1, 2, 3
 RUN_TIMED, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, RUN_BITS, RUN_LDS_RANGES, RUN_ROW_ORDER, RUN_FUSED_TAIL = \
     1, 2, 4, 8, 16, 32, 64
+RUN_PAIRED_LEVELS = 128
 EVAL_KINDS = {"group": EVAL_GROUP, "thread": EVAL_THREAD, "wave": EVAL_WAVE, "lds": EVAL_LDS}
 
 TROPICAL_INF = 1000000000

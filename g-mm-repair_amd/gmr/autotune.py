@@ -12,7 +12,7 @@ import json
 import os
 
 from . import (BOOLEAN, EMIT_GATHER, EMIT_SPOT, EVAL_LDS, RESIDENT, RUN_BITS, RUN_FUSED_TAIL, RUN_LDS_RANGES,
-               RUN_ROW_ORDER, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, VARIANT_GRAPH, VARIANT_LEVELS, VARIANT_PERSISTENT, GmrError)
+               RUN_PAIRED_LEVELS, RUN_ROW_ORDER, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, VARIANT_GRAPH, VARIANT_LEVELS, VARIANT_PERSISTENT, GmrError)
 
 VARIANT_NAMES = {VARIANT_LEVELS: "levels", VARIANT_GRAPH: "graph", VARIANT_PERSISTENT: "persistent"}
 EMIT_NAMES = {EMIT_SPOT: "spot", EMIT_GATHER: "gather"}
@@ -33,7 +33,8 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
     sizes, and the LDS-staged node update (GMR_EVAL_LDS) at several window
     sizes; one vector in the Resident layout repeats the levels and graph
     schedules (bit mode aside) with the top levels fused into one workgroup
-    (GMR_RUN_FUSED_TAIL). Persistent shapes are (block, grid cap); a cap of 0 means the
+    (GMR_RUN_FUSED_TAIL), the row-gather ones also with small levels two to a
+    launch (GMR_RUN_PAIRED_LEVELS). Persistent shapes are (block, grid cap); a cap of 0 means the
     engine's default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
     tiles = sorted({B} | {t for t in (16, 32, 64, 128) if t < B})
@@ -90,9 +91,12 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
                     out.append(dict(variant=v, emit=e, b_tile=0, block=blocks[0], grid_cap=0, gather_order=0,
                                     eval_kind=EVAL_LDS, lds_values=win))
         if layout == RESIDENT:
-            # every levels / graph schedule (bit mode aside) also with its top levels in one workgroup
-            out += [dict(c, flags=c.get("flags", 0) | RUN_FUSED_TAIL) for c in out
-                    if c["variant"] != VARIANT_PERSISTENT and not c.get("flags", 0) & RUN_BITS]
+            # every levels / graph schedule (bit mode aside) also with its top levels in one
+            # workgroup, and the row-gather ones also with their small levels two to a launch
+            base_ = [c for c in out if c["variant"] != VARIANT_PERSISTENT and not c.get("flags", 0) & RUN_BITS]
+            out += [dict(c, flags=c.get("flags", 0) | RUN_FUSED_TAIL) for c in base_]
+            out += [dict(c, flags=c.get("flags", 0) | RUN_FUSED_TAIL | RUN_PAIRED_LEVELS) for c in base_
+                    if c["emit"] == EMIT_GATHER]
     return out
 
 
@@ -117,6 +121,7 @@ def describe(c):
             + (f"/lds-ranges(values={lds or 'default'})" if fl & RUN_LDS_RANGES else "")
             + ("/row-order" if fl & RUN_ROW_ORDER else "")
             + ("/fused-tail" if fl & RUN_FUSED_TAIL else "")
+            + ("/paired-levels" if fl & RUN_PAIRED_LEVELS else "")
             + (f"/xcd-passes={c['xcd_passes'] or 'auto'}" if "xcd_passes" in c else "")
             + (f"/chunks={c['chunks']}" if c.get("chunks") else "")
             + (f"/eval=lds(window={lds or 'default'})" if c.get("eval_kind") == EVAL_LDS else ""))

@@ -81,7 +81,11 @@ enum gmr_run_flags {
      * that stages the values of their children below them: two launches instead of one per
      * level. Without such levels, or where a workgroup may not hold that much LDS, the sweep
      * runs as without the flag. */
-    GMR_RUN_FUSED_TAIL = 64
+    GMR_RUN_FUSED_TAIL = 64,
+    /* One vector, Resident layout, levels or graph variant: levels run two to a launch; a rule
+     * of the upper level of a pair computes its children in the lower level from their own
+     * kids instead of waiting for them (levels below the fused tail, with GMR_RUN_FUSED_TAIL). */
+    GMR_RUN_PAIRED_LEVELS = 128
 };
 
 typedef struct gmr_run_opts {

@@ -125,6 +125,8 @@ def test_candidates_search_the_level_chunk_sizes():
             assert f"levels/gather/tile=B/blk=256/xcd-passes=2/chunks={ch}" in d
             assert f"graph/gather/tile=B/blk=256/lds-ranges(values=40960)/row-order/fused-tail/chunks={ch}" in d
         assert "levels/spot/tile=B/blk=256/fused-tail" in d
+        assert "levels/gather/tile=B/blk=256/lds-ranges(values=40960)/row-order/fused-tail/paired-levels/chunks=4" in d
+        assert not any("paired-levels" in x for x in d if "/spot/" in x)
         assert not any("fused-tail" in x for x in d if x.startswith("persistent") or "/bits(" in x)
         assert not any("lds-ranges" in x for x in d if x.startswith("persistent"))
     for B in (4, 64):

@@ -130,9 +130,11 @@ def test_lds_range_gather_row_order_empty_rows():
 
 @pytest.mark.parametrize("sr", [0, 1, 2])
 def test_fused_tail(hap, geno, sr):
-    """GMR_RUN_FUSED_TAIL: the top levels with at most 16,384 rules together in
-    one workgroup, with spot and gather emission, the levels and graph variants
-    and the one-vector gathers; on a grammar that small every level is in it."""
+    """GMR_RUN_FUSED_TAIL (the top levels with at most 8,192 rules together in
+    one workgroup) and GMR_RUN_PAIRED_LEVELS (levels two to a launch), alone
+    and together, with spot and gather emission, the levels and graph variants
+    and the one-vector gathers; on the smallest grammar every level is in the
+    tail."""
     small = gmr.Matrix.haplotype(20261204, 512, 256, founders=16, seg_mean=32.0)
     sseq, sV = small.arrays()
     seng = small.repair().schedule(512, 256, "resident").engine()
@@ -146,23 +148,27 @@ def test_fused_tail(hap, geno, sr):
                        dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=8192),
                        dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER, lds_values=8192),
                        dict(emit=gmr.EMIT_SPOT, eval_kind=gmr.EVAL_LDS)):
-                kw = dict(kw, flags=kw.get("flags", 0) | gmr.RUN_FUSED_TAIL)
-                _say(name, "fused tail", sr, var, kw)
-                y = run(eng, sr, x, variant=var, **kw)
-                check(sr, y, y_star, den, f"{name} fused tail var={var} {kw}")
+                for extra in (gmr.RUN_FUSED_TAIL, gmr.RUN_PAIRED_LEVELS, gmr.RUN_FUSED_TAIL | gmr.RUN_PAIRED_LEVELS):
+                    kw2 = dict(kw, flags=kw.get("flags", 0) | extra)
+                    _say(name, "fused tail / paired levels", sr, var, kw2)
+                    y = run(eng, sr, x, variant=var, **kw2)
+                    check(sr, y, y_star, den, f"{name} var={var} {kw2}")
 
 
 def test_fused_tail_refusals(hap):
     eng, seq, V = hap
     x = torch.zeros(2048, device="cuda")
     for kw in (dict(variant=gmr.VARIANT_PERSISTENT, flags=gmr.RUN_FUSED_TAIL),
-               dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_FUSED_TAIL | gmr.RUN_BITS)):
+               dict(variant=gmr.VARIANT_PERSISTENT, flags=gmr.RUN_PAIRED_LEVELS),
+               dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_FUSED_TAIL | gmr.RUN_BITS),
+               dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_PAIRED_LEVELS | gmr.RUN_BITS)):
         with pytest.raises(gmr.GmrError) as e:
             eng.run_torch(1, x, **kw)
         assert e.value.code == gmr.ERR_INVALID, kw
-    with pytest.raises(gmr.GmrError) as e:
-        eng.run_torch(0, torch.zeros(2048, 4, device="cuda"), flags=gmr.RUN_FUSED_TAIL)  # one vector only
-    assert e.value.code == gmr.ERR_INVALID
+    for fl in (gmr.RUN_FUSED_TAIL, gmr.RUN_PAIRED_LEVELS):
+        with pytest.raises(gmr.GmrError) as e:
+            eng.run_torch(0, torch.zeros(2048, 4, device="cuda"), flags=fl)  # one vector only
+        assert e.value.code == gmr.ERR_INVALID
     m = gmr.Matrix.haplotype(20261204, 512, 256, founders=16, seg_mean=32.0)
     weng = m.repair().schedule(512, 256, "wave64").engine()
     with pytest.raises(gmr.GmrError) as e:

@@ -75,6 +75,13 @@ def test_unit_grammar_on_gpu():
                 _say("unit", lay, var, em)
                 y = run_engine(eng, 0, x, variant=gmr.VARIANTS[var], emit=gmr.EMITS[em])
                 np.testing.assert_allclose(y, f["expect_y"], atol=1e-6)
+            if lay == "resident":
+                # levels two to a launch (the run-length rule's coefficient applied to a
+                # recomputed child) and the top levels in one workgroup
+                for fl in (gmr.RUN_PAIRED_LEVELS, gmr.RUN_FUSED_TAIL, gmr.RUN_PAIRED_LEVELS | gmr.RUN_FUSED_TAIL):
+                    for em in ("spot", "gather"):
+                        y = run_engine(eng, 0, x, emit=gmr.EMITS[em], flags=fl)
+                        np.testing.assert_allclose(y, f["expect_y"], atol=1e-6)
 
 
 def test_readme_example_on_gpu():
