@@ -89,6 +89,8 @@ def test_config2_single_vector(cfg2):
                   lds_values=40960, chunks=4),
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER | gmr.RUN_FUSED_TAIL,
                   lds_values=40960, chunks=8),
+             dict(emit=gmr.EMIT_GATHER, lds_values=40960, chunks=4,
+                  flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER | gmr.RUN_FUSED_TAIL | gmr.RUN_PAIRED_LEVELS),
              dict(emit=gmr.EMIT_SPOT, flags=gmr.RUN_FUSED_TAIL),
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_SINGLE_LIST),
              dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_ROW_SEGMENTS),
@@ -178,6 +180,8 @@ def test_config4b_minplus_relaxation():
         for kw in (dict(emit=gmr.EMIT_GATHER),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER | gmr.RUN_FUSED_TAIL),
+                   dict(emit=gmr.EMIT_GATHER, lds_values=32768,
+                        flags=gmr.RUN_LDS_RANGES | gmr.RUN_FUSED_TAIL | gmr.RUN_PAIRED_LEVELS),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_SINGLE_LIST),
                    dict(emit=gmr.EMIT_SPOT),
                    dict(variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_SPOT)):
