@@ -623,7 +623,7 @@ const Engine::XParts& Engine::lds_lists(int span, bool row_order) {
     if (!resident) throw std::invalid_argument("the LDS-range gather needs the Resident layout");
     const long u = U;
     auto key = [u](int c) -> long { return c < 0 ? static_cast<long>(-c - 1) : u + c; };
-    if (row_order && !d_row_new) {
+    if (row_order && !row_order_built) {
         const long nsym = crow_host[0];
         std::vector<int> crow(static_cast<std::size_t>(rows) + 1), csym(static_cast<std::size_t>(nsym));
         GMR_HIP(hipDeviceSynchronize());
@@ -634,6 +634,7 @@ const Engine::XParts& Engine::lds_lists(int span, bool row_order) {
         for (int j = 0; j < rows; ++j) inv[static_cast<std::size_t>(row_order_host[j])] = j;
         d_row_new = dev_upload(inv.data(), inv.size(), &struct_bytes);
         GMR_HIP(hipMalloc(&ws_yrow, sizeof(float) * std::max(rows, 1)));
+        row_order_built = true;
     }
     return lparts.emplace(mk, range_lists(*this, u + NL, span, key, true, row_order ? &row_order_host : nullptr))
         .first->second;
@@ -842,7 +843,7 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         }
     } else if (a.gather && a.lspan) {
         // one LDS range per list; workgroups take equal shares of all chunks
-        const long nch = en.lparts.at(2 * a.lspan + (a.ynew ? 1 : 0)).poff.back();
+        const long nch = en.lparts.at(2 * a.lspan + a.lorder).poff.back();
         const std::size_t lds = static_cast<std::size_t>(a.lspan) * sizeof(typename S::T);
         const unsigned g = lds_gather_grid(en, nch, lds);
         if (nch > 0) {
@@ -858,7 +859,7 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
             }
         }
         // rows in the lists' order: y into the caller's Y in matrix row order
-        if (a.ynew) hipLaunchKernelGGL(dev::k_rows_to_matrix_order<S>, dim3(grid_for(a.rows, block)), dim3(block), 0, st, a);
+        if (a.lorder) hipLaunchKernelGGL(dev::k_rows_to_matrix_order<S>, dim3(grid_for(a.rows, block)), dim3(block), 0, st, a);
     } else if (a.gather) {
         // (the B-wide row gather takes two segments per lane group at once)
         const long threads = a.flat ? static_cast<long>(en.n_fchunk) * 64
@@ -1071,6 +1072,7 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
         const Engine::XParts& x = en.lds_lists(span, ordered);
         if (ordered) {
             // pieces combine in the row-ordered copy, which the init zeroes
+            a.lorder = 1;
             a.ynew = en.d_row_new;
             a.Yout = a.Y;
             a.Y = en.ws_yrow;

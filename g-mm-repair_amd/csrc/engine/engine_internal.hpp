@@ -94,6 +94,7 @@ struct Engine {
     // d_row_new[r] is the number of matrix row r, ws_yrow the row-ordered copy of y (rows values)
     int* d_row_new = nullptr;
     void* ws_yrow = nullptr;
+    bool row_order_built = false;
     std::vector<int> row_order_host;  // new number -> matrix row (built with the first ordered lists)
     // GMR_RUN_FUSED_TAIL: staged child values of the fused top levels (2 x 8,192
     // values), allocated with the first such run

@@ -147,9 +147,10 @@ struct SweepArgs {
     // inside the range (0xFFFF: padding); xord / xrow are then not used
     int packed;
     const int* cbase;
-    // LDS-range gather, rows renumbered (GMR_RUN_ROW_ORDER): the lists' rows are
-    // new numbers, pieces combine into Y (then the row-ordered copy) and a last
-    // pass writes Yout[r] = Y[ynew[r]]; nullptr: rows in matrix order
+    // LDS-range gather, rows renumbered (GMR_RUN_ROW_ORDER, lorder = 1): the
+    // lists' rows are new numbers, pieces combine into Y (then the row-ordered
+    // copy) and a last pass writes Yout[r] = Y[ynew[r]]
+    int lorder;
     const int* ynew;
     void* Yout;
     // GMR_RUN_FUSED_TAIL: 2 x kTailRules values staging the tail's children that
