@@ -884,6 +884,42 @@ def run_rmat(args):
                     "per_form_ms": {r[3]: round(r[0], 3) for r in runs},
                     "moved": int((got != np.minimum(x, 1e9)).sum())}
             log(tag, "traversal", trav)
+            # the same traversal over an engine of M^T: every step a right product
+            # (Boolean: bit mode; min/plus: LDS ranges with the fused tail)
+            t0 = time.time()
+            gt = m.transpose().repair()
+            # one-vector right-product forms: Boolean as bits; min/plus, whose x turns
+            # finite as the traversal goes (then every row piece emits), also the row
+            # segments (each row reduced in registers, stored once)
+            T_ = gmr.RUN_TRANSPOSED
+            tkws = ([dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS | T_, lds_values=262144, chunks=8)]
+                    if sr == gmr.BOOLEAN else
+                    [dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_ROW_SEGMENTS | gmr.RUN_FUSED_TAIL | T_),
+                     dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_FUSED_TAIL | T_),
+                     dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_FUSED_TAIL | T_,
+                          lds_values=32768, chunks=4)])
+            truns = []
+            for lab, gg in (("as built", gt), ("top rules used <= 8 times in C inlined", gt.inline_top(8))):
+                et = gg.schedule(n, n, "resident").engine()
+                for tkw in tkws:
+                    xs = torch.from_numpy(x).cuda()
+                    et.traverse_torch(sr, xs.clone(), max_iters=2, **tkw)  # builds the lists
+                    torch.cuda.synchronize()
+                    t1 = time.time()
+                    ti = et.traverse_torch(sr, xs, **tkw)
+                    torch.cuda.synchronize()
+                    truns.append(((time.time() - t1) * 1e3, ti, xs.cpu().numpy(), lab, autotune.describe(
+                        dict(tkw, variant=gmr.VARIANT_LEVELS, b_tile=0, block=256))))
+                    log(tag, "transposed traversal", lab, truns[-1][4], f"{truns[-1][0]:.2f} ms, {ti} products")
+                del et
+            tt_ms, t_it, _, t_lab, t_sched = min(truns, key=lambda r: r[0])
+            trav["transposed"] = {"ms": tt_ms, "products": t_it, "ms_per_product": tt_ms / t_it,
+                                  "grammar_form": t_lab, "schedule": t_sched,
+                                  "bit_exact": all(np.array_equal(clip(r[2]), want) for r in truns),
+                                  "per_form_ms": {r[3] + " " + r[4]: round(r[0], 3) for r in truns},
+                                  "transpose_and_repair_s": round(time.time() - t0, 1)}
+            log(tag, "transposed traversal", trav["transposed"])
+            del gt
         info = s_res.info()
         res[tag] = {"nnz": nnz, "ms": ms, "nnz_per_s": nnz / (ms * 1e-3), "schedule": autotune.describe(best),
                     "grammar_form": form["label"], "phases": phases, "roofline": roof,

@@ -1022,6 +1022,8 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
             throw std::invalid_argument("lds_values (the EV_LDS window) must be in [1024, 16384]");
         a.lds_window = o.lds_values;
     }
+    if (o.flags & GMR_RUN_TRANSPOSED)
+        throw std::invalid_argument("GMR_RUN_TRANSPOSED applies to gmr_engine_traverse only");
     if ((o.flags & (GMR_RUN_FUSED_TAIL | GMR_RUN_PAIRED_LEVELS)) &&
         (!en.resident || B != 1 || o.variant == GMR_VARIANT_PERSISTENT || (o.flags & GMR_RUN_BITS)))
         throw std::invalid_argument(
@@ -1687,6 +1689,8 @@ void Engine::run_left(int semiring, const float* X, int B, void* Y, const gmr_ru
         throw std::invalid_argument("block must be a multiple of 64 in [64, 1024]");
     if (o.variant != GMR_VARIANT_LEVELS && o.variant != GMR_VARIANT_GRAPH)
         throw std::invalid_argument("the left product runs as per-level launches or one hipGraph");
+    if (o.flags & GMR_RUN_TRANSPOSED)
+        throw std::invalid_argument("GMR_RUN_TRANSPOSED applies to gmr_engine_traverse only");
     build_left();
     switch (semiring) {
         case GMR_PLUS_TIMES: run_left_t<dev::PlusTimesOp>(*this, semiring, X, B, Y, o, st); break;
@@ -1718,9 +1722,14 @@ int Engine::traverse(int semiring, float* x, int max_iters, const gmr_run_opts& 
     int* flag = reinterpret_cast<int*>(static_cast<char*>(ws_trav) + (need - 256));
     const int block = 256;
     const unsigned grid = grid_for(std::max(rows, 1), block);
+    // with GMR_RUN_TRANSPOSED the engine holds M^T: x^T M = (M^T x)^T, a right product
+    const bool right = (o.flags & GMR_RUN_TRANSPOSED) != 0;
+    gmr_run_opts op = o;
+    op.flags &= ~GMR_RUN_TRANSPOSED;
     int it = 0;
     for (; it < max_iters;) {
-        run_left(semiring, x, 1, y, o, st);
+        if (right) run(semiring, x, 1, y, op, st);
+        else run_left(semiring, x, 1, y, op, st);
         hipLaunchKernelGGL(dev::k_clear_flag, dim3(1), dim3(64), 0, st, flag);
         if (semiring == GMR_BOOLEAN)
             hipLaunchKernelGGL(dev::k_relax<dev::BooleanOp>, dim3(grid), dim3(block), 0, st, x, y, rows, flag);

@@ -35,7 +35,7 @@ EMITS = {"spot": EMIT_SPOT, "gather": EMIT_GATHER}
 EVAL_GROUP, EVAL_THREAD, EVAL_WAVE, EVAL_LDS = 0, 1, 2, 3
 RUN_TIMED, RUN_ROW_SEGMENTS, RUN_SINGLE_LIST, RUN_BITS, RUN_LDS_RANGES, RUN_ROW_ORDER, RUN_FUSED_TAIL = \
     1, 2, 4, 8, 16, 32, 64
-RUN_PAIRED_LEVELS = 128
+RUN_PAIRED_LEVELS, RUN_TRANSPOSED = 128, 256
 EVAL_KINDS = {"group": EVAL_GROUP, "thread": EVAL_THREAD, "wave": EVAL_WAVE, "lds": EVAL_LDS}
 
 TROPICAL_INF = 1000000000
@@ -127,6 +127,31 @@ class Matrix:
     def nnz(self):
         i = self.info()
         return i["len"] - i["rows"]
+
+    def transpose(self):
+        """M^T as a Matrix (cols x rows, the same values), built on the host from
+        the CSRV string: terminal 1 + v*cols + c of row r becomes 1 + v*rows + r
+        of row c, rows' terminals in column order."""
+        i = self.info()
+        rows, cols = i["rows"], i["cols"]
+        seq, V = self.arrays()
+        ends = np.flatnonzero(seq == 0)
+        lens = np.diff(np.r_[-1, ends]) - 1
+        r = np.repeat(np.arange(rows, dtype=np.int64), lens)
+        t = seq[seq != 0].astype(np.int64) - 1
+        v, c = t // cols, t % cols
+        order = np.lexsort((r, c))
+        nt = v[order] * rows + r[order] + 1
+        if len(nt) and nt.max() >= 2 ** 31:
+            raise ValueError("transpose: terminal ids exceed int32")
+        counts = np.bincount(c, minlength=cols)
+        out = np.empty(len(nt) + cols, np.int32)
+        sep = np.cumsum(counts + 1) - 1
+        keep = np.ones(len(out), bool)
+        keep[sep] = False
+        out[keep] = nt
+        out[sep] = 0
+        return Matrix.from_csrv(cols, rows, out, V)
 
     def row_block(self, r0, r1):
         """Rows [r0, r1) as a Matrix of their own (same columns and values)."""
@@ -421,7 +446,9 @@ class Engine:
 
     def traverse_torch(self, semiring, x, max_iters=1000, **kw):
         """Fixpoint x <- x (+) x^T M in place on a float32 CUDA vector of
-        length rows (square matrix); returns the number of products computed."""
+        length rows (square matrix); returns the number of products computed.
+        With flags including RUN_TRANSPOSED the engine holds M^T and every
+        step is its right product (the other keywords are that product's)."""
         import torch
         if isinstance(semiring, str):
             semiring = SEMIRINGS[semiring]

@@ -85,7 +85,12 @@ enum gmr_run_flags {
     /* One vector, Resident layout, levels or graph variant: levels run two to a launch; a rule
      * of the upper level of a pair computes its children in the lower level from their own
      * kids instead of waiting for them (levels below the fused tail, with GMR_RUN_FUSED_TAIL). */
-    GMR_RUN_PAIRED_LEVELS = 128
+    GMR_RUN_PAIRED_LEVELS = 128,
+    /* gmr_engine_traverse only: the engine holds the transposed matrix, so every step is the
+     * right product y = M^T x, with the one-vector gathers of the right product (GMR_RUN_BITS,
+     * GMR_RUN_LDS_RANGES and the flags above); the rest of opts is that product's. Other entry
+     * points refuse the flag. */
+    GMR_RUN_TRANSPOSED = 256
 };
 
 typedef struct gmr_run_opts {
@@ -170,9 +175,11 @@ int gmr_engine_run_left(gmr_engine* e, int semiring, const void* d_X, int B, voi
  * changes or max_iters products were computed. d_x: rows floats, in: the
  * start (Boolean 0/1 sources; Tropical distances, 1e9 = unreached), out: the
  * reachable set / shortest distances along row -> column edges. Synchronises
- * `stream` after every product; *iters_out = products computed. The
- * reference's fixpoint comparison is cuGraph BFS / SSSP
- * (reference cugraph_bench.py:53-61). */
+ * `stream` after every product; *iters_out = products computed. Each product
+ * is the left product of the engine's matrix M, or, with GMR_RUN_TRANSPOSED in
+ * opts->flags, the right product of the engine's matrix when it holds M^T
+ * (the same x^T M, by the one-vector right-product gathers). The reference's
+ * fixpoint comparison is cuGraph BFS / SSSP (reference cugraph_bench.py:53-61). */
 int gmr_engine_traverse(gmr_engine* e, int semiring, void* d_x, int max_iters, int* iters_out,
                         const gmr_run_opts* opts, void* stream);
 /* The persistent variant. Its grid barrier needs every workgroup of the launch

@@ -80,6 +80,52 @@ def test_reachability_and_shortest_paths(variant):
         assert iters == want_it
 
 
+@pytest.mark.parametrize("variant", ["levels", "graph"])
+def test_traversal_by_right_products_of_the_transpose(variant):
+    """GMR_RUN_TRANSPOSED: an engine of M^T traverses with right products (the
+    bit mode and the LDS-range gather among them) to the same reachable set and
+    distances as the left-product traversal of M, scipy and the oracle."""
+    for nw, sr in ((1, 1), (100, 2)):
+        m = gmr.Matrix.rmat(310 + nw, 12, 30000, nweights=nw)
+        A, seq, V = _csr(m)
+        n = m.rows
+        eng_t = m.transpose().repair().schedule(n, n, "resident").engine()
+        src = np.random.default_rng(nw + 7).choice(n, 3, replace=False)
+        if sr == 1:
+            x0 = np.zeros(n, np.float32)
+            x0[src] = 1
+            opts = [dict(emit=gmr.EMIT_GATHER), dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS),
+                    dict(emit=gmr.EMIT_SPOT)]
+        else:
+            x0 = np.full(n, 1e9, np.float32)
+            x0[src] = 0
+            opts = [dict(emit=gmr.EMIT_GATHER),
+                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_FUSED_TAIL, lds_values=4096),
+                    dict(emit=gmr.EMIT_SPOT)]
+        want, want_it = _fixpoint_oracle(sr, n, seq, V, x0)
+        for kw in opts:
+            x = torch.from_numpy(x0).cuda()
+            kw = dict(kw, flags=kw.get("flags", 0) | gmr.RUN_TRANSPOSED)
+            iters = eng_t.traverse_torch(sr, x, variant=gmr.VARIANTS[variant], **kw)
+            got = x.cpu().numpy()
+            print(f"transposed sr={sr} {variant} {kw}: {iters} products", flush=True)
+            np.testing.assert_array_equal(got, want)
+            assert iters == want_it
+
+
+def test_transposed_flag_only_for_traversal():
+    m = gmr.Matrix.rmat(9, 6, 300)
+    eng = m.repair().schedule(m.rows, m.rows, "resident").engine()
+    x = torch.zeros(m.rows, device="cuda")
+    with pytest.raises(gmr.GmrError) as e:
+        eng.run_torch(1, x, flags=gmr.RUN_TRANSPOSED)
+    assert e.value.code == gmr.ERR_INVALID
+    with pytest.raises(gmr.GmrError) as e:
+        eng.run_left_torch(1, x, flags=gmr.RUN_TRANSPOSED)
+    assert e.value.code == gmr.ERR_INVALID
+    torch.cuda.synchronize()
+
+
 def test_traversal_errors():
     m = gmr.Matrix.haplotype(5, 64, 32, founders=4, seg_mean=8.0)
     eng = m.repair().schedule(64, 32, "resident").engine()

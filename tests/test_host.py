@@ -312,6 +312,28 @@ def test_csr_arrays_from_the_csrv_string():
     np.testing.assert_array_equal(D, M)
 
 
+def test_transpose_of_the_csrv_string():
+    """Matrix.transpose() is the CSRV string of M^T (the same string
+    from_dense builds from M.T, values and empty rows and columns included), and
+    transposing twice gives M back."""
+    rng = np.random.default_rng(41)
+    for shape in ((37, 23), (5, 60), (64, 1)):
+        M = (rng.random(shape) < 0.2) * rng.integers(1, 6, shape).astype(np.float64)
+        M[0] = 0.0
+        M[:, -1] = 0.0
+        m = gmr.Matrix.from_dense(M)
+        t = m.transpose()
+        s_t, V_t = t.arrays()
+        s_d, V_d = gmr.Matrix.from_dense(M.T.copy()).arrays()
+        assert (t.rows, t.cols) == (shape[1], shape[0])
+        np.testing.assert_array_equal(V_t, V_d)
+        np.testing.assert_array_equal(s_t, s_d)
+        s_tt, V_tt = t.transpose().arrays()
+        s_m, V_m = m.arrays()
+        np.testing.assert_array_equal(s_tt, s_m)
+        np.testing.assert_array_equal(V_tt, V_m)
+
+
 def test_row_blocks_partition_the_csrv_string():
     rng = np.random.default_rng(12)
     M = rng.integers(0, 3, (23, 9)) * (rng.random((23, 9)) < 0.4)
