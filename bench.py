@@ -388,6 +388,10 @@ def main():
                     "traffic": read_traffic(f"{args.workload}{args.rows}x{args.cols}_B1"
                                             + (f"_inline{form1['max_emits']}" if form1["max_emits"] else ""),
                                             "k_gather_lds" if lds1 else "k_gather")}
+        # the whole one-vector sweep on the amended DAG count of the form that runs
+        sb1 = amended_bytes(form1["schedule"].info(), 1)
+        one_roof["sweep"] = {"ms": one_ms, "bytes": int(sb1), "achieved": round(sb1 / (one_ms * 1e-3) / 1e9, 1),
+                             "frac": round(sb1 / (one_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log("best single vector", autotune.describe(best1), "gather", one_roof)
     del eng_ref
 
