@@ -494,7 +494,8 @@ namespace {
 // Packs every entry of range lists (already indices inside their range, -1
 // for padding) as (row - first row of its chunk) << 16 | index (0xFFFF for
 // padding), with the chunks' first rows in cbase. Fails (false, L unchanged)
-// when a range is 65,535 keys or more or a chunk spans 65,536 rows or more.
+// when a range is 65,535 keys or more or a chunk spans 65,536 rows or more
+// (lists built with max_rows = 65,536 never do).
 bool pack_entries(PartLists& L, long span, std::vector<int>& cbase) {
     if (span >= 0xFFFF) return false;
     const long nch = static_cast<long>(L.bits.size());
@@ -545,8 +546,9 @@ std::vector<int> similar_row_order(const std::vector<int>& crow, const std::vect
 
 // Lists by fixed key ranges of `span` keys (one list per range, each entry
 // stored as its key's index inside the range; padding -1), uploaded. With
-// `pack`, entries are packed with their row (pack_entries) where possible.
-// With `row_order` (new number -> matrix row) the lists number rows by it.
+// `pack`, entries are packed with their row (pack_entries; ranges must be
+// below 65,535 keys). With `row_order` (new number -> matrix row) the lists
+// number rows by it.
 Engine::XParts range_lists(Engine& en, long nkey, long span, const std::function<long(int)>& key,
                            bool pack = false, const std::vector<int>* row_order = nullptr) {
     const long nsym = en.crow_host[0];
@@ -580,7 +582,9 @@ Engine::XParts range_lists(Engine& en, long nkey, long span, const std::function
         }
     Engine::XParts x;
     std::vector<int> cbase;
-    x.packed = pack && pack_entries(L, span, cbase);
+    if (pack && !pack_entries(L, span, cbase))
+        throw std::invalid_argument("range lists: a range of 65,535 keys or more cannot be packed");
+    x.packed = pack;
     x.poff = L.poff;
     x.d_sym = dev_upload(L.sym.data(), L.sym.size(), &en.struct_bytes);
     x.d_bits = dev_upload(L.bits.data(), L.bits.size(), &en.struct_bytes);
@@ -848,15 +852,9 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         const unsigned g = lds_gather_grid(en, nch, lds);
         if (nch > 0) {
             const dim3 gr(g), bl(dev::kLdsGatherBlock);
-            if (a.packed) {
-                if (a.chunks == 2) hipLaunchKernelGGL((dev::k_gather_lds<S, 2, true>), gr, bl, lds, st, a);
-                else if (a.chunks == 8) hipLaunchKernelGGL((dev::k_gather_lds<S, 8, true>), gr, bl, lds, st, a);
-                else hipLaunchKernelGGL((dev::k_gather_lds<S, 4, true>), gr, bl, lds, st, a);
-            } else {
-                if (a.chunks == 2) hipLaunchKernelGGL((dev::k_gather_lds<S, 2, false>), gr, bl, lds, st, a);
-                else if (a.chunks == 8) hipLaunchKernelGGL((dev::k_gather_lds<S, 8, false>), gr, bl, lds, st, a);
-                else hipLaunchKernelGGL((dev::k_gather_lds<S, 4, false>), gr, bl, lds, st, a);
-            }
+            if (a.chunks == 2) hipLaunchKernelGGL((dev::k_gather_lds<S, 2>), gr, bl, lds, st, a);
+            else if (a.chunks == 8) hipLaunchKernelGGL((dev::k_gather_lds<S, 8>), gr, bl, lds, st, a);
+            else hipLaunchKernelGGL((dev::k_gather_lds<S, 4>), gr, bl, lds, st, a);
         }
         // rows in the lists' order: y into the caller's Y in matrix row order
         if (a.lorder) hipLaunchKernelGGL(dev::k_rows_to_matrix_order<S>, dim3(grid_for(a.rows, block)), dim3(block), 0, st, a);
@@ -1066,10 +1064,10 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
     }
     if (o.flags & GMR_RUN_LDS_RANGES) {
         const int span = o.lds_values > 0 ? o.lds_values : 32768;
-        if (span < 4096 || static_cast<long>(span) * vbytes > static_cast<long>(en.max_lds))
+        if (span < 4096 || span >= 0xFFFF || static_cast<long>(span) * vbytes > static_cast<long>(en.max_lds))
             throw std::invalid_argument(
-                "lds_values (values per range of GMR_RUN_LDS_RANGES) must be at least 4096 and fit the LDS of a "
-                "workgroup (" + std::to_string(en.max_lds) + " bytes)");
+                "lds_values (values per range of GMR_RUN_LDS_RANGES) must be at least 4096, below 65,535 and fit "
+                "the LDS of a workgroup (" + std::to_string(en.max_lds) + " bytes)");
         const bool ordered = (o.flags & GMR_RUN_ROW_ORDER) != 0;
         const Engine::XParts& x = en.lds_lists(span, ordered);
         if (ordered) {
@@ -1080,7 +1078,6 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
             a.Y = en.ws_yrow;
         }
         a.lspan = span;
-        a.packed = x.packed ? 1 : 0;
         a.cbase = x.d_cbase;
         // the range fills compute the terminal values; the init computes only
         // those the rules read

@@ -143,9 +143,8 @@ struct SweepArgs {
     // values rules read (tref, n_tref of them); the range fills compute the rest
     const int* tref;
     int n_tref;
-    // LDS-range gather, packed entries: entry = (row - cbase[chunk]) << 16 | index
-    // inside the range (0xFFFF: padding); xord / xrow are then not used
-    int packed;
+    // LDS-range gather: entry = (row - cbase[chunk]) << 16 | index inside the
+    // range (0xFFFF: padding); xord / xrow are not used
     const int* cbase;
     // LDS-range gather, rows renumbered (GMR_RUN_ROW_ORDER, lorder = 1): the
     // lists' rows are new numbers, pieces combine into Y (then the row-ordered
@@ -1578,10 +1577,10 @@ __device__ __forceinline__ void fill_range(const SweepArgs& a, typename S::T* ld
     }
 }
 
-// PACKED: every entry carries its row as an offset from its chunk's first row
-// (the high 16 bits; the index inside the range is the low 16 bits), so the
-// row of a piece comes from the entry that ends it, with no dependent load.
-template <class S, int K, bool PACKED>
+// Every entry carries its row as an offset from its chunk's first row (the high
+// 16 bits; the index inside the range is the low 16 bits), so the row of a
+// piece comes from the entry that ends it, with no dependent load.
+template <class S, int K>
 __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
     using T = typename S::T;
     T* lds = reinterpret_cast<T*>(dyn_lds);
@@ -1607,8 +1606,8 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
         const long step = static_cast<long>(nwave) * K;
         long c = c0 + static_cast<long>(wave) * K;
         unsigned l[K];
-        // lane j < K holds chunk c + j's row-start word and first row (or row
-        // order): one load for the K chunks, read out per chunk where used
+        // lane j < K holds chunk c + j's row-start word and first row: one load
+        // for the K chunks, read out per chunk where used
         unsigned long long fw;
         int ow;
         auto load_chunks = [&](long cc) {
@@ -1620,28 +1619,27 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
             const long cl = cc + (lane < K ? lane : 0);
             const long cw = cl < c1 ? cl : c1 - 1;
             fw = a.xbits[cw];
-            ow = PACKED ? a.cbase[cw] : a.xord[cw];
+            ow = a.cbase[cw];
         };
         load_chunks(c);
         while (c < c1) {
             T v[K];
             bool end[K];
-            int ri[K];
             unsigned long long fb[K];
-            int ord[K];
+            int base[K];  // the row of chunk j's first entry
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 fb[j] = readlane64(fw, j);
-                ord[j] = __builtin_amdgcn_readlane(ow, j);
+                base[j] = __builtin_amdgcn_readlane(ow, j);
             }
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                const unsigned li = PACKED ? (l[j] & 0xFFFFu) : l[j];
+                const unsigned li = l[j] & 0xFFFFu;
                 v[j] = lds[li < static_cast<unsigned>(span) ? li : 0u];
             }
 #pragma unroll
             for (int j = 0; j < K; ++j) {
-                if ((PACKED ? (l[j] & 0xFFFFu) : l[j]) >= static_cast<unsigned>(span)) v[j] = S::zero();  // padding
+                if ((l[j] & 0xFFFFu) >= static_cast<unsigned>(span)) v[j] = S::zero();  // padding
                 const unsigned long long bits = fb[j];
                 // Segmented scan over the lanes of each row piece (DPP; the K
                 // chunks' steps interleave).
@@ -1650,13 +1648,10 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
                 v[j] = seg_scan<S>(v[j], dist, lane);
                 const bool ends = lane == 63 || ((bits >> (lane + 1)) & 1ull) != 0ull;
                 end[j] = c + j < c1 && ends && !S::skip(v[j]);
-                const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                    static_cast<unsigned>(bits >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bits), 0u)));
-                ri[j] = end[j] ? ord[j] - 1 + below + static_cast<int>((bits >> lane) & 1ull) : 0;
             }
             int row[K];
 #pragma unroll
-            for (int j = 0; j < K; ++j) row[j] = PACKED ? ord[j] + static_cast<int>(l[j] >> 16) : a.xrow[ri[j]];
+            for (int j = 0; j < K; ++j) row[j] = base[j] + static_cast<int>(l[j] >> 16);
             const long cn = c + step;
             load_chunks(cn);
 #pragma unroll

@@ -107,8 +107,8 @@ typedef struct gmr_run_opts {
     const int* level_kind;  /* optional host array, max_depth + 1 entries ([0] unused) */
     /* Level-chunk sizes (0: the defaults below; the autotuner searches them):
      * lds_values  values a workgroup stages in LDS, a multiple of 1024: with GMR_RUN_LDS_RANGES
-     *             the values per range (default 32768; at least 4096, at most what one
-     *             workgroup's LDS holds); with GMR_RUN_BITS the bits per range (default 524288,
+     *             the values per range (default 32768; at least 4096, below 65535, at most what
+     *             one workgroup's LDS holds); with GMR_RUN_BITS the bits per range (default 524288,
      *             131072..1048576); otherwise the left-child window of GMR_EVAL_LDS levels in the
      *             levels and graph variants (default 4096, 1024..16384)
      * chunks      64-symbol chunks a wave of a chunked single-vector gather has in flight:

@@ -200,7 +200,7 @@ def test_fused_tail_in_a_batch(hap, geno):
     torch.cuda.synchronize()
 
 
-def test_lds_range_gather_unpacked_rows():
+def test_lds_range_gather_sparse_rows():
     """Entries carry their row as a 16-bit offset from their chunk's first row,
     so a row piece that starts 65,536 rows or more after its chunk's first row
     starts a new chunk (here: three non-zeros in 150,000 rows, with and without
