@@ -255,14 +255,15 @@ template <class S>
 __device__ __forceinline__ typename S::T seg_scan(typename S::T v, int dist, int lane) {
     using T = typename S::T;
     const int r = lane & 15;
+    const int m = min(dist, r);  // how far the piece reaches back inside the row
     T t = dpp_move<0x111, 0xF>(v);  // row_shr:1
-    if (dist >= 1 && r >= 1) v = S::oplus(v, t);
+    if (m >= 1) v = S::oplus(v, t);
     t = dpp_move<0x112, 0xF>(v);  // row_shr:2
-    if (dist >= 2 && r >= 2) v = S::oplus(v, t);
+    if (m >= 2) v = S::oplus(v, t);
     t = dpp_move<0x114, 0xF>(v);  // row_shr:4
-    if (dist >= 4 && r >= 4) v = S::oplus(v, t);
+    if (m >= 4) v = S::oplus(v, t);
     t = dpp_move<0x118, 0xF>(v);  // row_shr:8
-    if (dist >= 8 && r >= 8) v = S::oplus(v, t);
+    if (m >= 8) v = S::oplus(v, t);
     t = dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
     if ((lane & 16) && dist > r) v = S::oplus(v, t);
     t = dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
@@ -1590,7 +1591,6 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
     const int nwave = blockDim.x >> 6;
     T* Y = tp<S>(a.Y);
     const unsigned* loc = reinterpret_cast<const unsigned*>(a.xsym);  // key index inside the list's range
-    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // bits 0..lane
     const long N = a.xpoff[a.nbl];
     const long g0 = N * blockIdx.x / gridDim.x, g1 = N * (blockIdx.x + 1) / gridDim.x;
     int p = g0 < g1 ? list_of_chunk(a, g0, reinterpret_cast<unsigned*>(dyn_lds)) : 0;
@@ -1642,9 +1642,10 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
                 if ((l[j] & 0xFFFFu) >= static_cast<unsigned>(span)) v[j] = S::zero();  // padding
                 const unsigned long long bits = fb[j];
                 // Segmented scan over the lanes of each row piece (DPP; the K
-                // chunks' steps interleave).
-                const unsigned long long st = bits & upto;
-                const int dist = lane - (st ? 63 - __clzll(st) : 0);
+                // chunks' steps interleave). dist: lanes back to the last piece
+                // start at or below this lane (all of them when there is none).
+                const unsigned long long sh = bits << (63 - lane);
+                const int dist = sh ? static_cast<int>(__clzll(sh)) : lane;
                 v[j] = seg_scan<S>(v[j], dist, lane);
                 const bool ends = lane == 63 || ((bits >> (lane + 1)) & 1ull) != 0ull;
                 end[j] = c + j < c1 && ends && !S::skip(v[j]);
