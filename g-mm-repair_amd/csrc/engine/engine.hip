@@ -756,9 +756,24 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
     if (a.bits) {
         const long init = std::max<long>(static_cast<long>(a.tw) * 64, std::max<long>(a.rows, (en.NL + 63) / 64));
         hipLaunchKernelGGL(dev::k_bits_init, dim3(grid_for(init, block, 4096)), dim3(block), 0, st, a);
+        // GMR_RUN_PAIRED_LEVELS: levels two to a launch (each pair's time is its lower level's)
+        const bool paired = (o.flags & GMR_RUN_PAIRED_LEVELS) != 0;
         for (int k = 1; k <= en.L; ++k) {
             const long lo = en.level_off[k - 1], hi = en.level_off[k];
             if (level_ev) GMR_HIP(hipEventRecord(level_ev[2 * k - 1], st));
+            if (paired && k + 1 <= en.L) {
+                const long hi2 = en.level_off[k + 1];
+                if (hi2 > lo)
+                    hipLaunchKernelGGL(dev::k_bits_eval_pair, dim3(grid_for(((hi2 + 63) / 64 - lo / 64) * 64, block)),
+                                       dim3(block), 0, st, a, k);
+                if (level_ev) {
+                    GMR_HIP(hipEventRecord(level_ev[2 * k], st));
+                    GMR_HIP(hipEventRecord(level_ev[2 * k + 1], st));
+                    GMR_HIP(hipEventRecord(level_ev[2 * k + 2], st));
+                }
+                ++k;
+                continue;
+            }
             if (hi > lo)
                 hipLaunchKernelGGL(dev::k_bits_eval, dim3(grid_for(((hi + 63) / 64 - lo / 64) * 64, block)), dim3(block),
                                    0, st, a, k);
@@ -1023,10 +1038,12 @@ SweepArgs make_args(Engine& en, const float* X, int B, void* Y, int vbytes, int 
     if (o.flags & GMR_RUN_TRANSPOSED)
         throw std::invalid_argument("GMR_RUN_TRANSPOSED applies to gmr_engine_traverse only");
     if ((o.flags & (GMR_RUN_FUSED_TAIL | GMR_RUN_PAIRED_LEVELS)) &&
-        (!en.resident || B != 1 || o.variant == GMR_VARIANT_PERSISTENT || (o.flags & GMR_RUN_BITS)))
+        (!en.resident || B != 1 || o.variant == GMR_VARIANT_PERSISTENT))
         throw std::invalid_argument(
             "GMR_RUN_FUSED_TAIL and GMR_RUN_PAIRED_LEVELS need one vector, the Resident layout and the levels or graph "
-            "variant (not the bit mode)");
+            "variant");
+    if ((o.flags & GMR_RUN_FUSED_TAIL) && (o.flags & GMR_RUN_BITS))
+        throw std::invalid_argument("GMR_RUN_FUSED_TAIL does not apply to the bit mode (GMR_RUN_PAIRED_LEVELS does)");
     if ((o.flags & GMR_RUN_FUSED_TAIL) && !en.ws_tail) {
         en.forbid_alloc("the fused tail's staging buffer");
         GMR_HIP(hipMalloc(&en.ws_tail, Engine::kTailStageBytes));

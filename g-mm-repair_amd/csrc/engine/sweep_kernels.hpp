@@ -1254,8 +1254,43 @@ __device__ void ph_bits_init(const SweepArgs& a, long tid, long nth) {
     }
     const long rw = (static_cast<long>(a.level_off[a.L]) + 63) >> 6;
     for (long i = tid; i < rw; i += nth) a.vbits[a.tw + i] = 0ull;
+    // y zeroed four rows per store where it is 16-byte aligned
     int* Y = static_cast<int*>(a.Y);
-    for (long i = tid; i < a.rows; i += nth) Y[i] = 0;
+    const long head = (reinterpret_cast<uintptr_t>(Y) & 15u) ? static_cast<long>((16u - (reinterpret_cast<uintptr_t>(Y) & 15u)) >> 2) : 0L;
+    const long h = head < a.rows ? head : static_cast<long>(a.rows);
+    for (long i = tid; i < h; i += nth) Y[i] = 0;
+    int4* Y4 = reinterpret_cast<int4*>(Y + h);
+    const long n4 = (a.rows - h) >> 2;
+    for (long i = tid; i < n4; i += nth) Y4[i] = make_int4(0, 0, 0, 0);
+    for (long i = h + 4 * n4 + tid; i < a.rows; i += nth) Y[i] = 0;
+}
+
+// Levels k and k + 1 of the bit mode in one launch (GMR_RUN_PAIRED_LEVELS): a
+// level k + 1 rule computes each child in level k as the OR of that child's
+// kids' bits (written by earlier launches) instead of reading its bit, so the
+// two levels need no barrier between them. Words wholly inside the pair are
+// stored, words shared with the levels around it ORed in.
+__device__ void ph_bits_eval_pair(const SweepArgs& a, int k, long tid, long nth) {
+    const int lane = static_cast<int>(tid & 63);
+    const long lo = a.level_off[k - 1], mid = a.level_off[k], hi = a.level_off[k + 1];
+    const long s0 = (lo >> 6) << 6, s1 = (hi + 63) >> 6 << 6;
+    for (long s = s0 + tid; s < s1; s += nth) {
+        // every lane loads (slots outside the pair read the nearest one in it)
+        const long sc = s < lo ? lo : (s < hi ? s : hi - 1);
+        const int2 c = a.kids[sc];
+        const bool il = s >= mid && c.x >= lo && c.x < mid, ir = s >= mid && c.y >= lo && c.y < mid;
+        const int2 gl = a.kids[il ? c.x : sc], gr = a.kids[ir ? c.y : sc];
+        const int p0 = il ? gl.x : c.x, p1 = il ? gl.y : c.x, q0 = ir ? gr.x : c.y, q1 = ir ? gr.y : c.y;
+        const bool v = value_bit(a.vbits, a.tw, p0) | value_bit(a.vbits, a.tw, p1) |
+                       value_bit(a.vbits, a.tw, q0) | value_bit(a.vbits, a.tw, q1);
+        const unsigned long long m = __ballot(s >= lo && s < hi && v);
+        if (lane == 0) {
+            const long w = s >> 6;
+            unsigned long long* dst = a.vbits + a.tw + w;
+            if ((w << 6) >= lo && (w << 6) + 64 <= hi) *dst = m;
+            else if (m) atomicOr(dst, m);
+        }
+    }
 }
 
 // Level k: rule slots [lo, hi), swept as the 64-slot words that hold them. A
@@ -1680,6 +1715,10 @@ __global__ void k_bits_init(SweepArgs a) { ph_bits_init(a, xcd_tid(), static_cas
 
 __global__ void k_bits_eval(SweepArgs a, int k) {
     ph_bits_eval(a, k, xcd_tid(), static_cast<long>(gridDim.x) * blockDim.x);
+}
+
+__global__ void k_bits_eval_pair(SweepArgs a, int k) {
+    ph_bits_eval_pair(a, k, xcd_tid(), static_cast<long>(gridDim.x) * blockDim.x);
 }
 
 __global__ void k_bits_gather(SweepArgs a) {

@@ -34,7 +34,8 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
     sizes; one vector in the Resident layout repeats the levels and graph
     schedules (bit mode aside) with the top levels fused into one workgroup
     (GMR_RUN_FUSED_TAIL), the row-gather ones also with small levels two to a
-    launch (GMR_RUN_PAIRED_LEVELS). Persistent shapes are (block, grid cap); a cap of 0 means the
+    launch (GMR_RUN_PAIRED_LEVELS), and the bit mode repeats its schedules
+    with every level pair in one launch. Persistent shapes are (block, grid cap); a cap of 0 means the
     engine's default of 16 waves per CU."""
     emits = [EMIT_SPOT, EMIT_GATHER] if layout == RESIDENT else [EMIT_SPOT]
     tiles = sorted({B} | {t for t in (16, 32, 64, 128) if t < B})
@@ -94,9 +95,12 @@ def candidates(B, layout, blocks=(256,), persistent_shapes=((256, 0), (512, 0), 
             # every levels / graph schedule (bit mode aside) also with its top levels in one
             # workgroup, and the row-gather ones also with their small levels two to a launch
             base_ = [c for c in out if c["variant"] != VARIANT_PERSISTENT and not c.get("flags", 0) & RUN_BITS]
+            bits_ = [c for c in out if c.get("flags", 0) & RUN_BITS]
             out += [dict(c, flags=c.get("flags", 0) | RUN_FUSED_TAIL) for c in base_]
             out += [dict(c, flags=c.get("flags", 0) | RUN_FUSED_TAIL | RUN_PAIRED_LEVELS) for c in base_
                     if c["emit"] == EMIT_GATHER]
+            # the bit mode with its levels two to a launch
+            out += [dict(c, flags=c["flags"] | RUN_PAIRED_LEVELS) for c in bits_]
     return out
 
 

@@ -94,14 +94,16 @@ def test_candidates_for_narrow_tiles_cover_both_gathers():
 
 def test_bit_mode_candidates_only_for_boolean_single_vectors():
     """The Boolean bit mode is searched for one vector with gather emission in
-    the levels and graph variants, and nowhere else."""
+    the levels and graph variants, with and without paired levels, and
+    nowhere else."""
     d = {autotune.describe(c) for c in autotune.candidates(1, gmr.RESIDENT, semiring=gmr.BOOLEAN)}
     bits = {x for x in d if "bits" in x}
     for v in ("levels", "graph"):
         for r in (262144, 524288, 1048576):
             for ch in (4, 8):
                 assert f"{v}/gather/tile=B/blk=256/bits(range={r})/chunks={ch}" in bits
-    assert len(bits) == 12
+                assert f"{v}/gather/tile=B/blk=256/bits(range={r})/paired-levels/chunks={ch}" in bits
+    assert len(bits) == 24
     for B, sr, lay in ((1, gmr.PLUS_TIMES, gmr.RESIDENT), (1, gmr.TROPICAL, gmr.RESIDENT), (4, gmr.BOOLEAN, gmr.RESIDENT),
                        (1, gmr.BOOLEAN, gmr.WAVE64), (1, None, gmr.RESIDENT)):
         assert not any(c.get("flags", 0) & gmr.RUN_BITS for c in autotune.candidates(B, lay, semiring=sr))
@@ -127,6 +129,11 @@ def test_candidates_search_the_level_chunk_sizes():
         assert "levels/spot/tile=B/blk=256/fused-tail" in d
         assert "levels/gather/tile=B/blk=256/lds-ranges(values=40960)/row-order/fused-tail/paired-levels/chunks=4" in d
         assert not any("paired-levels" in x for x in d if "/spot/" in x)
+        bits = {x for x in d if "/bits(" in x}
+        assert bool(bits) == (sr == gmr.BOOLEAN)
+        assert all(x.replace("/paired-levels", "") in d for x in bits)
+        if sr == gmr.BOOLEAN:
+            assert "levels/gather/tile=B/blk=256/bits(range=262144)/paired-levels/chunks=8" in d
         assert not any("fused-tail" in x for x in d if x.startswith("persistent") or "/bits(" in x)
         assert not any("lds-ranges" in x for x in d if x.startswith("persistent"))
     for B in (4, 64):

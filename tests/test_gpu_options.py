@@ -161,7 +161,8 @@ def test_fused_tail_refusals(hap):
     for kw in (dict(variant=gmr.VARIANT_PERSISTENT, flags=gmr.RUN_FUSED_TAIL),
                dict(variant=gmr.VARIANT_PERSISTENT, flags=gmr.RUN_PAIRED_LEVELS),
                dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_FUSED_TAIL | gmr.RUN_BITS),
-               dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_PAIRED_LEVELS | gmr.RUN_BITS)):
+               dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_FUSED_TAIL | gmr.RUN_PAIRED_LEVELS | gmr.RUN_BITS),
+               dict(variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_GATHER, flags=gmr.RUN_PAIRED_LEVELS | gmr.RUN_BITS)):
         with pytest.raises(gmr.GmrError) as e:
             eng.run_torch(1, x, **kw)
         assert e.value.code == gmr.ERR_INVALID, kw
@@ -309,6 +310,11 @@ def test_level_chunk_options(hap, geno):
                 _say(name, "bits", rng, ch)
                 y = run(eng, 1, xb, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS, lds_values=rng, chunks=ch)
                 check(1, y, yb_star, None, f"{name} bits range={rng} chunks={ch}")
+        # the bit mode's levels two to a launch (GMR_RUN_PAIRED_LEVELS), levels and graph
+        for var in (gmr.VARIANT_LEVELS, gmr.VARIANT_GRAPH):
+            _say(name, "bits paired", var)
+            y = run(eng, 1, xb, variant=var, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS | gmr.RUN_PAIRED_LEVELS)
+            check(1, y, yb_star, None, f"{name} bits paired var={var}")
 
 
 def test_option_errors(hap):

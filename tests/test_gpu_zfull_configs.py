@@ -156,6 +156,7 @@ def test_config4a_boolean_reachability():
     for label, eng in forms:
         for kw in (dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS),
                    dict(variant=gmr.VARIANT_GRAPH, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS, lds_values=262144),
+                   dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS | gmr.RUN_PAIRED_LEVELS, lds_values=262144, chunks=8),
                    dict(emit=gmr.EMIT_GATHER),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER),
