@@ -901,7 +901,7 @@ def run_rmat(args):
                     [dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_ROW_SEGMENTS | gmr.RUN_FUSED_TAIL | T_),
                      dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_FUSED_TAIL | T_),
                      dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_FUSED_TAIL | T_,
-                          lds_values=32768, chunks=4)])
+                          lds_values=16384, chunks=2)])
             truns = []
             for lab, gg in (("as built", gt), ("top rules used <= 8 times in C inlined", gt.inline_top(8))):
                 et = gg.schedule(n, n, "resident").engine()

@@ -864,12 +864,21 @@ void launch_levels(const Engine& en, const SweepArgs& a, const gmr_run_opts& o, 
         // one LDS range per list; workgroups take equal shares of all chunks
         const long nch = en.lparts.at(2 * a.lspan + a.lorder).poff.back();
         const std::size_t lds = static_cast<std::size_t>(a.lspan) * sizeof(typename S::T);
-        const unsigned g = lds_gather_grid(en, nch, lds);
+        // two workgroups per CU where two ranges fit in one CU's LDS
+        const bool two = 2 * lds <= static_cast<std::size_t>(en.max_lds);
+        const unsigned g = static_cast<unsigned>(
+            std::max(1L, std::min<long>((two ? 2 : 1) * static_cast<long>(en.num_cus), (nch + 63) / 64)));
         if (nch > 0) {
             const dim3 gr(g), bl(dev::kLdsGatherBlock);
-            if (a.chunks == 2) hipLaunchKernelGGL((dev::k_gather_lds<S, 2>), gr, bl, lds, st, a);
-            else if (a.chunks == 8) hipLaunchKernelGGL((dev::k_gather_lds<S, 8>), gr, bl, lds, st, a);
-            else hipLaunchKernelGGL((dev::k_gather_lds<S, 4>), gr, bl, lds, st, a);
+            if (two) {
+                if (a.chunks == 2) hipLaunchKernelGGL((dev::k_gather_lds<S, 2, 8>), gr, bl, lds, st, a);
+                else if (a.chunks == 8) hipLaunchKernelGGL((dev::k_gather_lds<S, 8, 8>), gr, bl, lds, st, a);
+                else hipLaunchKernelGGL((dev::k_gather_lds<S, 4, 8>), gr, bl, lds, st, a);
+            } else {
+                if (a.chunks == 2) hipLaunchKernelGGL((dev::k_gather_lds<S, 2, 4>), gr, bl, lds, st, a);
+                else if (a.chunks == 8) hipLaunchKernelGGL((dev::k_gather_lds<S, 8, 4>), gr, bl, lds, st, a);
+                else hipLaunchKernelGGL((dev::k_gather_lds<S, 4, 4>), gr, bl, lds, st, a);
+            }
         }
         // rows in the lists' order: y into the caller's Y in matrix row order
         if (a.lorder) hipLaunchKernelGGL(dev::k_rows_to_matrix_order<S>, dim3(grid_for(a.rows, block)), dim3(block), 0, st, a);

@@ -1281,9 +1281,9 @@ __device__ void ph_bits_eval_pair(const SweepArgs& a, int k, long tid, long nth)
         const bool il = s >= mid && c.x >= lo && c.x < mid, ir = s >= mid && c.y >= lo && c.y < mid;
         const int2 gl = a.kids[il ? c.x : sc], gr = a.kids[ir ? c.y : sc];
         const int p0 = il ? gl.x : c.x, p1 = il ? gl.y : c.x, q0 = ir ? gr.x : c.y, q1 = ir ? gr.y : c.y;
-        const bool v = value_bit(a.vbits, a.tw, p0) | value_bit(a.vbits, a.tw, p1) |
-                       value_bit(a.vbits, a.tw, q0) | value_bit(a.vbits, a.tw, q1);
-        const unsigned long long m = __ballot(s >= lo && s < hi && v);
+        const int v = static_cast<int>(value_bit(a.vbits, a.tw, p0)) | static_cast<int>(value_bit(a.vbits, a.tw, p1)) |
+                      static_cast<int>(value_bit(a.vbits, a.tw, q0)) | static_cast<int>(value_bit(a.vbits, a.tw, q1));
+        const unsigned long long m = __ballot(s >= lo && s < hi && v != 0);
         if (lane == 0) {
             const long w = s >> 6;
             unsigned long long* dst = a.vbits + a.tw + w;
@@ -1561,39 +1561,38 @@ constexpr int kLdsGatherBlock = 1024;
 // one-vector LDS range: terminal leaves computed from their value and x
 // (terminal 0 is the semiring zero), rule values copied from W, zero past the
 // last key. Every load is clamped into its array, so none is conditional, and
-// a lane issues kFillBatch of each before using any.
-// A range of rule values only (no terminal key) is filled kRuleFill values per
-// lane at a time: its loads do not depend on other loads, so one round trip
-// fills a 40,960-value range of a 1,024-lane workgroup.
-constexpr int kRuleFill = 40;
-
-template <class S>
+// a lane issues FB of each before using any.
+// A range of rule values only (no terminal key) is filled RF values per lane
+// at a time: its loads do not depend on other loads, so with RF = 40 one round
+// trip fills a 40,960-value range of a 1,024-lane workgroup. Terminal ranges
+// take FB values of each array per lane per round trip.
+template <class S, int FB, int RF>
 __device__ __forceinline__ void fill_range(const SweepArgs& a, typename S::T* lds, long k0, int span) {
     using T = typename S::T;
     const T* W = tp<S>(a.W0);
     const long U = a.U, nw = a.nkey - a.U;  // terminal keys; rule values held in W
     if (k0 >= U && nw > 0) {
-        for (int i0 = threadIdx.x; i0 < span; i0 += kRuleFill * blockDim.x) {
-            T wv[kRuleFill];
+        for (int i0 = threadIdx.x; i0 < span; i0 += RF * blockDim.x) {
+            T wv[RF];
 #pragma unroll
-            for (int u = 0; u < kRuleFill; ++u) {
+            for (int u = 0; u < RF; ++u) {
                 const long kw = k0 + i0 + static_cast<long>(u) * blockDim.x - U;
                 wv[u] = W[kw < nw ? kw : nw - 1];
             }
 #pragma unroll
-            for (int u = 0; u < kRuleFill; ++u) {
+            for (int u = 0; u < RF; ++u) {
                 const int i = i0 + u * static_cast<int>(blockDim.x);
                 if (i < span) lds[i] = k0 + i < a.nkey ? wv[u] : S::zero();
             }
         }
         return;
     }
-    for (int i0 = threadIdx.x; i0 < span; i0 += kFillBatch * blockDim.x) {
-        float tv[kFillBatch];
-        int tc[kFillBatch];
-        T wv[kFillBatch];
+    for (int i0 = threadIdx.x; i0 < span; i0 += FB * blockDim.x) {
+        float tv[FB];
+        int tc[FB];
+        T wv[FB];
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) {
+        for (int u = 0; u < FB; ++u) {
             const long k = k0 + i0 + static_cast<long>(u) * blockDim.x;
             const long kt = k < U ? k : U - 1;
             tv[u] = a.tval[kt];
@@ -1601,11 +1600,11 @@ __device__ __forceinline__ void fill_range(const SweepArgs& a, typename S::T* ld
             const long kw = k - U;
             wv[u] = nw > 0 ? W[kw < 0 ? 0 : (kw < nw ? kw : nw - 1)] : S::zero();
         }
-        float xv[kFillBatch];
+        float xv[FB];
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) xv[u] = a.X[static_cast<long>(tc[u]) * a.B];
+        for (int u = 0; u < FB; ++u) xv[u] = a.X[static_cast<long>(tc[u]) * a.B];
 #pragma unroll
-        for (int u = 0; u < kFillBatch; ++u) {
+        for (int u = 0; u < FB; ++u) {
             const int i = i0 + u * static_cast<int>(blockDim.x);
             const long k = k0 + i;
             if (i < span) lds[i] = k < U ? (k == 0 ? S::zero() : S::leaf(tv[u], xv[u])) : (k < a.nkey ? wv[u] : S::zero());
@@ -1616,8 +1615,13 @@ __device__ __forceinline__ void fill_range(const SweepArgs& a, typename S::T* ld
 // Every entry carries its row as an offset from its chunk's first row (the high
 // 16 bits; the index inside the range is the low 16 bits), so the row of a
 // piece comes from the entry that ends it, with no dependent load.
-template <class S, int K>
-__global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
+// WPS: waves per SIMD the launch bounds ask for. 4 is one 1,024-lane
+// workgroup per CU (ranges above 80 KiB), with fills of 16 terminal or 40 rule
+// values per lane per round trip; 8 is two workgroups per CU (ranges up to 80
+// KiB each), which caps a lane at 64 VGPRs and so takes 8 and 24.
+template <class S, int K, int WPS>
+__global__ __launch_bounds__(kLdsGatherBlock, WPS) void k_gather_lds(SweepArgs a) {
+    static_assert(WPS == 4 || WPS == 8, "one or two workgroups per CU");
     using T = typename S::T;
     T* lds = reinterpret_cast<T*>(dyn_lds);
     const int span = a.lspan;
@@ -1634,7 +1638,8 @@ __global__ __launch_bounds__(kLdsGatherBlock) void k_gather_lds(SweepArgs a) {
         const long c1 = a.xpoff[p + 1] < g1 ? static_cast<long>(a.xpoff[p + 1]) : g1;
         __syncthreads();  // the previous range's lookups are done
         const long k0 = static_cast<long>(p) * span;
-        fill_range<S>(a, lds, k0, span);  // (the init phase computed only the terminal values rules read)
+        // (the init phase computed only the terminal values rules read)
+        fill_range<S, WPS == 4 ? 16 : 8, WPS == 4 ? 40 : 24>(a, lds, k0, span);
         __syncthreads();
         // As in the bit-mode gather: unconditional, clamped loads, and the next
         // iteration's chunk loads issued before this one's emits.

@@ -80,11 +80,12 @@ enum gmr_run_flags {
      * levels), run in one workgroup that keeps their values in LDS (160 KiB), after a launch
      * that stages the values of their children below them: two launches instead of one per
      * level. Without such levels, or where a workgroup may not hold that much LDS, the sweep
-     * runs as without the flag. */
+     * runs as without the flag. Not with GMR_RUN_BITS. */
     GMR_RUN_FUSED_TAIL = 64,
     /* One vector, Resident layout, levels or graph variant: levels run two to a launch; a rule
      * of the upper level of a pair computes its children in the lower level from their own
-     * kids instead of waiting for them (levels below the fused tail, with GMR_RUN_FUSED_TAIL). */
+     * kids instead of waiting for them (levels below the fused tail, with GMR_RUN_FUSED_TAIL;
+     * with GMR_RUN_BITS every level, the bits of a child being the OR of its kids' bits). */
     GMR_RUN_PAIRED_LEVELS = 128,
     /* gmr_engine_traverse only: the engine holds the transposed matrix, so every step is the
      * right product y = M^T x, with the one-vector gathers of the right product (GMR_RUN_BITS,
@@ -108,7 +109,8 @@ typedef struct gmr_run_opts {
     /* Level-chunk sizes (0: the defaults below; the autotuner searches them):
      * lds_values  values a workgroup stages in LDS, a multiple of 1024: with GMR_RUN_LDS_RANGES
      *             the values per range (default 32768; at least 4096, below 65535, at most what
-     *             one workgroup's LDS holds); with GMR_RUN_BITS the bits per range (default 524288,
+     *             one workgroup's LDS holds; ranges of which two fit one CU's LDS, up to 20480
+     *             4-byte values on MI355X, run two workgroups per CU); with GMR_RUN_BITS the bits per range (default 524288,
      *             131072..1048576); otherwise the left-child window of GMR_EVAL_LDS levels in the
      *             levels and graph variants (default 4096, 1024..16384)
      * chunks      64-symbol chunks a wave of a chunked single-vector gather has in flight:

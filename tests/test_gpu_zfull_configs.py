@@ -183,6 +183,7 @@ def test_config4b_minplus_relaxation():
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER | gmr.RUN_FUSED_TAIL),
                    dict(emit=gmr.EMIT_GATHER, lds_values=32768,
                         flags=gmr.RUN_LDS_RANGES | gmr.RUN_FUSED_TAIL | gmr.RUN_PAIRED_LEVELS),
+                   dict(emit=gmr.EMIT_GATHER, lds_values=16384, chunks=2, flags=gmr.RUN_LDS_RANGES),
                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_SINGLE_LIST),
                    dict(emit=gmr.EMIT_SPOT),
                    dict(variant=gmr.VARIANT_PERSISTENT, emit=gmr.EMIT_SPOT)):
