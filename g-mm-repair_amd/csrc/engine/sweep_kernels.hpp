@@ -1587,6 +1587,29 @@ __device__ __forceinline__ void fill_range(const SweepArgs& a, typename S::T* ld
         }
         return;
     }
+    if (k0 + span <= U) {
+        // terminal keys only: no rule value to load
+        for (int i0 = threadIdx.x; i0 < span; i0 += FB * blockDim.x) {
+            float tv[FB];
+            int tc[FB];
+#pragma unroll
+            for (int u = 0; u < FB; ++u) {
+                const long k = k0 + i0 + static_cast<long>(u) * blockDim.x;
+                const long kt = k < U ? k : U - 1;
+                tv[u] = a.tval[kt];
+                tc[u] = a.tcol[kt];
+            }
+            float xv[FB];
+#pragma unroll
+            for (int u = 0; u < FB; ++u) xv[u] = a.X[static_cast<long>(tc[u]) * a.B];
+#pragma unroll
+            for (int u = 0; u < FB; ++u) {
+                const int i = i0 + u * static_cast<int>(blockDim.x);
+                if (i < span) lds[i] = k0 + i == 0 ? S::zero() : S::leaf(tv[u], xv[u]);
+            }
+        }
+        return;
+    }
     for (int i0 = threadIdx.x; i0 < span; i0 += FB * blockDim.x) {
         float tv[FB];
         int tc[FB];

@@ -232,13 +232,16 @@ def test_lds_range_fills_over_renumbered_terminals():
     """The engine numbers terminals in column order and the range fills compute
     their values from that numbering: terminals spread thinly over many columns
     (one non-zero every 150 of 300,000 columns), more than 65,536 distinct
-    values, and a small dense matrix, against the oracle."""
+    values, 60,000 distinct values over dense columns and a small dense matrix,
+    at range sizes of two workgroups and of one workgroup per CU, against the
+    oracle."""
     rng = np.random.default_rng(77)
     cases = []
     M = np.zeros((6, 300000))
     M[rng.integers(0, 6, 2000), np.arange(2000) * 150] = rng.integers(1, 4, 2000)
     cases.append(("sparse columns", M))
     cases.append(("many values", rng.uniform(1.0, 2.0, (8, 9000)).round(7)))
+    cases.append(("60,000 values", rng.integers(1, 60001, (4, 30000)).astype(np.float64)))
     cases.append(("dense", rng.integers(0, 3, (500, 3000)).astype(np.float64)))
     for name, M in cases:
         m = gmr.Matrix.from_dense(M)
@@ -248,9 +251,10 @@ def test_lds_range_fills_over_renumbered_terminals():
         for sr in (0, 1, 2):
             x = x_for(sr, cols, 1200 + sr)
             y_star, den = O.brute_force(sr, rows, cols, seq, V, x)
-            _say(name, sr, len(V), "values")
-            y = run(eng, sr, x, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=4096, chunks=4)
-            check(sr, y, y_star, den, f"{name} lds ranges")
+            for span in (4096, 40960):
+                _say(name, sr, len(V), "values", span)
+                y = run(eng, sr, x, emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES, lds_values=span, chunks=4)
+                check(sr, y, y_star, den, f"{name} lds ranges span={span}")
 
 
 def test_graph_cache_keys_on_every_level_chunk_option():
