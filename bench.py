@@ -842,9 +842,10 @@ def run_rmat(args):
                   "gather_GBps": round(gbytes / (float(ph[-1]) * 1e-3) / 1e9, 1)}
         log(tag, "phases", phases)
         # the gather kernel against HBM peak; traffic as counted by rocprofv3 for the
-        # Boolean bit-mode gather on the grammar as built (profiles/traffic.txt)
+        # same gather kernel and grammar form (profiles/traffic.txt)
         bits = bool(best.get("flags", 0) & gmr.RUN_BITS) and not (best.get("flags", 0) & gmr.RUN_SINGLE_LIST)
-        kernel = "k_bits_gather_lds" if bits else "k_gather"
+        kernel = ("k_bits_gather_lds" if bits else
+                  "k_gather_lds" if best.get("flags", 0) & gmr.RUN_LDS_RANGES else "k_gather")
         roof = {"kernel": kernel, "bound": "hbm", "achieved": phases["gather_GBps"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(phases["gather_GBps"] / HBM_PEAK_GBS, 4),
                 "traffic": read_traffic(f"rmat{scale}_{tag}" + (f"_inline{form['max_emits']}" if form["max_emits"] else ""),
