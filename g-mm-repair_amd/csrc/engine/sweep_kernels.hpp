@@ -1703,6 +1703,10 @@ __global__ __launch_bounds__(kLdsGatherBlock, WPS) void k_gather_lds(SweepArgs a
 #pragma unroll
             for (int j = 0; j < K; ++j) {
                 if ((l[j] & 0xFFFFu) >= static_cast<unsigned>(span)) v[j] = S::zero();  // padding
+                end[j] = false;
+                // A chunk whose values are all ones the emission skips (the
+                // semiring zero of min/plus and Boolean) has no piece to emit.
+                if (__ballot(!S::skip(v[j])) == 0ull) continue;
                 const unsigned long long bits = fb[j];
                 // Segmented scan over the lanes of each row piece (DPP; the K
                 // chunks' steps interleave). dist: lanes back to the last piece
