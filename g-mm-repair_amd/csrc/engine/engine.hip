@@ -1733,7 +1733,9 @@ int Engine::traverse(int semiring, float* x, int max_iters, const gmr_run_opts& 
         throw std::invalid_argument("traversal runs in the Boolean or Tropical semiring");
     if (!x) throw std::invalid_argument("x is NULL");
     if (max_iters < 1) throw std::invalid_argument("max_iters must be >= 1");
-    const std::size_t need = static_cast<std::size_t>(std::max(rows, 1)) * sizeof(int) + 256;
+    // y, then the frontier (the vertices the last step moved), then the flag
+    const std::size_t vb = (static_cast<std::size_t>(std::max(rows, 1)) * sizeof(int) + 255) / 256 * 256;
+    const std::size_t need = 2 * vb + 256;
     if (need > ws_trav_bytes) {
         GMR_HIP(hipStreamSynchronize(st));
         if (ws_trav) GMR_HIP(hipFree(ws_trav));
@@ -1742,22 +1744,29 @@ int Engine::traverse(int semiring, float* x, int max_iters, const gmr_run_opts& 
         ws_trav_bytes = need;
     }
     void* y = ws_trav;
-    int* flag = reinterpret_cast<int*>(static_cast<char*>(ws_trav) + (need - 256));
+    float* delta = reinterpret_cast<float*>(static_cast<char*>(ws_trav) + vb);
+    int* flag = reinterpret_cast<int*>(static_cast<char*>(ws_trav) + 2 * vb);
     const int block = 256;
     const unsigned grid = grid_for(std::max(rows, 1), block);
     // with GMR_RUN_TRANSPOSED the engine holds M^T: x^T M = (M^T x)^T, a right product
     const bool right = (o.flags & GMR_RUN_TRANSPOSED) != 0;
     gmr_run_opts op = o;
     op.flags &= ~GMR_RUN_TRANSPOSED;
+    // Both semirings are idempotent, so a step's product needs only the vertices
+    // the step before moved (the first step: all of x): a product over a sparse
+    // frontier lets the gathers skip the chunks whose values are all zero.
     int it = 0;
     for (; it < max_iters;) {
-        if (right) run(semiring, x, 1, y, op, st);
-        else run_left(semiring, x, 1, y, op, st);
+        const float* in = it == 0 ? x : delta;
+        if (right) run(semiring, in, 1, y, op, st);
+        else run_left(semiring, in, 1, y, op, st);
         hipLaunchKernelGGL(dev::k_clear_flag, dim3(1), dim3(64), 0, st, flag);
         if (semiring == GMR_BOOLEAN)
-            hipLaunchKernelGGL(dev::k_relax<dev::BooleanOp>, dim3(grid), dim3(block), 0, st, x, y, rows, flag);
+            hipLaunchKernelGGL(dev::k_relax_frontier<dev::BooleanOp>, dim3(grid), dim3(block), 0, st, x, delta, y,
+                               rows, flag);
         else
-            hipLaunchKernelGGL(dev::k_relax<dev::TropicalOp>, dim3(grid), dim3(block), 0, st, x, y, rows, flag);
+            hipLaunchKernelGGL(dev::k_relax_frontier<dev::TropicalOp>, dim3(grid), dim3(block), 0, st, x, delta, y,
+                               rows, flag);
         GMR_HIP(hipGetLastError());
         int h = 0;
         GMR_HIP(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -198,22 +198,31 @@ __global__ void k_left_cols(LeftArgs a, int cols) {
     ph_left_cols<S>(a, cols, tid / a.G, nth / a.G, static_cast<int>(tid % a.G));
 }
 
-// One relaxation step of a fixpoint traversal: x <- x (+) y for every vertex;
-// *changed is set (plain stores of 1) when some x moved.
+// One relaxation step of a fixpoint traversal: x <- x (+) y for every vertex,
+// *changed set (plain stores of 1) when some x moved, and delta the frontier:
+// each vertex that moved (its new value) and the semiring zero elsewhere (0,
+// or +inf for min/plus), so the next product runs over what changed only. With
+// P the step's product and an idempotent (+), x_k (+) P(x_(k-1)) = x_k and
+// P(x_k) = P(x_(k-1)) (+) P(delta_k), hence x_k (+) P(x_k) = x_k (+) P(delta_k):
+// the steps and their count are those of the traversal over all of x.
 template <class S>
-__global__ void k_relax(float* x, const void* y_, int n, int* changed) {
+__global__ void k_relax_frontier(float* x, float* delta, const void* y_, int n, int* changed) {
     using T = typename S::T;
     const T* y = static_cast<const T*>(y_);
+    const float zero_x = static_cast<float>(S::zero());
     const long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long nth = static_cast<long>(gridDim.x) * blockDim.x;
     bool moved = false;
     for (long i = tid; i < n; i += nth) {
         const T old = LeftOps<S>::from_x(x[i]);
         const T nv = S::oplus(old, y[i]);
+        float d = zero_x;
         if (nv != old) {
-            x[i] = static_cast<float>(nv);
+            d = static_cast<float>(nv);
+            x[i] = d;
             moved = true;
         }
+        delta[i] = d;
     }
     if (moved) *changed = 1;
 }

@@ -84,7 +84,9 @@ def test_reachability_and_shortest_paths(variant):
 def test_traversal_by_right_products_of_the_transpose(variant):
     """GMR_RUN_TRANSPOSED: an engine of M^T traverses with right products (the
     bit mode and the LDS-range gather among them) to the same reachable set and
-    distances as the left-product traversal of M, scipy and the oracle."""
+    distances as the left-product traversal of M, scipy and the oracle, in the
+    same number of steps (each step after the first multiplies only the
+    vertices the step before moved)."""
     for nw, sr in ((1, 1), (100, 2)):
         m = gmr.Matrix.rmat(310 + nw, 12, 30000, nweights=nw)
         A, seq, V = _csr(m)
@@ -97,17 +99,23 @@ def test_traversal_by_right_products_of_the_transpose(variant):
             opts = [dict(emit=gmr.EMIT_GATHER), dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS),
                     dict(emit=gmr.EMIT_SPOT)]
         else:
-            x0 = np.full(n, 1e9, np.float32)
+            # +infinity given as INT32_MAX/2, above the 1e9 cap (the frontier's
+            # zero is 1e9 itself)
+            x0 = np.full(n, np.float32(2 ** 31 - 1) // 2, np.float32)
             x0[src] = 0
             opts = [dict(emit=gmr.EMIT_GATHER),
                     dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_FUSED_TAIL, lds_values=4096),
+                    dict(emit=gmr.EMIT_GATHER, flags=gmr.RUN_LDS_RANGES | gmr.RUN_ROW_ORDER, lds_values=16384,
+                         chunks=2),
                     dict(emit=gmr.EMIT_SPOT)]
-        want, want_it = _fixpoint_oracle(sr, n, seq, V, x0)
+        # every value at or above the 1e9 cap is +infinity: the engine leaves an
+        # unreached vertex's INT32_MAX/2 as given, the oracle's min writes 1e9
+        want, want_it = _fixpoint_oracle(sr, n, seq, V, np.minimum(x0, np.float32(1e9)))
         for kw in opts:
             x = torch.from_numpy(x0).cuda()
             kw = dict(kw, flags=kw.get("flags", 0) | gmr.RUN_TRANSPOSED)
             iters = eng_t.traverse_torch(sr, x, variant=gmr.VARIANTS[variant], **kw)
-            got = x.cpu().numpy()
+            got = np.minimum(x.cpu().numpy(), np.float32(1e9))
             print(f"transposed sr={sr} {variant} {kw}: {iters} products", flush=True)
             np.testing.assert_array_equal(got, want)
             assert iters == want_it
