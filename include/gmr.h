@@ -180,8 +180,12 @@ int gmr_engine_run_left(gmr_engine* e, int semiring, const void* d_X, int B, voi
  * `stream` after every product; *iters_out = products computed. Each product
  * is the left product of the engine's matrix M, or, with GMR_RUN_TRANSPOSED in
  * opts->flags, the right product of the engine's matrix when it holds M^T
- * (the same x^T M, by the one-vector right-product gathers). The reference's
- * fixpoint comparison is cuGraph BFS / SSSP (reference cugraph_bench.py:53-61). */
+ * (the same x^T M, by the one-vector right-product gathers). Every product
+ * after the first multiplies only the vertices the step before moved (both
+ * semirings are idempotent, so the steps and their count are unchanged); the
+ * engine holds that frontier beside y (2 x rows x 4 bytes, allocated on the
+ * first call). The reference's fixpoint comparison is cuGraph BFS / SSSP
+ * (reference cugraph_bench.py:53-61). */
 int gmr_engine_traverse(gmr_engine* e, int semiring, void* d_x, int max_iters, int* iters_out,
                         const gmr_run_opts* opts, void* stream);
 /* The persistent variant. Its grid barrier needs every workgroup of the launch
