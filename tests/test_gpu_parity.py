@@ -603,7 +603,7 @@ def test_boolean_bit_mode(hap):
     chunks), an R-MAT graph, a weighted R-MAT graph whose bitset spans three
     LDS ranges, and a grammar without rules; frontiers of 1%, 30% and all
     ones; levels and graph variants, the graph replayed into the same buffers
-    with a new x in between."""
+    with a new x in between; y not 16-byte aligned, with paired levels."""
     m, g, seq, V = hap
     rng = np.random.default_rng(4242)
     rows, cols = 40, 300
@@ -641,6 +641,21 @@ def test_boolean_bit_mode(hap):
             torch.cuda.synchronize()
             y_star, _ = O.brute_force(1, mm.rows, mm.cols, sq, VV, x)
             check(1, Yt.cpu().numpy(), y_star, None, f"bits replay {name} {rep}")
+        # y at 4, 8 and 12 bytes past a 16-byte boundary (the init zeroes y four
+        # rows per store between a head and a tail of single rows), paired levels
+        x = (rng.random(mm.cols) < 0.05).astype(np.float32)
+        y_star, _ = O.brute_force(1, mm.rows, mm.cols, sq, VV, x)
+        Xt = torch.from_numpy(x).cuda()
+        base = torch.full((mm.rows + 8,), 7, dtype=torch.int32, device="cuda")
+        for shift in (1, 2, 3):
+            _say("bits unaligned y", name, shift)
+            base.fill_(7)
+            Yt = base[shift:shift + mm.rows]
+            eng.run_torch(1, Xt, Yt, emit=gmr.EMIT_GATHER, flags=gmr.RUN_BITS | gmr.RUN_PAIRED_LEVELS)
+            torch.cuda.synchronize()
+            check(1, Yt.cpu().numpy(), y_star, None, f"bits unaligned y {name} shift={shift}")
+            got = base.cpu().numpy()
+            assert (got[:shift] == 7).all() and (got[shift + mm.rows:] == 7).all(), "wrote outside y"
     # a grammar without rules: terminals only, empty rows
     t = gmr.Grammar.from_arrays(4, [], [], [1, 2, 0, 0, 3, 0], [1.0])
     eng = t.schedule(3, 3, "resident").engine()
