@@ -19,12 +19,12 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
-#include <type_traits>
 #include <type_traits>
 #include <vector>
 
@@ -692,7 +692,29 @@ using dev::SweepArgs;
 
 // Row width of T and W for a tile of Bt vectors: a multiple of 4 so rows are
 // 16-byte aligned and every lane moves four values (one vector stays 1).
-inline int padded_width(int Bt) { return Bt <= 1 ? 1 : (Bt + 3) / 4 * 4; }
+// 128-byte lines a row of w floats touches, averaged over the offsets rows of
+// that stride take in an array with a line-aligned base.
+inline double row_lines(int w) {
+    const int bytes = 4 * w;
+    int g = 128;
+    while (bytes % g) g >>= 1;  // rows start at the multiples of gcd(bytes, 128)
+    double sum = 0.0;
+    for (int off = 0; off < 128; off += g) sum += (off + bytes + 127) / 128;
+    return sum * g / 128.0;
+}
+
+// Row width of the value arrays for tiles of Bt vectors: a multiple of 4
+// (dwordx4 lanes), or 8, 16 or a multiple of 32 floats where that touches at
+// least 15% fewer 128-byte lines per row (every kernel reads and writes whole
+// padded rows, so the lines a row touches are what a random row access costs:
+// 27 floats in rows of 28 touch 1.75 lines on average, in rows of 32 one).
+inline int padded_width(int Bt) {
+    if (Bt <= 1) return 1;
+    int best = (Bt + 3) / 4 * 4;
+    for (int w : {8, 16, (Bt + 31) / 32 * 32})
+        if (w > best && row_lines(w) * 1.15 <= row_lines(best)) best = w;
+    return best;
+}
 
 // Blocks for `work` threads, rounded up to a multiple of 8 (one share per XCD;
 // the kernels' grid-stride loops leave surplus threads idle).

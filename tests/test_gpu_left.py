@@ -72,7 +72,7 @@ def hap():
 
 
 @pytest.mark.parametrize("sr", [0, 1, 2])
-@pytest.mark.parametrize("B", [1, 5, 64])
+@pytest.mark.parametrize("B", [1, 5, 27, 64])
 def test_haplotype_left_product(hap, sr, B):
     m, g, seq, V, eng = hap
     X = x_for(sr, 4096, 40 + B, B=None if B == 1 else B)

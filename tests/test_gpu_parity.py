@@ -138,8 +138,11 @@ def test_haplotype_single_vector(hap, sr, lay):
         check(sr, y, y_star, den, f"{lay}/{var}/{em}")
 
 
-@pytest.mark.parametrize("B", [4, 16, 64, 256, 3, 7, 100, 125])
+@pytest.mark.parametrize("B", [4, 16, 64, 256, 3, 7, 100, 125, 9, 27, 50, 91])
 def test_haplotype_batched_distinct_columns(hap, B):
+    """B vectors with distinct columns, every layout and plan; 9, 27, 50 and 91
+    vectors run in value rows padded to 16, 32, 64 and 96 floats (fewer
+    128-byte lines per row than the next multiple of 4)."""
     m, g, seq, V = hap
     X = x_for(0, 2048, 7 + B, B=B)
     Y_star, den = O.brute_force_batched(0, 4096, 2048, seq, V, X)
