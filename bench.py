@@ -382,16 +382,18 @@ def main():
         gb1 = (4 * (fi1["emit_total"] + fi1["num_term_emit"]) + 4 * args.rows
                + (4 * (fi1["U"] + fi1["num_layered"]) if lds1 else 0))
         ach1 = gb1 / (float(ph1[-1]) * 1e-3) / 1e9
+        key1 = (f"{args.workload}{args.rows}x{args.cols}_B1"
+                + (f"_inline{form1['max_emits']}" if form1["max_emits"] else ""))
         one_roof = {"kernel": "k_gather_lds" if lds1 else "k_gather", "bound": "hbm", "achieved": round(ach1, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach1 / HBM_PEAK_GBS, 4),
                     "launch_ms": round(float(ph1[-1]), 4), "bytes_per_launch": int(gb1),
-                    "traffic": read_traffic(f"{args.workload}{args.rows}x{args.cols}_B1"
-                                            + (f"_inline{form1['max_emits']}" if form1["max_emits"] else ""),
-                                            "k_gather_lds" if lds1 else "k_gather")}
-        # the whole one-vector sweep on the amended DAG count of the form that runs
+                    "traffic": read_traffic(key1, "k_gather_lds" if lds1 else "k_gather")}
+        # the whole one-vector sweep on the amended DAG count of the form that runs,
+        # and the traffic rocprofv3 counted for every launch of one such sweep
         sb1 = amended_bytes(form1["schedule"].info(), 1)
         one_roof["sweep"] = {"ms": one_ms, "bytes": int(sb1), "achieved": round(sb1 / (one_ms * 1e-3) / 1e9, 1),
-                             "frac": round(sb1 / (one_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                             "frac": round(sb1 / (one_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "traffic": read_traffic(key1, "sweep")}
         log("best single vector", autotune.describe(best1), "gather", one_roof)
     del eng_ref
 
