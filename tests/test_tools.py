@@ -68,3 +68,12 @@ def test_bench_measures_one_gpu_only():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_level_times_refuses_without_a_gpu():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from gmr import level_times
+    with pytest.raises(SystemExit, match="needs a GPU"):
+        level_times.main(["--rows", "64", "--cols", "32"])
