@@ -52,6 +52,7 @@ int semiring_from_env() {
 struct Resident {
     std::vector<LayeredRule> rules;
     std::vector<int> level_off, emit_off, emit_pos, emit_row;
+    std::vector<int> term_sym, term_row;  // terminal emissions (index into T, row)
     int L = 0, max_width = 0;
 };
 
@@ -121,6 +122,95 @@ Resident to_resident(int total, const LayeredRule* r, const GPUSchedule& s) {
     o.emit_pos.swap(ep);
     o.emit_row.swap(er);
     return o;
+}
+
+// Top rules written out in the top sequence: a rule that no rule references,
+// that C uses 1..max_emits times, with coeff 1 and no zero-terminal child, is
+// removed, and each of its emissions becomes the emissions of its two children
+// (a terminal child: a terminal emission). A row's result is unchanged in
+// every semiring, whose (+) is associative and commutative: the emission of
+// N -> A B added v(A) (+) v(B), the two emissions add the same. It trades a
+// node update (read A and B, write N) and e reads of N for 2e reads of the
+// children in the row gather; the thresholds are those the autotuner picks on
+// the 500,000 x 10,000 genotype grammar (one vector: 8; two to four and 64 or
+// more vectors: 2; between: 0, the grammar as the reference scheduled it).
+void inline_top(Resident& o, int max_emits) {
+    const int n = static_cast<int>(o.rules.size());
+    if (max_emits <= 0 || n == 0) return;
+    std::vector<int> refs(n, 0), emits(n, 0);
+    for (const LayeredRule& r : o.rules) {
+        if (r.left >= 0) ++refs[r.left];
+        if (r.right >= 0) ++refs[r.right];
+    }
+    for (int p : o.emit_pos) ++emits[p];
+    std::vector<char> sel(n, 0);
+    std::vector<int> new_id(n, -1);
+    int kept = 0;
+    for (int i = 0; i < n; ++i) {
+        const LayeredRule& r = o.rules[i];
+        sel[i] = refs[i] == 0 && emits[i] >= 1 && emits[i] <= max_emits && r.coeff == 1.0f && r.left != -1 &&
+                 r.right != -1;
+        if (!sel[i]) new_id[i] = kept++;
+    }
+    if (kept == n) return;
+    // levels of the old slots; children stay (a referenced rule is kept)
+    std::vector<int> level(n);
+    for (int k = 1; k <= o.L; ++k)
+        for (int i = o.level_off[k - 1]; i < o.level_off[k]; ++i) level[i] = k;
+    Resident t;
+    t.rules.reserve(kept);
+    t.level_off.assign(o.L + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        if (sel[i]) continue;
+        LayeredRule r = o.rules[i];
+        if (r.left >= 0) r.left = new_id[r.left];
+        if (r.right >= 0) r.right = new_id[r.right];
+        t.rules.push_back(r);
+        ++t.level_off[level[i]];
+    }
+    for (int k = 1; k <= o.L; ++k) t.level_off[k] += t.level_off[k - 1];
+    // emissions, grouped by the level that owns their slot
+    std::vector<int> ep, er, lv;
+    ep.reserve(o.emit_pos.size() + o.emit_pos.size() / 4);
+    auto emit = [&](int c, int row) {
+        if (c >= 0) {
+            ep.push_back(new_id[c]);
+            er.push_back(row);
+            lv.push_back(level[c]);
+        } else {
+            o.term_sym.push_back(-c - 1);
+            o.term_row.push_back(row);
+        }
+    };
+    for (std::size_t e = 0; e < o.emit_pos.size(); ++e) {
+        const int p = o.emit_pos[e], row = o.emit_row[e];
+        if (sel[p]) {
+            emit(o.rules[p].left, row);
+            emit(o.rules[p].right, row);
+        } else {
+            emit(p, row);
+        }
+    }
+    t.emit_off.assign(o.L + 1, 0);
+    for (int k : lv) ++t.emit_off[k];
+    for (int k = 1; k <= o.L; ++k) t.emit_off[k] += t.emit_off[k - 1];
+    std::vector<int> fill(t.emit_off.begin(), t.emit_off.end() - 1);
+    t.emit_pos.resize(ep.size());
+    t.emit_row.resize(ep.size());
+    for (std::size_t e = 0; e < ep.size(); ++e) {
+        const int d = fill[lv[e] - 1]++;
+        t.emit_pos[d] = ep[e];
+        t.emit_row[d] = er[e];
+    }
+    // only top levels can empty (a rule of level k + 1 references one of level k)
+    t.L = o.L;
+    while (t.L > 0 && t.level_off[t.L] == t.level_off[t.L - 1]) --t.L;
+    t.level_off.resize(t.L + 1);
+    t.emit_off.resize(t.L + 1);
+    for (int k = 1; k <= t.L; ++k) t.max_width = std::max(t.max_width, t.level_off[k] - t.level_off[k - 1]);
+    t.term_sym.swap(o.term_sym);
+    t.term_row.swap(o.term_row);
+    o = std::move(t);
 }
 
 // Energy per vector, as the reference's NVML loop (gpu_engine.cu:286-330):
@@ -233,6 +323,9 @@ void dropin(int rows, int cols, int alpha, int total, const LayeredRule* h_rules
     hip_check(hipMemGetInfo(&free0, &tot0), "hipMemGetInfo");
 
     Resident res = to_resident(total, h_rules, s);
+    res.term_sym.assign(s.term_emit_sym, s.term_emit_sym + s.num_term_emit);
+    res.term_row.assign(s.term_emit_row, s.term_emit_row + s.num_term_emit);
+    inline_top(res, B == 1 ? 8 : (B <= 4 || B >= 64) ? 2 : 0);
     GPUSchedule rs{};
     rs.max_depth = res.L;
     rs.max_width = res.max_width;
@@ -244,9 +337,9 @@ void dropin(int rows, int cols, int alpha, int total, const LayeredRule* h_rules
     rs.emit_pos = res.emit_pos.data();
     rs.emit_row = res.emit_row.data();
     rs.emit_total = static_cast<int>(res.emit_pos.size());
-    rs.term_emit_sym = s.term_emit_sym;
-    rs.term_emit_row = s.term_emit_row;
-    rs.num_term_emit = s.num_term_emit;
+    rs.term_emit_sym = res.term_sym.data();
+    rs.term_emit_row = res.term_row.data();
+    rs.num_term_emit = static_cast<int>(res.term_sym.size());
     gmr_engine* eng = nullptr;
     if (gmr_engine_create(rows, cols, alpha, static_cast<int>(res.rules.size()), res.rules.data(), h_val, h_col, h_V,
                           nvals, &rs, GMR_LAYOUT_RESIDENT, &eng) != GMR_OK)

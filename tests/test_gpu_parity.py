@@ -313,11 +313,15 @@ def test_dropin_entry_points(hap, monkeypatch):
         _say("dropin layered", env)
         y = _dropin(False, s, 4096, 2048, V, x, 1, env, monkeypatch)
         check(sr, y, y_star, den, f"dropin {env}")
-    X = x_for(0, 2048, 400, B=8)
-    Y_star, den = O.brute_force_batched(0, 4096, 2048, seq, V, X)
-    _say("dropin batched")
-    Y = _dropin(True, s, 4096, 2048, V, X.reshape(-1), 8, "plustimes", monkeypatch).reshape(4096, 8)
-    check(0, Y, Y_star, den, "dropin batched")
+    # the drop-in writes top rules out in C at B = 1 (up to 8 uses), 2-4 and >= 64 (up to 2 uses);
+    # B = 8 runs the grammar as scheduled
+    for sr, env, B in ((0, "plustimes", 4), (0, "plustimes", 8), (0, "plustimes", 64), (1, "boolean", 4),
+                       (2, "tropical", 64)):
+        X = x_for(sr, 2048, 400 + B + sr, B=B)
+        Y_star, den = O.brute_force_batched(sr, 4096, 2048, seq, V, X)
+        _say("dropin batched", env, B)
+        Y = _dropin(True, s, 4096, 2048, V, X.reshape(-1), B, env, monkeypatch).reshape(4096, B)
+        check(sr, Y, Y_star, den, f"dropin batched {env} B={B}")
 
 
 @pytest.mark.parametrize("B", [1, 3, 4, 64])
