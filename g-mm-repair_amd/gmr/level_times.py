@@ -44,6 +44,7 @@ def main(argv=None):
           f"{int(lo[L])} rules, depth {L}, built in {time.time() - t:.1f} s", flush=True)
     X = torch.randn(args.cols, args.B, device="cuda")
     Y = torch.empty(args.rows, args.B, device="cuda")
+    eng.run_torch(0, X, Y, emit=gmr.EMIT_GATHER)  # set-up (gather lists) outside the timings
     res = {}
     for blk in (int(b) for b in args.blocks.split(",")):
         for name, kind in (("group", gmr.EVAL_GROUP), ("wave", gmr.EVAL_WAVE), ("thread", gmr.EVAL_THREAD)):
